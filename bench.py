@@ -1,0 +1,178 @@
+"""Benchmark: BFS GTEPS on one MI355X, RMAT scale 24 (edge factor 16,
+a/b/c/d = .57/.19/.19/.05, seed 2, int64 offsets / int32 ids), 16 sources.
+
+One step = one BFS from each of the 16 sources (the committed source list of
+tests/golden/rmat24_seed2.json, drawn from the largest component).  The graph
+is generated on the device before the timed region; depth arrays are device
+resident.  value = sum over sources of the TEPS numerator (sum of degrees of
+reached vertices, algorithms/bfs.hpp:88-99) / wall time of the steps.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS on 1 MI355X (BFS/SSSP RMAT s24), % HBM peak; learned+batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
+    """Algorithmic bytes of one BFS level (byte model: 2*w_o per expanded or
+    scanned vertex, 8 B per inspected arc = 4 B id + 4 B status probe, 4 B per
+    discovered vertex, three bitmap passes)."""
+    if ls.direction == 0:
+        return 2 * off_bytes * ls.frontier_vertices + 8 * ls.frontier_edges + 4 * ls.discovered + 3 * w0_bytes
+    return 2 * off_bytes * ls.vertices_scanned + 8 * ls.edges_inspected + 4 * ls.discovered + 3 * w0_bytes
+
+
+def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect):
+    b = sg.BFS(G)
+    edges_total = 0
+    per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0]}  # dir -> [bytes, ms, launches]
+    w0_bytes = ((G.n + 63) // 64) * 8
+    kernel_ms_total = 0.0
+    for _ in range(warmup):
+        for s in sources:
+            b.init(s)
+            b.run(policy, alpha, beta)
+    G.dev.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sources:
+            b.init(s)
+            st = b.run(policy, alpha, beta)
+            edges_total += st.reached_edges
+            kernel_ms_total += st.kernel_ms
+            if collect:
+                for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
+                    ls = st.level[L]
+                    k = per_kernel[ls.direction]
+                    k[0] += level_bytes(ls, w0_bytes)
+                    k[1] += ls.kernel_ms
+                    k[2] += 1
+    G.dev.synchronize()
+    wall = time.perf_counter() - t0
+    return edges_total, wall, kernel_ms_total, per_kernel
+
+
+def cpu_baseline(oracle, G, sources, budget_s: float):
+    off, cols, _ = G.download()
+    g = oracle.CSRGraph(G.n, off, cols)
+    threads = oracle.omp_max_threads()
+    deg = g.degrees()
+    edges = 0
+    t_used = 0.0
+    done = 0
+    for s in sources:
+        t0 = time.perf_counter()
+        d, _ = oracle.bfs_plain_omp(g, s, oracle.BFS_HYBRID, 15.0, 24.0)
+        t_used += time.perf_counter() - t0
+        edges += int(deg[d != g.n + 1].sum())
+        done += 1
+        if t_used > budget_s:
+            break
+    return {"value": edges / t_used / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"{done} of {len(sources)} sources, OpenMP restatement of the reference plain method "
+                      f"(2-level bitmap frontier, hybrid alpha=15 beta=24), {t_used:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--policy", default="hybrid", choices=["push", "pull", "hybrid"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the push-only / pull-only side measurements")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the traversal core runs on a "
+              f"single device; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import sygraph_amd as sg
+
+    dev = sg.Device(0)
+    info = dev.info()
+    log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+    t0 = time.perf_counter()
+    G = sg.Graph.rmat(dev, a.scale, 16, 0.57, 0.19, 0.19, seed=2)
+    log(f"graph: RMAT s{a.scale} n={G.n} m={G.m} generated on device in {time.perf_counter() - t0:.2f}s")
+    golden = os.path.join(ROOT, "tests", "golden", "rmat24_seed2.json")
+    if a.scale == 24 and os.path.exists(golden):
+        with open(golden) as f:
+            sources = json.load(f)["sources"]
+    else:
+        rng = np.random.default_rng(2)
+        off, _, _ = G.download()
+        cand = np.nonzero(np.diff(off) > 0)[0]
+        sources = [int(x) for x in rng.choice(cand, 16, replace=False)]
+    pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[a.policy]
+
+    edges, wall, kms, per_kernel = run_policy(sg, G, sources, pol, 15.0, 24.0, a.steps, a.warmup, True)
+    gteps = edges / wall / 1e9
+    gteps_kernel = edges / (kms / 1e3) / 1e9 if kms > 0 else None
+    log(f"{a.policy}: {gteps:.2f} GTEPS wall, {gteps_kernel:.2f} GTEPS device, {wall / a.steps * 1e3:.2f} ms/step")
+
+    dom = max(per_kernel, key=lambda k: per_kernel[k][1])
+    byts, ms, launches = per_kernel[dom]
+    achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    kernel_names = {0: "push_kernel", 1: "pull_kernel"}
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel_names[dom],
+            "launches": launches, "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
+            "alg_bytes_per_launch": round(byts / max(launches, 1))}
+    for k, name in kernel_names.items():
+        if per_kernel[k][2]:
+            b_, m_, n_ = per_kernel[k]
+            log(f"  {name}: {n_} launches, avg {m_ / n_ * 1e3:.1f} us, {b_ / (m_ / 1e3) / 1e9:.0f} GB/s algorithmic")
+
+    extra = {}
+    if not a.no_extra:
+        for name, p in [("push", sg.POLICY_PUSH), ("pull", sg.POLICY_PULL), ("hybrid", sg.POLICY_HYBRID)]:
+            if name == a.policy:
+                continue
+            e2, w2, k2, _ = run_policy(sg, G, sources, p, 15.0, 24.0, 1, 0, False)
+            extra[f"gteps_{name}"] = round(e2 / w2 / 1e9, 3)
+            log(f"{name}: {e2 / w2 / 1e9:.2f} GTEPS wall")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        import oracle
+
+        log("cpu baseline ...")
+        cpu = cpu_baseline(oracle, G, sources, a.cpu_budget)
+        log(f"cpu: {cpu['value']:.3f} GTEPS on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic (RMAT generated on device)",
+        "config": {"workload": f"BFS {a.policy} (alpha=15, beta=24) from 16 sources, RMAT scale {a.scale} edge factor "
+                               f"16 a/b/c/d=.57/.19/.19/.05 seed 2, int64 offsets, int32 ids",
+                   "vertices": G.n, "arcs": G.m, "sources": len(sources), "policy": a.policy,
+                   "gteps_device_time": round(gteps_kernel, 3) if gteps_kernel else None, **extra},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,283 @@
+/*
+ * sygraph_hip.h -- C ABI of libsygraph_hip, the MI355X (gfx950) runtime and
+ * traversal kernels behind the SYgraph operator API.
+ *
+ * The reference is a header-only C++20/SYCL library with no ABI of its own;
+ * every entry point below replaces a host-side operation of that library that
+ * a C / ctypes / FFI caller would otherwise have to reach through templates.
+ * Each declaration cites the reference interface it stands in for
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every function returns an int status: SG_OK (0) or a negative SG_ERR_*.
+ *    sg_last_error() returns a thread-local message for the last failure.
+ *    No exception crosses this boundary (reference: std::runtime_error throws,
+ *    e.g. operators/advance/advance.hpp:63, algorithms/bfs.hpp:155).
+ *  - Vertex ids are 32-bit; edge offsets are 32- or 64-bit (chosen per graph);
+ *    edge weights are 4-byte (uint32 or float32).
+ *  - All work of one device handle is ordered on that handle's HIP stream.
+ *    Calls return after completion unless their name ends in _async.
+ *  - Pointers documented as "device" must come from sg_alloc (or another HIP
+ *    allocation on the same device); "host" pointers are ordinary memory.
+ */
+#ifndef SYGRAPH_HIP_H
+#define SYGRAPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_ABI_VERSION 1
+
+enum sg_status {
+  SG_OK = 0,
+  SG_ERR_INVALID = -1,     /* bad argument */
+  SG_ERR_HIP = -2,         /* HIP runtime error */
+  SG_ERR_NOMEM = -3,       /* allocation failed */
+  SG_ERR_UNSUPPORTED = -4, /* valid but not implemented combination */
+  SG_ERR_NODEVICE = -5,    /* no HIP device present */
+  SG_ERR_STATE = -6        /* object not initialised / wrong state */
+};
+
+typedef struct sg_device_s* sg_device_t;
+typedef struct sg_graph_s* sg_graph_t;
+typedef struct sg_frontier_s* sg_frontier_t;
+typedef struct sg_event_s* sg_event_t;
+
+/* ------------------------------------------------------------------ errors */
+/* Thread-local text of the last error (reference: the what() of the
+ * std::runtime_error its host code throws). Never NULL. */
+const char* sg_last_error(void);
+int sg_abi_version(void);
+
+/* ------------------------------------------------------------------ device */
+typedef struct sg_device_info {
+  char name[256];          /* e.g. "AMD Instinct MI355X" */
+  char arch[64];           /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+  int32_t compute_units;   /* utils/device.hpp:31-34 getNumComputeUnits */
+  int32_t wave_size;       /* utils/device.hpp:21-24 getSubgroupSize */
+  int32_t max_workgroup;   /* utils/device.hpp:26-29 getWorkgroupSize */
+  int32_t max_waves_per_wg;/* utils/device.hpp:16-19 getMaxNumSubgroups */
+  int64_t lds_bytes;       /* LDS per workgroup */
+  int64_t global_mem_bytes;
+  int64_t l2_bytes;
+  int32_t clock_khz;
+  int32_t memory_clock_khz;
+  int32_t memory_bus_bits;
+  int32_t device_ordinal;
+} sg_device_info;
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int sg_device_count(int* count);
+/* Open device `ordinal` with its own stream. Replaces constructing a
+ * sycl::queue{sycl::gpu_selector_v} (examples/bfs/bfs.cpp:120-124,
+ * tests/include/test_utils.hpp:49-61). */
+int sg_device_open(int ordinal, sg_device_t* out);
+int sg_device_close(sg_device_t dev);
+/* Device queries (utils/device.hpp:16-34). */
+int sg_device_info_get(sg_device_t dev, sg_device_info* info);
+/* The handle's hipStream_t, for header templates that launch functor kernels
+ * on the same stream (the queue a GraphCSR carries, graph_csr.hpp:391). */
+int sg_device_stream(sg_device_t dev, void** stream);
+/* queue.wait() (used throughout, e.g. mlb_frontier.hpp:170). */
+int sg_device_synchronize(sg_device_t dev);
+
+/* ------------------------------------------------------------------ memory */
+enum sg_space { SG_SPACE_HOST = 0, SG_SPACE_DEVICE = 1, SG_SPACE_SHARED = 2 };
+/* utils/memory.hpp:12-39 memoryAlloc<T, space> (host = pinned, device,
+ * shared = managed). */
+int sg_alloc(sg_device_t dev, size_t bytes, int space, void** out);
+/* utils/memory.hpp:41-46 releaseUSM. NULL is a no-op. */
+int sg_free(sg_device_t dev, void* ptr);
+/* queue.copy (e.g. graph_csr.hpp:401-403); direction inferred by HIP. */
+int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes);
+/* queue.fill for 1/4/8-byte patterns (e.g. algorithms/bfs.hpp:62-66). */
+int sg_fill(sg_device_t dev, void* dst, uint64_t pattern, int pattern_bytes, size_t count);
+
+/* ------------------------------------------------------------------ events */
+/* sygraph::Event::getRuntime (sycl/event.hpp:36-41) as start/stop hipEvents. */
+int sg_event_create(sg_device_t dev, sg_event_t* out);
+int sg_event_destroy(sg_event_t ev);
+int sg_event_record(sg_event_t ev);
+int sg_event_synchronize(sg_event_t ev);
+int sg_event_elapsed_ms(sg_event_t start, sg_event_t stop, float* ms);
+
+/* ------------------------------------------------------------------- graph */
+enum sg_graph_flags {
+  SG_GRAPH_DIRECTED = 1,       /* graph/properties.hpp:29-32 Properties::directed */
+  SG_GRAPH_WEIGHTED = 2,       /* Properties::weighted */
+  SG_GRAPH_INPUT_DEVICE = 4,   /* input arrays are device pointers */
+  SG_GRAPH_WEIGHT_FLOAT = 8,   /* weights are float32 (else uint32) */
+  SG_GRAPH_SPACE_SHARED = 16,  /* store in managed memory (GRAPH_LOCATION=2) */
+  SG_GRAPH_SPACE_HOST = 32     /* store in pinned host memory (GRAPH_LOCATION=0) */
+};
+/* Upload a CSR graph (graph/impls/graph_csr.hpp:170-178,394-426 +
+ * graph/build.hpp:31-41).  offsets has n+1 entries of offset_bytes (4 or 8),
+ * cols has m int32 ids, weights has m 4-byte values or is NULL.  For a
+ * directed graph the inverse CSR is built on the device (counting sort
+ * transpose); an undirected graph aliases it, as graph_csr.hpp:423-425. */
+int sg_graph_create(sg_device_t dev, int64_t n, int64_t m, const void* offsets, int offset_bytes, const int32_t* cols,
+                    const void* weights, int flags, sg_graph_t* out);
+int sg_graph_destroy(sg_graph_t g);
+
+typedef struct sg_graph_view {
+  int64_t n, m;
+  int32_t offset_bytes;
+  int32_t flags;
+  void* offsets;          /* device, n+1 entries */
+  int32_t* cols;          /* device, m entries */
+  void* weights;          /* device, m entries or NULL */
+  void* inv_offsets;      /* inverse CSR (== offsets when undirected) */
+  int32_t* inv_cols;
+  void* inv_weights;
+} sg_graph_view;
+/* GraphCSR::getDeviceGraph / getInverseDeviceGraph (graph_csr.hpp:206-208). */
+int sg_graph_get_view(sg_graph_t g, sg_graph_view* view);
+
+/* Synthetic graphs, generated and assembled on the device (symmetrise, drop
+ * self loops, sort, deduplicate; int64 offsets).  No reference counterpart:
+ * the reference reads graphs from files (io/read_csr.hpp); the bench and the
+ * tests need the seeded generators described in the benchmark configuration.
+ *  rmat: 2^scale vertices, edge_factor*2^scale tuples, quadrant probabilities
+ *        a,b,c (d=1-a-b-c), optional seeded id relabel;
+ *  grid: rows x cols 4-neighbour lattice, each edge deleted with p_delete;
+ *  er:   G(n, m_pairs) with uniform endpoints.
+ * weight_seed != 0 attaches uint32 weights in [1,255], one per undirected edge. */
+int sg_graph_generate_rmat(sg_device_t dev, int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                           int relabel, uint64_t weight_seed, sg_graph_t* out);
+int sg_graph_generate_grid(sg_device_t dev, int rows, int cols, double p_delete, uint64_t seed, uint64_t weight_seed,
+                           sg_graph_t* out);
+int sg_graph_generate_er(sg_device_t dev, int64_t n, int64_t m_pairs, uint64_t seed, uint64_t weight_seed,
+                         sg_graph_t* out);
+/* Copy the CSR back to host buffers (any may be NULL). offsets get n+1 int64. */
+int sg_graph_download(sg_graph_t g, int64_t* offsets, int32_t* cols, uint32_t* weights);
+/* Order-sensitive digests sum_i mix64(i*K ^ x_i) of offsets and cols, computed
+ * on the device (used to pin generated graphs without moving them). */
+int sg_graph_digest(sg_graph_t g, uint64_t* offsets_digest, uint64_t* cols_digest);
+/* Sum of degrees of vertices whose depth != n+1 (algorithms/bfs.hpp:88-99,
+ * the TEPS numerator). depth is a device int32 array. */
+int sg_graph_reached_edges(sg_graph_t g, const int32_t* depth, int64_t* edges, int64_t* vertices);
+
+/* ---------------------------------------------------------------- frontier */
+/* FrontierMLB<T, 2> (frontier/impls/mlb_frontier.hpp:155-449): level 0 holds
+ * one bit per element in 64-bit words, level 1 one bit per non-empty level-0
+ * word.  Elements are < n. */
+int sg_frontier_create(sg_device_t dev, int64_t n, sg_frontier_t* out);
+int sg_frontier_destroy(sg_frontier_t f);
+typedef struct sg_frontier_view {
+  int64_t n;
+  int64_t words0, words1;
+  uint64_t* l0;        /* device */
+  uint64_t* l1;        /* device */
+  int32_t* list;       /* device compaction output (capacity n) */
+  uint32_t* list_size; /* device counter written by compaction */
+} sg_frontier_view;
+/* MLBDevice accessors (mlb_frontier.hpp:58-153). */
+int sg_frontier_get_view(sg_frontier_t f, sg_frontier_view* view);
+/* check / insert / remove of one element (mlb_frontier.hpp:248-285). */
+int sg_frontier_check(sg_frontier_t f, int64_t idx, int* present);
+int sg_frontier_insert(sg_frontier_t f, int64_t idx);
+int sg_frontier_remove(sg_frontier_t f, int64_t idx);
+/* clear (mlb_frontier.hpp:367-380). */
+int sg_frontier_clear(sg_frontier_t f);
+int sg_frontier_clear_async(sg_frontier_t f);
+/* empty / size (mlb_frontier.hpp:216-246, 287-311). */
+int sg_frontier_empty(sg_frontier_t f, int* empty);
+int sg_frontier_size(sg_frontier_t f, int64_t* size);
+/* this |= other / this &= other; both rebuild the level-1 summary
+ * (mlb_frontier.hpp:313-337; the reference's merge leaves level 1 stale). */
+int sg_frontier_merge(sg_frontier_t f, sg_frontier_t other);
+int sg_frontier_intersect(sg_frontier_t f, sg_frontier_t other);
+/* out = a & b in one pass (operators/intersection/bitmap_intersect_impl.hpp:21-30). */
+int sg_frontier_and(sg_frontier_t out, sg_frontier_t a, sg_frontier_t b);
+/* computeActiveFrontier (mlb_frontier.hpp:388-442) as an ordered
+ * set-scan-compact.  mode 0: element ids of set bits (ascending) into
+ * view.list; mode 1: level-0 word ids that contain a set bit; mode 2 (pull):
+ * level-0 word ids that are not all ones.  *count may be NULL (device count
+ * stays in view.list_size). */
+int sg_frontier_compact(sg_frontier_t f, int mode, int64_t* count);
+int sg_frontier_compact_async(sg_frontier_t f, int mode);
+/* saveState / loadState: copy both levels to / from host words
+ * (mlb_frontier.hpp:339-365). Buffers hold words0 + words1 uint64. */
+int sg_frontier_save(sg_frontier_t f, uint64_t* host_words);
+int sg_frontier_load(sg_frontier_t f, const uint64_t* host_words);
+/* Swap the storage of two frontiers of equal size (frontier.hpp:92-99). */
+int sg_frontier_swap(sg_frontier_t a, sg_frontier_t b);
+
+/* -------------------------------------------------------------- algorithms */
+enum sg_direction_policy {
+  SG_POLICY_PUSH = 0,    /* bfs_direction::push */
+  SG_POLICY_PULL = 1,    /* bfs_direction::pull */
+  SG_POLICY_HYBRID = 2,  /* bfs_direction::hybrid, reference alpha/beta rule */
+  SG_POLICY_LEARNED = 3  /* per-level MLP choice of direction and balancer */
+};
+enum sg_load_balancer {
+  SG_LB_WORKITEM = 0,    /* load_balancer::workitem_mapped (thread per vertex) */
+  SG_LB_SUBGROUP = 1,    /* load_balancer::subgroup_mapped (wave per vertex) */
+  SG_LB_WORKGROUP = 2,   /* load_balancer::workgroup_mapped (edge-balanced merge path) */
+  SG_LB_BUCKETING = 3    /* load_balancer::bucketing (thread/wave/workgroup bins) */
+};
+
+#define SG_MAX_LEVELS_RECORDED 4096
+typedef struct sg_bfs_options {
+  int32_t policy;          /* sg_direction_policy */
+  int32_t load_balancer;   /* sg_load_balancer for push levels */
+  float alpha, beta;       /* hybrid thresholds (algorithms/bfs.hpp:153) */
+  int32_t record_levels;   /* fill per-level stats in sg_bfs_stats */
+  int32_t sync_per_level;  /* 1: host reads stats every level (default 0) */
+} sg_bfs_options;
+
+typedef struct sg_level_stat {
+  int32_t direction;       /* 0 push, 1 pull */
+  int32_t load_balancer;
+  int64_t frontier_vertices; /* |F| entering the level */
+  int64_t frontier_edges;    /* sum deg over F */
+  int64_t discovered;        /* vertices discovered at this level */
+  int64_t edges_inspected;   /* arcs read (push: sum deg F; pull: sum k_u) */
+  int64_t vertices_scanned;  /* pull: unvisited vertices with degree > 0 */
+  int64_t unexplored_edges;  /* sum deg over unvisited after the level */
+  int64_t max_degree;        /* largest degree among the discovered vertices */
+  float kernel_ms;           /* device time of the level's advance kernel (record_levels) */
+  float convert_ms;          /* device time of a queue<->bitmap conversion after it */
+} sg_level_stat;
+
+typedef struct sg_bfs_stats {
+  int32_t levels;          /* BFSRunDetails::iterations (algorithms/bfs.hpp:31-35) */
+  int32_t n_push, n_pull;
+  int64_t reached_vertices;
+  int64_t reached_edges;   /* TEPS numerator */
+  float kernel_ms;         /* device time of the traversal (hipEvents) */
+  sg_level_stat level[SG_MAX_LEVELS_RECORDED];
+} sg_bfs_stats;
+
+void sg_bfs_options_default(sg_bfs_options* opt);
+/* BFS::run (algorithms/bfs.hpp:153-238): depth (device int32[n]) receives
+ * hop counts, unreached = n+1 (bfs.hpp:62-67).  stats may be NULL. */
+int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt, int32_t* depth, sg_bfs_stats* stats);
+
+/* SSSP::run (algorithms/sssp.hpp:152-210): frontier Bellman-Ford with atomic
+ * min; dist (device, 4-byte, same type as the weights) unreached = n+1. */
+int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, int32_t* rounds);
+/* CC::run (algorithms/cc.hpp:103-169): labels (device uint32[n]) converge to
+ * the largest vertex id of each component. */
+int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);
+/* TC::run + getNumTriangles (algorithms/tc.hpp:67-138): exact count on the
+ * (degree, id)-oriented DAG; adjacency lists must be sorted. per_vertex
+ * (device uint64[n] or NULL) gets the triangles attributed to each DAG source. */
+int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex);
+/* BC::run (algorithms/bc.hpp:118-200) for many sources: bc (device
+ * float[n]) += sum_s delta_s(v), v != s, no /2.  Sources run 64 per batch,
+ * one bit-lane each; sigma is FP32 with per-(lane, level) power-of-two
+ * scaling. */
+int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
+/* Multi-source BFS, 64 sources per batch packed per bit-lane: depth is device
+ * int32[nsrc][n] (row s = source s), unreached = n+1. */
+int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYGRAPH_HIP_H */

@@ -1,0 +1,110 @@
+/*
+ * internal.hpp -- private definitions of libsygraph_hip (not installed).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "sygraph_hip.h"
+#include "sygraph/detail/hip_device.hpp"
+
+namespace sg {
+
+void set_error(const char* fmt, ...);
+
+/* Evaluate a HIP call; on failure record the message and return SG_ERR_HIP. */
+#define SG_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      ::sg::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return SG_ERR_HIP;                                                                       \
+    }                                                                                          \
+  } while (0)
+
+#define SG_CHECK(cond, code, ...)  \
+  do {                             \
+    if (!(cond)) {                 \
+      ::sg::set_error(__VA_ARGS__); \
+      return (code);               \
+    }                              \
+  } while (0)
+
+#define SG_TRY(expr)          \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != SG_OK) return s_; \
+  } while (0)
+
+} // namespace sg
+
+struct sg_device_s {
+  int ordinal = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+  int cus = 0;
+  /* small pinned host scratch for stat read-backs */
+  void* host_scratch = nullptr;
+  size_t host_scratch_bytes = 0;
+};
+
+struct sg_event_s {
+  sg_device_t dev = nullptr;
+  hipEvent_t ev = nullptr;
+};
+
+struct sg_graph_s {
+  sg_device_t dev = nullptr;
+  int64_t n = 0, m = 0;
+  int offset_bytes = 8;
+  int flags = 0;
+  void* offsets = nullptr;
+  int32_t* cols = nullptr;
+  void* weights = nullptr;
+  void* inv_offsets = nullptr;
+  int32_t* inv_cols = nullptr;
+  void* inv_weights = nullptr;
+  bool owns_inverse = false;
+  /* lazily built helpers */
+  uint64_t* isolated = nullptr; /* bit set for vertices with in-degree 0 */
+  int64_t total_degree = 0;
+  void* bfs_ws = nullptr;       /* traversal workspace, reused across calls */
+};
+
+struct sg_frontier_s {
+  sg_device_t dev = nullptr;
+  int64_t n = 0, w0 = 0, w1 = 0;
+  uint64_t* l0 = nullptr;
+  uint64_t* l1 = nullptr;
+  int32_t* list = nullptr;
+  uint32_t* list_size = nullptr;
+  uint32_t* tile_counts = nullptr; /* compaction scratch */
+  int64_t ntiles = 0;
+};
+
+namespace sg {
+
+inline int grid_for(sg_device_t dev, int64_t items, int block, int max_blocks_per_cu = 8) {
+  int64_t want = (items + block - 1) / block;
+  int64_t cap = static_cast<int64_t>(dev->cus) * max_blocks_per_cu;
+  if (want < 1) want = 1;
+  return static_cast<int>(want < cap ? want : cap);
+}
+
+int alloc_device(sg_device_t dev, void** p, size_t bytes);
+int zero_async(sg_device_t dev, void* p, size_t bytes);
+
+/* graph helpers */
+int graph_ensure_isolated(sg_graph_t g);
+void bfs_workspace_release(sg_graph_t g);
+
+/* frontier compaction used by other modules */
+int frontier_compact_launch(sg_frontier_t f, int mode);
+
+} // namespace sg

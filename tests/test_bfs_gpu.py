@@ -1,0 +1,206 @@
+"""BFS on the MI355X through the C ABI, bit-exact against the oracle.
+
+Covers the reference's BFS known answers (tests/algorithms/bfs.cpp,
+tests/operators/advance.cpp), every direction policy, directed graphs (pull
+over the inverse CSR), isolated sources, high-diameter grids, the hybrid
+direction schedule (algorithms/bfs.hpp:325-350) and, at full size, RMAT
+scale 24 through committed digests.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+POLICIES = ["push", "pull", "hybrid"]
+
+
+def _cases():
+    with open(os.path.join(GOLDEN, "reference_cases.json")) as f:
+        return json.load(f)
+
+
+def _upload(sg, device, g, directed=False):
+    return sg.Graph.from_csr(device, g.offsets, g.cols, directed=directed)
+
+
+def _run(sg, G, src, policy, alpha=15.0, beta=24.0):
+    b = sg.BFS(G)
+    b.init(src)
+    pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[policy]
+    st = b.run(pol, alpha, beta)
+    return b.get_distances(), st, b.directions()
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_line5_known_answer(sg, device, oracle_mod, policy):
+    c = _cases()
+    g = oracle_mod.from_dense(oracle_mod.parse_matrix_text(c["matrices"]["line_5"]))
+    G = _upload(sg, device, g)
+    d, st, dirs = _run(sg, G, 0, policy, 1.0, 1.0)
+    assert d.tolist() == c["expect"]["bfs_line_5"]
+    assert st.levels == c["expect"]["bfs_line_5_iterations"]
+    if policy == "push":
+        assert set(dirs) == {"push"}
+    if policy == "pull":
+        assert set(dirs) == {"pull"}
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_symmetric6_known_answer(sg, device, oracle_mod, policy):
+    c = _cases()
+    g = oracle_mod.from_dense(oracle_mod.parse_matrix_text(c["matrices"]["symmetric_6nodes"]))
+    d, _, _ = _run(sg, _upload(sg, device, g), 0, policy)
+    assert d.tolist() == c["expect"]["advance_bfs_symmetric_6nodes"]
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_isolated_source(sg, device, oracle_mod, policy):
+    c = _cases()
+    g = oracle_mod.from_dense(oracle_mod.parse_matrix_text(c["matrices"]["two_cc"]))
+    d, st, _ = _run(sg, _upload(sg, device, g), 5, policy)
+    assert d.tolist() == [7, 7, 7, 7, 7, 0] and st.levels == 1
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_directed_graph(sg, device, oracle_mod, policy):
+    c = _cases()
+    g = oracle_mod.from_dense(oracle_mod.parse_matrix_text(c["matrices"]["weighted_directed_5"]))
+    G = _upload(sg, device, g, directed=True)
+    for src in range(5):
+        ref, lv = oracle_mod.bfs(g, src)
+        d, st, _ = _run(sg, G, src, policy)
+        assert np.array_equal(d, ref) and st.levels == lv
+
+
+@pytest.fixture(scope="module")
+def rmat14(sg, device, oracle_mod):
+    z = np.load(os.path.join(GOLDEN, "rmat14_seed1.npz"))
+    G = sg.Graph.rmat(device, 14, 16, 0.57, 0.19, 0.19, seed=1)
+    return G, z
+
+
+def test_rmat14_generator_matches_oracle(sg, rmat14):
+    G, z = rmat14
+    assert G.n == int(z["n"]) and G.m == int(z["m"])
+    assert list(G.digest()) == [int(x) for x in z["digest"]]
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_rmat14_golden_depths(sg, oracle_mod, rmat14, policy):
+    G, z = rmat14
+    d, st, _ = _run(sg, G, int(z["source"]), policy)
+    assert np.array_equal(d, z["depth"])
+    assert st.levels == int(z["levels"])
+
+
+def test_rmat14_hybrid_schedule_matches_reference_rule(sg, oracle_mod, rmat14):
+    G, z = rmat14
+    off, cols, _ = G.download()
+    g = oracle_mod.CSRGraph(G.n, off, cols)
+    for src in [int(z["source"]), 1, 777, 9000]:
+        for alpha, beta in [(15.0, 24.0), (1.0, 1.0), (4.0, 100.0)]:
+            ref, lv, ref_dirs = oracle_mod.bfs_plain_omp(g, src, oracle_mod.BFS_HYBRID, alpha, beta, with_dirs=True)
+            d, st, dirs = _run(sg, G, src, "hybrid", alpha, beta)
+            assert np.array_equal(d, ref)
+            assert dirs == ["push" if x == 0 else "pull" for x in ref_dirs]
+
+
+def test_rmat14_level_counters_match_oracle(sg, oracle_mod, rmat14):
+    """Per-level discovered counts and the pull scan cost sum k_u (the byte model)."""
+    G, z = rmat14
+    off, cols, _ = G.download()
+    g = oracle_mod.CSRGraph(G.n, off, cols)
+    src = int(z["source"])
+    ref = z["depth"]
+    for policy in ["push", "pull"]:
+        _, st, _ = _run(sg, G, src, policy)
+        for L in range(st.levels):
+            ls = st.level[L]
+            assert ls.discovered == int((ref == L + 1).sum())
+            if policy == "pull":
+                tot, cnt = oracle_mod.pull_scan_count(g, ref, L)
+                assert (ls.edges_inspected, ls.vertices_scanned) == (tot, cnt)
+            else:
+                fr = ref == L
+                assert ls.frontier_vertices == int(fr.sum())
+                assert ls.edges_inspected == int(g.degrees()[fr].sum())
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_random_sources_and_shapes(sg, device, oracle_mod, policy):
+    graphs = [
+        oracle_mod.rmat(12, 8, seed=21),
+        oracle_mod.rmat(11, 32, a=0.45, b=0.22, c=0.22, seed=22),
+        oracle_mod.grid(64, 80, 0.15, seed=23),
+        oracle_mod.erdos_renyi(5000, 9000, seed=24),
+    ]
+    rng = np.random.default_rng(5)
+    for g in graphs:
+        G = _upload(sg, device, g)
+        for src in rng.integers(0, g.n, size=4):
+            ref, lv = oracle_mod.bfs(g, int(src))
+            d, st, _ = _run(sg, G, int(src), policy)
+            assert np.array_equal(d, ref), (g.n, int(src))
+            assert st.levels == lv
+
+
+def test_generators_match_oracle(sg, device, oracle_mod):
+    cases = [
+        (lambda: sg.Graph.grid(device, 100, 73, 0.1, seed=4), lambda: oracle_mod.grid(100, 73, 0.1, seed=4)),
+        (lambda: sg.Graph.erdos_renyi(device, 7000, 30000, seed=9), lambda: oracle_mod.erdos_renyi(7000, 30000, 9)),
+        (lambda: sg.Graph.rmat(device, 13, 8, 0.45, 0.25, 0.2, seed=3, relabel=False),
+         lambda: oracle_mod.rmat(13, 8, 0.45, 0.25, 0.2, seed=3, relabel=False)),
+    ]
+    for mk_gpu, mk_cpu in cases:
+        G, g = mk_gpu(), mk_cpu()
+        assert (G.n, G.m) == (g.n, g.m)
+        assert list(G.digest()) == list(g.digest())
+
+
+def test_weights_match_oracle(sg, device, oracle_mod):
+    G = sg.Graph.rmat(device, 12, 8, seed=5, weight_seed=3)
+    off, cols, w = G.download(weights=True)
+    g = oracle_mod.CSRGraph(G.n, off, cols)
+    assert np.array_equal(w, oracle_mod.fill_weights(g, 3))
+
+
+def test_repeatable_and_reusable(sg, oracle_mod, rmat14):
+    G, z = rmat14
+    b = sg.BFS(G)
+    for _ in range(3):
+        b.init(int(z["source"]))
+        b.run(sg.POLICY_HYBRID, 15.0, 24.0)
+        assert np.array_equal(b.get_distances(), z["depth"])
+
+
+def test_invalid_source_raises(sg, rmat14):
+    G, _ = rmat14
+    b = sg.BFS(G)
+    b.init(G.n + 5)
+    with pytest.raises(sg.SygraphError):
+        b.run()
+
+
+def test_rmat24_full_size(sg, device, oracle_mod):
+    """RMAT scale 24 at full size: generator digest, then BFS from the 16 committed
+    sources under every policy against the committed per-source depth digests."""
+    with open(os.path.join(GOLDEN, "rmat24_seed2.json")) as f:
+        ref = json.load(f)
+    G = sg.Graph.rmat(device, 24, 16, 0.57, 0.19, 0.19, seed=2)
+    assert (G.n, G.m) == (ref["n"], ref["m"])
+    assert [str(x) for x in G.digest()] == ref["digest"]
+    b = sg.BFS(G)
+    for rec in ref["bfs"]:
+        for policy in POLICIES:
+            b.init(rec["source"])
+            pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[policy]
+            st = b.run(pol, 15.0, 24.0)
+            d = b.get_distances()
+            assert str(oracle_mod.digest_i32(d)) == rec["depth_digest"], (rec["source"], policy)
+            assert st.levels == rec["levels"]
+            e, v = G.reached_edges(b.depth)
+            assert (e, v) == (rec["reached_edges"], rec["reached_vertices"])
