@@ -258,9 +258,19 @@ void sg_bfs_options_default(sg_bfs_options* opt);
  * hop counts, unreached = n+1 (bfs.hpp:62-67).  stats may be NULL. */
 int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt, int32_t* depth, sg_bfs_stats* stats);
 
+typedef struct sg_sssp_stats {
+  int32_t rounds;            /* frontier rounds until no distance changed */
+  int64_t reached_vertices;  /* dist != n+1 */
+  int64_t reached_edges;     /* TEPS numerator (algorithms/sssp.hpp:66-77) */
+  int64_t relaxations;       /* arcs relaxed over all rounds */
+  float kernel_ms;           /* device time of the whole run */
+  int64_t round_vertices[SG_MAX_LEVELS_RECORDED]; /* |F| per round */
+  int64_t round_edges[SG_MAX_LEVELS_RECORDED];    /* sum deg over F per round */
+} sg_sssp_stats;
 /* SSSP::run (algorithms/sssp.hpp:152-210): frontier Bellman-Ford with atomic
- * min; dist (device, 4-byte, same type as the weights) unreached = n+1. */
-int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, int32_t* rounds);
+ * min; dist (device, 4-byte, same type as the weights) unreached = n+1.
+ * stats may be NULL. */
+int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats);
 /* CC::run (algorithms/cc.hpp:103-169): labels (device uint32[n]) converge to
  * the largest vertex id of each component. */
 int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);

@@ -15,6 +15,7 @@ raises.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import os
 from typing import Optional, Sequence
 
@@ -76,6 +77,12 @@ class LevelStat(C.Structure):
                 ("kernel_ms", C.c_float), ("convert_ms", C.c_float)]
 
 
+class SsspStats(C.Structure):
+    _fields_ = [("rounds", C.c_int32), ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
+                ("relaxations", C.c_int64), ("kernel_ms", C.c_float),
+                ("round_vertices", C.c_int64 * MAX_LEVELS_RECORDED), ("round_edges", C.c_int64 * MAX_LEVELS_RECORDED)]
+
+
 class BfsStats(C.Structure):
     _fields_ = [("levels", C.c_int32), ("n_push", C.c_int32), ("n_pull", C.c_int32),
                 ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64), ("kernel_ms", C.c_float),
@@ -135,7 +142,7 @@ _SIGS = {
     "sg_frontier_swap": (_I, [_P, _P]),
     "sg_bfs_options_default": (None, [C.POINTER(BfsOptions)]),
     "sg_bfs": (_I, [_P, C.c_int32, C.POINTER(BfsOptions), _P, C.POINTER(BfsStats)]),
-    "sg_sssp": (_I, [_P, C.c_int32, C.c_int32, _P, C.POINTER(C.c_int32)]),
+    "sg_sssp": (_I, [_P, C.c_int32, C.c_int32, _P, C.POINTER(SsspStats)]),
     "sg_cc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
     "sg_tc": (_I, [_P, C.POINTER(C.c_uint64), _P]),
     "sg_bc": (_I, [_P, C.c_int32, _P, _P]),
@@ -180,6 +187,12 @@ def device_count() -> int:
     return c.value
 
 
+
+def _finalizing() -> bool:
+    """True while the interpreter shuts down: handles are then left to process
+    exit, since module globals (and the library handle) may already be gone."""
+    return sys is None or sys.is_finalizing()
+
 # ------------------------------------------------------------------ device --
 class Device:
     """A HIP device plus its stream (replaces ``sycl::queue``)."""
@@ -195,6 +208,8 @@ class Device:
             self.h = None
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -238,6 +253,8 @@ class DeviceBuffer:
             self.ptr = None
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.free()
         except Exception:
@@ -274,6 +291,8 @@ class Event:
         return ms.value
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             lib().sg_event_destroy(self.h)
         except Exception:
@@ -359,6 +378,8 @@ class Graph:
             self.h = None
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.destroy()
         except Exception:
@@ -447,6 +468,8 @@ class Frontier:
             self.h = None
 
     def __del__(self):
+        if _finalizing():
+            return
         try:
             self.destroy()
         except Exception:
@@ -503,3 +526,90 @@ class BFS:
 
     def directions(self) -> list[str]:
         return ["push" if self.stats.level[i].direction == 0 else "pull" for i in range(self.stats.levels)]
+
+
+class SSSP:
+    """algorithms::SSSP (algorithms/sssp.hpp:103-222): init(source), run(), getDistance(v).
+    Distances have the weight type (uint32 or float32); unreached = n+1."""
+
+    def __init__(self, g: Graph):
+        self.g = g
+        v = g.view()
+        if not v.weights:
+            raise SygraphError(SG_ERR_STATE, "graph has no edge weights")
+        self.dtype = np.float32 if (v.flags & GRAPH_WEIGHT_FLOAT) else np.uint32
+        self.dist: Optional[DeviceBuffer] = None
+        self.source: Optional[int] = None
+        self.stats = SsspStats()
+
+    def init(self, source: int):
+        self.source = int(source)
+        if self.dist is None:
+            self.dist = DeviceBuffer(self.g.dev, 4 * max(self.g.n, 1), dtype=self.dtype, shape=(self.g.n,))
+
+    def reset(self):
+        self.source = None
+
+    def run(self, load_balancer: int = LB_BUCKETING) -> SsspStats:
+        if self.source is None:
+            raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
+        check(lib().sg_sssp(self.g.h, self.source, load_balancer, C.c_void_p(self.dist.ptr), C.byref(self.stats)))
+        return self.stats
+
+    def get_distances(self) -> np.ndarray:
+        return self.dist.download(self.dtype, self.g.n)
+
+    def getDistance(self, v: int):
+        return self.get_distances()[v]
+
+
+class CC:
+    """algorithms::CC (algorithms/cc.hpp:74-192): run() leaves every vertex with the largest
+    vertex id of its component (undirected) / the largest id that reaches it (directed)."""
+
+    def __init__(self, g: Graph):
+        self.g = g
+        self.labels: Optional[DeviceBuffer] = None
+        self.rounds = 0
+
+    def init(self, source: int = 0):
+        if self.labels is None:
+            self.labels = DeviceBuffer(self.g.dev, 4 * max(self.g.n, 1), dtype=np.uint32, shape=(self.g.n,))
+
+    def run(self) -> int:
+        if self.labels is None:
+            raise SygraphError(SG_ERR_STATE, "CC instance not initialized")
+        r = C.c_int32()
+        check(lib().sg_cc(self.g.h, C.c_void_p(self.labels.ptr), C.byref(r)))
+        self.rounds = r.value
+        return self.rounds
+
+    def get_labels(self) -> np.ndarray:
+        return self.labels.download(np.uint32, self.g.n)
+
+
+class TC:
+    """algorithms::TC (algorithms/tc.hpp:57-143): exact uint64 triangle count."""
+
+    def __init__(self, g: Graph):
+        self.g = g
+        self.count: Optional[int] = None
+        self.per_vertex: Optional[DeviceBuffer] = None
+
+    def init(self):
+        self.count = None
+
+    def run(self, per_vertex: bool = False) -> int:
+        c = C.c_uint64()
+        pv = None
+        if per_vertex:
+            self.per_vertex = DeviceBuffer(self.g.dev, 8 * max(self.g.n, 1), dtype=np.uint64, shape=(self.g.n,))
+            pv = C.c_void_p(self.per_vertex.ptr)
+        check(lib().sg_tc(self.g.h, C.byref(c), pv))
+        self.count = int(c.value)
+        return self.count
+
+    def getNumTriangles(self) -> int:
+        if self.count is None:
+            raise SygraphError(SG_ERR_STATE, "TC instance not initialized")
+        return self.count

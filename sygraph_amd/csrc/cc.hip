@@ -1,0 +1,164 @@
+/*
+ * cc.hip -- connected components for gfx950 (CC::run, algorithms/cc.hpp:103-169).
+ *
+ * The reference propagates the larger label along arcs until nothing changes,
+ * so on an undirected graph every vertex ends with the largest vertex id of
+ * its component.  That fixpoint is reached here without per-level rounds:
+ *  - undirected graphs: concurrent union-find.  Every arc (u,v), u < v, hooks
+ *    the smaller of the two roots under the larger with a CAS, so parent[x] >= x
+ *    always holds and the surviving root of a component is its largest id
+ *    (the largest vertex can never be hooked under a larger root).  A final
+ *    pass points every vertex at its root.  Work is O(m) regardless of the
+ *    graph's diameter, which matters on the 4096x4096 grid.
+ *  - directed graphs: label propagation along out-arcs with atomic max
+ *    (labels[v] = largest id that reaches v), frontier by frontier, which is
+ *    exactly the reference's fixpoint on a directed graph.
+ * Arcs are visited through the degree-class queues of the whole vertex set
+ * (thread / wave / workgroup bins), so hubs are split across the grid.
+ */
+#include <algorithm>
+#include <cstring>
+
+#include "traverse.hpp"
+
+using namespace sg::trav;
+
+namespace {
+
+__device__ __forceinline__ uint32_t find_root(uint32_t* parent, uint32_t x) {
+  uint32_t p = atomic_load(&parent[x]);
+  while (p != x) {
+    const uint32_t gp = atomic_load(&parent[p]);
+    if (gp != p) atomic_store(&parent[x], gp); /* path halving; parents only grow */
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void hook(uint32_t* parent, uint32_t u, uint32_t v) {
+  uint32_t ru = find_root(parent, u), rv = find_root(parent, v);
+  while (ru != rv) {
+    const uint32_t lo = ru < rv ? ru : rv, hi = ru < rv ? rv : ru;
+    uint32_t expected = lo;
+    if (atomic_cas(&parent[lo], expected, hi)) return;
+    ru = find_root(parent, expected);
+    rv = find_root(parent, hi);
+  }
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) hook_kernel(QEntry* q0, QEntry* q1, QEntry* q2, const QueueHdr* hdr,
+                                                      const int32_t* cols, uint32_t* parent) {
+  QEntry* const q[3] = {q0, q1, q2};
+  for_each_queued_arc<OffT>(q, hdr, cols, [&](uint32_t u, uint32_t v, int64_t) {
+    if (u < v) hook(parent, u, v);
+  });
+}
+
+__global__ void iota_kernel(uint32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+/* After hooking every tree is final; each thread follows its own chain
+ * without path halving, because a halving write by another thread could
+ * replace an already flattened parent[i] with a (stale) non-root ancestor. */
+__global__ void flatten_kernel(uint32_t* parent, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t x = atomic_load(&parent[i]);
+    for (uint32_t p = atomic_load(&parent[x]); p != x; p = atomic_load(&parent[x])) x = p;
+    atomic_store(&parent[i], x);
+  }
+}
+
+__global__ void fill_words_kernel(uint64_t* bm, int64_t w0, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < w0; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t rem = n - i * 64;
+    bm[i] = rem >= 64 ? ~0ull : ((1ull << rem) - 1ull);
+  }
+}
+
+/* directed: push the larger label along out-arcs; changed vertices form the
+ * next frontier */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) maxprop_kernel(QEntry* q0, QEntry* q1, QEntry* q2, const QueueHdr* hdr,
+                                                         const int32_t* cols, uint32_t* labels, uint64_t* next_bm,
+                                                         uint64_t* zero_bm, int64_t w0) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) zero_bm[i] = 0ull;
+  QEntry* const q[3] = {q0, q1, q2};
+  for_each_queued_arc<OffT>(q, hdr, cols, [&](uint32_t u, uint32_t v, int64_t) {
+    const uint32_t lu = atomic_load(&labels[u]);
+    if (lu > labels[v]) {
+      const uint32_t old = atomic_max(&labels[v], lu);
+      if (lu > old) {
+        const uint64_t bit = 1ull << (v & 63);
+        if (!(next_bm[v >> 6] & bit)) atomic_or(&next_bm[v >> 6], bit);
+      }
+    }
+  });
+}
+
+template<typename OffT>
+int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
+  sg_device_t dev = g->dev;
+  hipStream_t s = dev->stream;
+  const int64_t n = g->n, w0 = (n + 63) / 64;
+  const Tuning tune = default_tuning();
+  QueueSet Q;
+  uint64_t* bm[2] = {nullptr, nullptr};
+  int st = Q.init(dev, n, g->m, tune);
+  if (st == SG_OK) st = sg::alloc_device(dev, (void**)&bm[0], 8 * (w0 + 1));
+  if (st == SG_OK) st = sg::alloc_device(dev, (void**)&bm[1], 8 * (w0 + 1));
+  auto cleanup = [&]() {
+    Q.release();
+    (void)hipFree(bm[0]);
+    (void)hipFree(bm[1]);
+  };
+  if (st != SG_OK) return cleanup(), st;
+  const OffT* off = (const OffT*)g->offsets;
+  const int grid = dev->cus * 8;
+  int32_t rounds = 0;
+  iota_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(labels, n);
+  fill_words_kernel<<<sg::grid_for(dev, w0, 256, 4), 256, 0, s>>>(bm[0], w0, n);
+  if (!(g->flags & SG_GRAPH_DIRECTED)) {
+    st = compact_to_queues<OffT, kFromBitmap>(s, Q, 0, off, nullptr, nullptr, bm[0], tune);
+    if (st == SG_OK) {
+      hook_kernel<OffT><<<grid, kBlock, 0, s>>>(Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, g->cols, labels);
+      flatten_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(labels, n);
+      rounds = 1;
+    }
+  } else {
+    /* every vertex starts active (the all-vertex pass of cc.hpp:135-145) */
+    QueueHdr* hh = (QueueHdr*)dev->host_scratch;
+    int cur = 0, qs = 0;
+    st = compact_to_queues<OffT, kFromBitmap>(s, Q, qs, off, nullptr, nullptr, bm[cur], tune);
+    if (st == SG_OK) st = sg::zero_async(dev, bm[1], 8 * w0);
+    while (st == SG_OK) {
+      maxprop_kernel<OffT><<<grid, kBlock, 0, s>>>(Q.q[qs][0], Q.q[qs][1], Q.q[qs][2], Q.hdr + qs, g->cols, labels,
+                                                   bm[cur ^ 1], bm[cur], w0);
+      ++rounds;
+      st = compact_to_queues<OffT, kFromBitmap>(s, Q, qs ^ 1, off, nullptr, nullptr, bm[cur ^ 1], tune);
+      if (st != SG_OK) break;
+      SG_HIP(hipMemcpyAsync(hh, Q.hdr + (qs ^ 1), sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
+      SG_HIP(hipStreamSynchronize(s));
+      qs ^= 1;
+      cur ^= 1;
+      if (hh->nvert == 0) break;
+    }
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  cleanup();
+  SG_HIP(e);
+  if (rounds_out) *rounds_out = rounds;
+  return st;
+}
+
+} // namespace
+
+extern "C" int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds) {
+  SG_CHECK(g && labels, SG_ERR_INVALID, "sg_cc: null argument");
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  if (g->offset_bytes == 8) return run_cc<int64_t>(g, labels, rounds);
+  return run_cc<uint32_t>(g, labels, rounds);
+}

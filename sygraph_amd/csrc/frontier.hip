@@ -13,6 +13,8 @@
  * reference's order depends on atomic timing).  The count stays on the device
  * (list_size), so the advance that follows needs no host round trip.
  */
+#include <utility>
+
 #include "internal.hpp"
 
 using namespace sygraph::hipdev;
@@ -352,9 +354,12 @@ int sg_frontier_load(sg_frontier_t f, const uint64_t* host_words) {
 int sg_frontier_swap(sg_frontier_t a, sg_frontier_t b) {
   SG_CHECK(a && b, SG_ERR_INVALID, "sg_frontier_swap: null frontier");
   SG_CHECK(a->n == b->n, SG_ERR_INVALID, "sg_frontier_swap: size mismatch");
-  sg_frontier_s t = *a;
-  *a = *b;
-  *b = t;
+  SG_CHECK(a->dev == b->dev, SG_ERR_INVALID, "sg_frontier_swap: frontiers on different devices");
+  std::swap(a->l0, b->l0);
+  std::swap(a->l1, b->l1);
+  std::swap(a->list, b->list);
+  std::swap(a->list_size, b->list_size);
+  std::swap(a->tile_counts, b->tile_counts);
   return SG_OK;
 }
 

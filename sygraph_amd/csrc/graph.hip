@@ -479,6 +479,7 @@ int sg_graph_destroy(sg_graph_t g) {
   if (!g) return SG_OK;
   (void)hipStreamSynchronize(g->dev->stream);
   sg::bfs_workspace_release(g);
+  sg::sssp_workspace_release(g);
   free_graph_arrays(g);
   delete g;
   return SG_OK;

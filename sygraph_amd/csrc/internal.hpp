@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -52,15 +53,47 @@ struct sg_device_s {
   /* small pinned host scratch for stat read-backs */
   void* host_scratch = nullptr;
   size_t host_scratch_bytes = 0;
+  /* sg_device_close and every graph, frontier and event on the device hold
+   * one reference; the stream and scratch go with the last one, so handles
+   * may be released in any order */
+  std::atomic<int> refs{1};
 };
 
+namespace sg {
+void device_release(sg_device_t dev);
+
+/* Owning reference to a device, held by graphs, frontiers and events. */
+class DevRef {
+public:
+  DevRef() = default;
+  DevRef(const DevRef&) = delete;
+  DevRef& operator=(const DevRef&) = delete;
+  ~DevRef() { reset(); }
+  DevRef& operator=(sg_device_t d) {
+    if (d) d->refs.fetch_add(1, std::memory_order_relaxed);
+    reset();
+    p_ = d;
+    return *this;
+  }
+  void reset() {
+    if (p_) device_release(p_);
+    p_ = nullptr;
+  }
+  operator sg_device_t() const { return p_; }
+  sg_device_s* operator->() const { return p_; }
+
+private:
+  sg_device_t p_ = nullptr;
+};
+} // namespace sg
+
 struct sg_event_s {
-  sg_device_t dev = nullptr;
+  sg::DevRef dev;
   hipEvent_t ev = nullptr;
 };
 
 struct sg_graph_s {
-  sg_device_t dev = nullptr;
+  sg::DevRef dev;
   int64_t n = 0, m = 0;
   int offset_bytes = 8;
   int flags = 0;
@@ -74,11 +107,12 @@ struct sg_graph_s {
   /* lazily built helpers */
   uint64_t* isolated = nullptr; /* bit set for vertices with in-degree 0 */
   int64_t total_degree = 0;
-  void* bfs_ws = nullptr;       /* traversal workspace, reused across calls */
+  void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */
+  void* sssp_ws = nullptr;
 };
 
 struct sg_frontier_s {
-  sg_device_t dev = nullptr;
+  sg::DevRef dev;
   int64_t n = 0, w0 = 0, w1 = 0;
   uint64_t* l0 = nullptr;
   uint64_t* l1 = nullptr;
@@ -103,6 +137,7 @@ int zero_async(sg_device_t dev, void* p, size_t bytes);
 /* graph helpers */
 int graph_ensure_isolated(sg_graph_t g);
 void bfs_workspace_release(sg_graph_t g);
+void sssp_workspace_release(sg_graph_t g);
 
 /* frontier compaction used by other modules */
 int frontier_compact_launch(sg_frontier_t f, int mode);

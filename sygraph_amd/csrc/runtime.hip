@@ -63,6 +63,19 @@ int zero_async(sg_device_t dev, void* p, size_t bytes) {
 
 } // namespace sg
 
+namespace sg {
+void device_release(sg_device_t dev) {
+  if (dev->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  (void)hipSetDevice(dev->ordinal);
+  if (dev->stream) {
+    (void)hipStreamSynchronize(dev->stream);
+    (void)hipStreamDestroy(dev->stream);
+  }
+  if (dev->host_scratch) (void)hipHostFree(dev->host_scratch);
+  delete dev;
+}
+} // namespace sg
+
 extern "C" {
 
 const char* sg_last_error(void) { return g_last_error.c_str(); }
@@ -103,14 +116,7 @@ int sg_device_open(int ordinal, sg_device_t* out) {
 }
 
 int sg_device_close(sg_device_t dev) {
-  if (!dev) return SG_OK;
-  (void)hipSetDevice(dev->ordinal);
-  if (dev->stream) {
-    (void)hipStreamSynchronize(dev->stream);
-    (void)hipStreamDestroy(dev->stream);
-  }
-  if (dev->host_scratch) (void)hipHostFree(dev->host_scratch);
-  delete dev;
+  if (dev) sg::device_release(dev);
   return SG_OK;
 }
 

@@ -1,0 +1,503 @@
+/*
+ * traverse.hpp -- pieces shared by the traversal kernels (BFS, SSSP, CC,
+ * batched BC): degree-class frontier queues, the ordered bitmap -> queue
+ * compaction, spread statistics slots and wave64 helpers.
+ *
+ * A frontier moves between levels as a bitmap (one uint64 word per wave,
+ * lane i <-> bit i).  When a level needs it as work items, an ordered
+ * set-scan-compact turns it into three queues by degree class -- the
+ * thread / wavefront / workgroup bins of the bucketing load balancer
+ * (operators/advance/bucketing.hpp:100-178):
+ *   class 0  degree <  t_medium : one lane per vertex (wave merge path)
+ *   class 1  degree <  t_large  : one wave per vertex
+ *   class 2  larger             : one workgroup per `chunk` arcs
+ * Each entry carries (vertex, first arc, arc count).  Compaction is three
+ * passes -- per-64-word tile counts, one scan, write -- so no counter is ever
+ * contended and queue order is deterministic (ascending vertex id).
+ */
+#pragma once
+
+#include "internal.hpp"
+
+namespace sg::trav {
+
+using namespace sygraph::hipdev;
+
+constexpr int kBlock = 256;
+constexpr int kSlots = 64;
+
+struct alignas(16) QEntry {
+  int64_t start;
+  int32_t v;
+  int32_t len;
+};
+
+/* one 128-byte cache line per slot */
+struct alignas(128) StatSlot {
+  unsigned long long disc, disc_deg, edges, scanned, max_deg;
+  unsigned long long pad[11];
+};
+
+/* per-64-word tile of the compaction */
+struct alignas(32) TileCount {
+  uint32_t c[3]; /* queue entries per class (class 2 counts chunks) */
+  uint32_t nvert;
+  unsigned long long deg;
+  unsigned long long maxdeg;
+};
+
+struct QueueHdr {
+  uint32_t count[3];
+  uint32_t pad;
+  unsigned long long nvert, deg, maxdeg;
+};
+
+struct Tuning {
+  int32_t t_medium; /* degree >= t_medium -> wave per vertex */
+  int32_t t_large;  /* degree >= t_large -> workgroup per chunk */
+  int32_t chunk;    /* arcs per class-2 entry */
+};
+
+inline Tuning default_tuning() { return Tuning{64, 1024, 1024}; }
+
+/* ------------------------------------------------------------ helpers --- */
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(x, o, 64);
+    if (lane_id() >= o) x += t;
+  }
+  return x;
+}
+
+/* 64-bit value of lane `src` (src may differ per lane; all lanes active) */
+__device__ __forceinline__ int64_t shfl_i64(int64_t x, int src) {
+  const uint32_t lo = __shfl((uint32_t)(x & 0xFFFFFFFFll), src, 64);
+  const uint32_t hi = __shfl((uint32_t)((uint64_t)x >> 32), src, 64);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) { return (uint64_t)shfl_i64((int64_t)x, src); }
+
+/* Position of the r-th (0-based) set bit of x; x has more than r bits set. */
+__device__ __forceinline__ uint32_t select_bit(uint64_t x, uint32_t r) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    const uint32_t c = (uint32_t)__popcll(x & ((1ull << w) - 1ull));
+    if (r >= c) {
+      r -= c;
+      x >>= w;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
+/* owner lane of flattened item t: first lane whose inclusive scan exceeds t
+ * (whole wave must call; t may exceed the total, then the result is 63) */
+__device__ __forceinline__ int scan_owner(uint32_t incl, uint32_t t) {
+  uint32_t lo = 0, hi = 63;
+#pragma unroll
+  for (int it = 0; it < 6; ++it) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (__shfl(incl, (int)mid, 64) > t) hi = mid;
+    else lo = mid + 1;
+  }
+  return (int)lo;
+}
+
+__device__ __forceinline__ void zero_slots(StatSlot* s) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(s);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < kSlots * 16; i += gridDim.x * kBlock) p[i] = 0ull;
+}
+
+/* Workgroup-reduce five counters and add them into slot blockIdx % kSlots.
+ * Every thread of the (kBlock-sized) workgroup must call. */
+__device__ __forceinline__ void flush_block_stats(StatSlot* slots, uint64_t disc, uint64_t disc_deg, uint64_t edges,
+                                                  uint64_t scanned, uint64_t max_deg) {
+  __shared__ unsigned long long red[kBlock / 64][5];
+  disc = wave_sum_u64(disc);
+  disc_deg = wave_sum_u64(disc_deg);
+  edges = wave_sum_u64(edges);
+  scanned = wave_sum_u64(scanned);
+  max_deg = wave_max_u64(max_deg);
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    red[wv][0] = disc;
+    red[wv][1] = disc_deg;
+    red[wv][2] = edges;
+    red[wv][3] = scanned;
+    red[wv][4] = max_deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < kBlock / 64; ++i) {
+      t[0] += red[i][0];
+      t[1] += red[i][1];
+      t[2] += red[i][2];
+      t[3] += red[i][3];
+      t[4] = red[i][4] > t[4] ? red[i][4] : t[4];
+    }
+    StatSlot* s = slots + (blockIdx.x % kSlots);
+    if (t[0]) atomic_add(&s->disc, t[0]);
+    if (t[1]) atomic_add(&s->disc_deg, t[1]);
+    if (t[2]) atomic_add(&s->edges, t[2]);
+    if (t[3]) atomic_add(&s->scanned, t[3]);
+    if (t[4]) atomic_max(&s->max_deg, t[4]);
+  }
+}
+
+__device__ __forceinline__ int degree_class(const Tuning& t, int64_t d) {
+  return d >= t.t_large ? 2 : (d >= t.t_medium ? 1 : 0);
+}
+
+/* ------------------------------------------------------- compaction --- */
+enum CompactMode {
+  kFromDiff = 0, /* frontier = visited & ~prev; writes frontier bitmap and prev */
+  kFromBitmapSyncPrev = 1, /* frontier = bitmap; prev := visited */
+  kFromBitmap = 2 /* frontier = bitmap */
+};
+
+template<typename OffT>
+struct CompactArgs {
+  const OffT* off;
+  int64_t w0, ntiles;
+  const uint64_t* visited; /* kFromDiff / kFromBitmapSyncPrev */
+  uint64_t* prev;          /* kFromDiff / kFromBitmapSyncPrev */
+  uint64_t* frontier;      /* bitmap: written (kFromDiff) or read */
+  TileCount* tiles;
+  QEntry* qout[3];
+  QueueHdr* hdr;
+  Tuning tune;
+};
+
+/* pass 1: one wave per 64-word tile */
+template<typename OffT, int kMode>
+__global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT> a) {
+  const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (tile >= a.ntiles) return; /* whole wave leaves together */
+  const int lane = lane_id();
+  const int64_t myw = tile * 64 + lane;
+  uint64_t mynew = 0;
+  if (myw < a.w0) {
+    if (kMode == kFromBitmap) {
+      mynew = a.frontier[myw];
+    } else {
+      const uint64_t vis = a.visited[myw];
+      const uint64_t pv = a.prev[myw];
+      if (kMode == kFromDiff) {
+        mynew = vis & ~pv;
+        a.frontier[myw] = mynew;
+      } else {
+        mynew = a.frontier[myw];
+      }
+      if (vis != pv) a.prev[myw] = vis;
+    }
+  }
+  uint32_t c0 = 0, c1 = 0, c2 = 0, nv = 0;
+  uint64_t deg = 0, mx = 0;
+  uint64_t todo = ballot(mynew != 0);
+  while (todo) {
+    const int k = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mynew, k);
+    if ((word >> lane) & 1ull) {
+      const int64_t v = (tile * 64 + k) * 64 + lane;
+      const int64_t d = (int64_t)(a.off[v + 1] - a.off[v]);
+      ++nv;
+      deg += d;
+      mx = (uint64_t)d > mx ? (uint64_t)d : mx;
+      if (d > 0) {
+        const int cls = degree_class(a.tune, d);
+        if (cls == 0) ++c0;
+        else if (cls == 1) ++c1;
+        else c2 += (uint32_t)((d + a.tune.chunk - 1) / a.tune.chunk);
+      }
+    }
+  }
+  c0 = (uint32_t)wave_sum_u64(c0);
+  c1 = (uint32_t)wave_sum_u64(c1);
+  c2 = (uint32_t)wave_sum_u64(c2);
+  nv = (uint32_t)wave_sum_u64(nv);
+  deg = wave_sum_u64(deg);
+  mx = wave_max_u64(mx);
+  if (lane == 0) {
+    TileCount t;
+    t.c[0] = c0;
+    t.c[1] = c1;
+    t.c[2] = c2;
+    t.nvert = nv;
+    t.deg = deg;
+    t.maxdeg = mx;
+    a.tiles[tile] = t;
+  }
+}
+
+/* pass 2: one workgroup scans the per-tile class counts (in place,
+ * exclusive) and writes the totals into the queue header */
+static __global__ void __launch_bounds__(kBlock) compact_scan_kernel(TileCount* tiles, int64_t ntiles, QueueHdr* hdr) {
+  __shared__ uint32_t wave_tot[kBlock / 64][3];
+  __shared__ unsigned long long red[kBlock / 64][3];
+  uint32_t carry[3] = {0, 0, 0};
+  unsigned long long nv = 0, deg = 0, mx = 0;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (int64_t base = 0; base < ntiles; base += kBlock) {
+    const int64_t i = base + threadIdx.x;
+    TileCount t{};
+    if (i < ntiles) t = tiles[i];
+    nv += t.nvert;
+    deg += t.deg;
+    mx = t.maxdeg > mx ? t.maxdeg : mx;
+    uint32_t incl[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      incl[c] = wave_inclusive_scan(t.c[c]);
+      if (lane == 63) wave_tot[wv][c] = incl[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uint32_t before = 0, tot = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wv) before += wave_tot[w][c];
+        tot += wave_tot[w][c];
+      }
+      if (i < ntiles) tiles[i].c[c] = carry[c] + before + incl[c] - t.c[c];
+      carry[c] += tot;
+    }
+    __syncthreads();
+  }
+  nv = wave_sum_u64(nv);
+  deg = wave_sum_u64(deg);
+  mx = wave_max_u64(mx);
+  if (lane == 0) {
+    red[wv][0] = nv;
+    red[wv][1] = deg;
+    red[wv][2] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    QueueHdr h{};
+    h.count[0] = carry[0];
+    h.count[1] = carry[1];
+    h.count[2] = carry[2];
+    for (int w = 0; w < kBlock / 64; ++w) {
+      h.nvert += red[w][0];
+      h.deg += red[w][1];
+      h.maxdeg = red[w][2] > h.maxdeg ? red[w][2] : h.maxdeg;
+    }
+    *hdr = h;
+  }
+}
+
+/* pass 3: emit the class queues in vertex order at the scanned offsets */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT> a) {
+  const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (tile >= a.ntiles) return;
+  const int lane = lane_id();
+  const int64_t myw = tile * 64 + lane;
+  const uint64_t mynew = myw < a.w0 ? a.frontier[myw] : 0ull;
+  const TileCount tc = a.tiles[tile];
+  uint32_t pos0 = tc.c[0], pos1 = tc.c[1], pos2 = tc.c[2];
+  uint64_t todo = ballot(mynew != 0);
+  while (todo) {
+    const int k = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const uint64_t word = shfl_u64(mynew, k);
+    const int64_t v = (tile * 64 + k) * 64 + lane;
+    const bool has = (word >> lane) & 1ull;
+    int64_t s = 0, d = 0;
+    if (has) {
+      s = (int64_t)a.off[v];
+      d = (int64_t)(a.off[v + 1] - a.off[v]);
+    }
+    const int cls = (has && d > 0) ? degree_class(a.tune, d) : -1;
+    const uint64_t m0 = ballot(cls == 0), m1 = ballot(cls == 1);
+    QEntry qe;
+    qe.start = s;
+    qe.v = (int32_t)v;
+    qe.len = (int32_t)d;
+    if (cls == 0) a.qout[0][pos0 + lanes_below(m0)] = qe;
+    if (cls == 1) a.qout[1][pos1 + lanes_below(m1)] = qe;
+    pos0 += (uint32_t)__popcll(m0);
+    pos1 += (uint32_t)__popcll(m1);
+    const uint32_t nch = cls == 2 ? (uint32_t)((d + a.tune.chunk - 1) / a.tune.chunk) : 0u;
+    const uint32_t incl = wave_inclusive_scan(nch);
+    if (nch) {
+      const uint32_t p = pos2 + incl - nch;
+      for (uint32_t c = 0; c < nch; ++c) {
+        QEntry ce;
+        ce.start = s + (int64_t)c * a.tune.chunk;
+        ce.v = (int32_t)v;
+        const int64_t rem = d - (int64_t)c * a.tune.chunk;
+        ce.len = (int32_t)(rem < a.tune.chunk ? rem : a.tune.chunk);
+        a.qout[2][p + c] = ce;
+      }
+    }
+    pos2 += __shfl(incl, 63, 64);
+  }
+}
+
+/* Single-vertex queue (level-0 source) written by one thread. */
+template<typename OffT>
+__device__ __forceinline__ void enqueue_single(const OffT* off, const Tuning& t, QEntry* const q[3], QueueHdr* hdr,
+                                               int32_t v) {
+  const int64_t s = (int64_t)off[v], d = (int64_t)(off[v + 1] - off[v]);
+  QueueHdr h{};
+  if (d > 0) {
+    const int cls = degree_class(t, d);
+    if (cls < 2) {
+      QEntry qe;
+      qe.start = s;
+      qe.v = v;
+      qe.len = (int32_t)d;
+      q[cls][0] = qe;
+      h.count[cls] = 1;
+    } else {
+      const uint32_t nch = (uint32_t)((d + t.chunk - 1) / t.chunk);
+      for (uint32_t c = 0; c < nch; ++c) {
+        QEntry ce;
+        ce.start = s + (int64_t)c * t.chunk;
+        ce.v = v;
+        const int64_t rem = d - (int64_t)c * t.chunk;
+        ce.len = (int32_t)(rem < t.chunk ? rem : t.chunk);
+        q[2][c] = ce;
+      }
+      h.count[2] = nch;
+    }
+  }
+  h.nvert = 1;
+  h.deg = (unsigned long long)d;
+  h.maxdeg = (unsigned long long)d;
+  *hdr = h;
+}
+
+/* Visit every queued arc with `visit(src, dst, arc_index)`: class 2 entries
+ * per workgroup, class 1 per wave, class 0 as a wave-level merge path (the
+ * arcs of 64 small vertices laid end to end, 64 per step).  All threads of
+ * the workgroup must call. */
+template<typename OffT, typename Visit>
+__device__ __forceinline__ void for_each_queued_arc(QEntry* const q[3], const QueueHdr* hdr, const int32_t* cols,
+                                                    Visit&& visit) {
+  const uint32_t n_small = hdr->count[0], n_med = hdr->count[1], n_large = hdr->count[2];
+  const int lane = lane_id();
+  const int wave_in_block = threadIdx.x >> 6;
+  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (uint32_t i = blockIdx.x; i < n_large; i += gridDim.x) {
+    const QEntry qe = q[2][i];
+    for (int64_t j = wave_in_block * 64 + lane; j < qe.len; j += kBlock) {
+      const int64_t e = qe.start + j;
+      visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+    }
+  }
+  for (int64_t i = gwave; i < n_med; i += nwaves) {
+    const QEntry qe = q[1][i];
+    for (int64_t j = lane; j < qe.len; j += 64) {
+      const int64_t e = qe.start + j;
+      visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+    }
+  }
+  for (int64_t base = gwave * 64; base < n_small; base += nwaves * 64) {
+    const int64_t i = base + lane;
+    QEntry qe;
+    qe.start = 0;
+    qe.v = 0;
+    qe.len = 0;
+    if (i < n_small) qe = q[0][i];
+    const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      const int owner = scan_owner(incl, t);
+      const uint32_t before = __shfl(incl - (uint32_t)qe.len, owner, 64);
+      const int64_t st = shfl_i64(qe.start, owner);
+      const int32_t src = __shfl(qe.v, owner, 64);
+      if (t < total) {
+        const int64_t e = st + (t - before);
+        visit((uint32_t)src, (uint32_t)cols[e], e);
+      }
+    }
+  }
+}
+
+/* --------------------------------------------------------- host side --- */
+/* Two sets of class queues (current / next) plus compaction scratch. */
+struct QueueSet {
+  int64_t w0 = 0, ntiles = 0, cap_large = 0;
+  QEntry* q[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  QueueHdr* hdr = nullptr; /* 2 headers */
+  TileCount* tiles = nullptr;
+
+  int init(sg_device_t dev, int64_t n, int64_t m, const Tuning& t) {
+    w0 = (n + 63) / 64;
+    ntiles = (w0 + 63) / 64;
+    cap_large = m / t.chunk + m / t.t_large + 64;
+    int st = SG_OK;
+    if ((st = alloc_device(dev, (void**)&hdr, sizeof(QueueHdr) * 2)) ||
+        (st = alloc_device(dev, (void**)&tiles, sizeof(TileCount) * (ntiles + 1))))
+      return st;
+    for (int s = 0; s < 2; ++s)
+      for (int k = 0; k < 3; ++k)
+        if ((st = alloc_device(dev, (void**)&q[s][k], sizeof(QEntry) * (k == 2 ? cap_large : n + 1)))) return st;
+    return SG_OK;
+  }
+  void release() {
+    for (auto& s : q)
+      for (auto*& p : s) {
+        (void)hipFree(p);
+        p = nullptr;
+      }
+    (void)hipFree(hdr);
+    (void)hipFree(tiles);
+    hdr = nullptr;
+    tiles = nullptr;
+  }
+  int grid_tiles() const { return (int)((ntiles * 64 + kBlock - 1) / kBlock); }
+};
+
+/* Launch the three compaction passes into queue set `dst`.  count_only skips
+ * the write pass (totals still land in hdr[dst]). */
+template<typename OffT, int kMode>
+inline int compact_to_queues(hipStream_t s, QueueSet& qs, int dst, const OffT* off, const uint64_t* visited,
+                             uint64_t* prev, uint64_t* frontier, const Tuning& t, bool count_only = false) {
+  CompactArgs<OffT> a{};
+  a.off = off;
+  a.w0 = qs.w0;
+  a.ntiles = qs.ntiles;
+  a.visited = visited;
+  a.prev = prev;
+  a.frontier = frontier;
+  a.tiles = qs.tiles;
+  for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
+  a.hdr = qs.hdr + dst;
+  a.tune = t;
+  compact_count_kernel<OffT, kMode><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  compact_scan_kernel<<<1, kBlock, 0, s>>>(qs.tiles, qs.ntiles, qs.hdr + dst);
+  if (!count_only) compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  SG_HIP(hipGetLastError());
+  return SG_OK;
+}
+
+/* Only the write pass (counts and scan already done by compact_to_queues
+ * with count_only = true on the same frontier). */
+template<typename OffT>
+inline int compact_write_only(hipStream_t s, QueueSet& qs, int dst, const OffT* off, uint64_t* frontier,
+                              const Tuning& t) {
+  CompactArgs<OffT> a{};
+  a.off = off;
+  a.w0 = qs.w0;
+  a.ntiles = qs.ntiles;
+  a.frontier = frontier;
+  a.tiles = qs.tiles;
+  for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
+  a.hdr = qs.hdr + dst;
+  a.tune = t;
+  compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  SG_HIP(hipGetLastError());
+  return SG_OK;
+}
+
+} // namespace sg::trav

@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py            # small fixtures (seconds)
     python tests/golden/make_golden.py --rmat24   # + the RMAT scale-24 summary (minutes)
+    python tests/golden/make_golden.py --c3       # + grid / RMAT-20 CC and TC summaries
 
 reference_cases.json -- the adjacency-matrix fixtures of the reference's own
   tests (tests/include/test_utils.hpp:21-45, include/sygraph/io/matrices.hpp:21-41)
@@ -12,6 +13,9 @@ rmat14_seed1.npz -- the CPU-device configuration (RMAT scale 14, edge factor 16,
   a/b/c/d = .57/.19/.19/.05, seed 1): digests, source (highest degree, lowest id
   on ties), oracle BFS depths, SSSP distances (weights seed 3), CC labels, TC
   count and FP64 BC for three sources; each cross-checked against scipy/networkx.
+c3_cc_tc.json -- CC labels digest / component count on the 4096x4096 grid with 10%
+  of edges deleted (seed 4) and on RMAT scale 20 edge factor 32 (seed 5); exact
+  triangle count of that RMAT graph.
 rmat24_seed2.json -- RMAT scale 24 (seed 2): vertex/arc counts, CSR digests, 16
   sources drawn uniformly (seed 2) from the largest component, and per source
   the BFS level count, reached vertices/edges and the depth-array digest; SSSP
@@ -126,6 +130,27 @@ def write_rmat14():
                         source=src, depth=d, levels=lv, sssp=ds, cc=lab, tc=tcount, bc_sources=bc_src, bc=b)
 
 
+def write_c3():
+    """CC on the 4096x4096 grid (10% of edges deleted, seed 4) and on RMAT scale 20
+    edge factor 32 (seed 5); TC on the same RMAT graph."""
+    t0 = time.time()
+    g = oracle.grid(4096, 4096, 0.1, seed=4)
+    lab = oracle.cc(g)
+    out = {"grid": {"n": g.n, "m": g.m, "digest": [str(x) for x in g.digest()],
+                    "components": int(len(np.unique(lab))), "labels_digest": str(oracle.digest_i32(lab.view(np.int32)))}}
+    print(f"grid done {time.time() - t0:.1f}s", flush=True)
+    g = oracle.rmat(20, 32, 0.57, 0.19, 0.19, seed=5)
+    lab = oracle.cc(g)
+    t1 = time.time()
+    tri = oracle.tc(g)
+    print(f"rmat20 tc {tri} in {time.time() - t1:.1f}s", flush=True)
+    out["rmat20"] = {"n": g.n, "m": g.m, "digest": [str(x) for x in g.digest()],
+                     "components": int(len(np.unique(lab))), "labels_digest": str(oracle.digest_i32(lab.view(np.int32))),
+                     "triangles": str(tri)}
+    with open(os.path.join(OUT, "c3_cc_tc.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 def write_rmat24():
     t0 = time.time()
     g = oracle.rmat(24, 16, 0.57, 0.19, 0.19, seed=2)
@@ -157,8 +182,11 @@ def write_rmat24():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--rmat24", action="store_true")
+    ap.add_argument("--c3", action="store_true")
     a = ap.parse_args()
     write_reference_cases()
     write_rmat14()
     if a.rmat24:
         write_rmat24()
+    if a.c3:
+        write_c3()
