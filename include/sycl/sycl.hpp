@@ -9,9 +9,10 @@
  * So that those callers compile unchanged, this header provides exactly that
  * host-side surface, backed by libsygraph_hip: a queue is an sg_device_t (one
  * HIP device and one stream), USM maps to hipMalloc / hipMallocManaged /
- * hipHostMalloc, and events are pairs of hipEvents.  There is no kernel
- * submission (parallel_for, handlers, accessors): all device code in this
- * repository is HIP.
+ * hipHostMalloc, and events are pairs of hipEvents.  Kernel submission is
+ * limited to queue::submit / handler::parallel_for over a one-dimensional
+ * range (a grid-stride HIP launch of the callable) and single_task; there are
+ * no buffers, accessors or nd_ranges.
  */
 #pragma once
 
@@ -191,6 +192,69 @@ private:
   std::shared_ptr<State> _s;
 };
 
+/* One-dimensional index space and item (tests/formats/csr.cpp:18-22 and
+ * tests/frontier/bitmap_frontier.cpp:17-20 submit such kernels). */
+template<int D = 1>
+class range {
+  static_assert(D == 1, "facade: one-dimensional ranges only");
+
+public:
+  __host__ __device__ range(size_t n) : _n(n) {}
+  __host__ __device__ size_t size() const { return _n; }
+  __host__ __device__ size_t operator[](int) const { return _n; }
+
+private:
+  size_t _n;
+};
+
+template<int D = 1>
+class id {
+  static_assert(D == 1, "facade: one-dimensional ids only");
+
+public:
+  __host__ __device__ id(size_t i = 0) : _i(i) {}
+  __host__ __device__ operator size_t() const { return _i; }
+  __host__ __device__ size_t operator[](int) const { return _i; }
+  __host__ __device__ size_t get(int) const { return _i; }
+
+private:
+  size_t _i;
+};
+
+namespace detail {
+template<typename F>
+__global__ void __launch_bounds__(256) range_kernel(size_t n, F f) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) f(id<1>(i));
+}
+template<typename F>
+__global__ void single_task_kernel(F f) {
+  f();
+}
+} // namespace detail
+
+/* Command-group handler: kernels are HIP launches on the queue's stream. */
+class handler {
+public:
+  explicit handler(hipStream_t s, unsigned max_blocks) : _s(s), _max_blocks(max_blocks) {}
+  template<typename F>
+  void parallel_for(range<1> r, F f) {
+    if (r.size() == 0) return;
+    const size_t want = (r.size() + 255) / 256;
+    const unsigned blocks = (unsigned)(want < _max_blocks ? want : _max_blocks);
+    detail::range_kernel<<<blocks, 256, 0, _s>>>(r.size(), f);
+    if (hipGetLastError() != hipSuccess) throw exception("parallel_for: kernel launch failed");
+  }
+  template<typename F>
+  void single_task(F f) {
+    detail::single_task_kernel<<<1, 1, 0, _s>>>(f);
+    if (hipGetLastError() != hipSuccess) throw exception("single_task: kernel launch failed");
+  }
+
+private:
+  hipStream_t _s;
+  unsigned _max_blocks;
+};
+
 class queue {
 public:
   queue() : queue(gpu_selector_v) {}
@@ -208,8 +272,26 @@ public:
     return static_cast<hipStream_t>(s);
   }
   hipEvent_t origin() const { return _origin ? _origin->ev : nullptr; }
+  unsigned max_blocks() const {
+    sg_device_info i{};
+    detail::check(sg_device_info_get(_dev.get(), &i));
+    return (unsigned)(i.compute_units > 0 ? i.compute_units * 8 : 2048);
+  }
 
   void wait() { detail::check(sg_device_synchronize(_dev.get())); }
+
+  template<typename CGF>
+  event submit(CGF&& cgf) {
+    event e = event::begin(handle(), stream(), origin());
+    handler h(stream(), max_blocks());
+    cgf(h);
+    e.finish();
+    return e;
+  }
+  template<typename F>
+  event parallel_for(range<1> r, F f) {
+    return submit([&](handler& h) { h.parallel_for(r, f); });
+  }
   void wait_and_throw() { wait(); }
 
   template<typename T>

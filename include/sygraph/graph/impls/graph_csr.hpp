@@ -6,9 +6,14 @@
  * caller's widths (32- or 64-bit offsets, 32-bit ids, 4-byte weights) and, for
  * a directed graph, builds the inverse CSR on the device; an undirected graph
  * aliases it (graph_csr.hpp:423-425).  Edge ids are OffsetT end to end (the
- * reference truncates them to the vertex type, graph_csr.hpp:71).  The host
- * CSR is kept, so host accessors (getRowOffsets, ...) work in every memory
- * space.
+ * reference truncates them to the vertex type, graph_csr.hpp:71).
+ *
+ * Host accessors follow the reference (graph_csr.hpp:306-371): with
+ * memory::space::device they return the host CSR kept by the graph; with
+ * shared or host space they return the graph's own arrays (managed or pinned
+ * memory), the ones the kernels use, so host and device code see the same
+ * data.  A values array always exists (all ones when the input has none), as
+ * the reference allocates one of m entries unconditionally.
  */
 #pragma once
 
@@ -122,6 +127,9 @@ public:
   using edge_t = OffsetT;
   using weight_t = ValueT;
   using device_graph_t = GraphCSRDevice<IndexT, OffsetT, ValueT>;
+  static_assert(sizeof(IndexT) == 4, "vertex ids are stored as 32-bit integers");
+  static_assert(sizeof(OffsetT) == 4 || sizeof(OffsetT) == 8, "row offsets are stored as 32- or 64-bit integers");
+  static_assert(sizeof(ValueT) == 4, "edge values are stored as 4-byte integers or floats");
 
   GraphCSR(sycl::queue& q, const formats::CSR<ValueT, IndexT, OffsetT>& csr, Properties properties)
       : Graph<IndexT, OffsetT, ValueT>(properties), _queue(q), _csr(csr) {
@@ -157,11 +165,11 @@ public:
 
   size_t getVertexCount() const override { return _device_graph.getVertexCount(); }
   size_t getEdgeCount() const override { return _device_graph.getEdgeCount(); }
-  size_t getDegree(vertex_t v) const override { return _csr.getRowOffsets()[v + 1] - _csr.getRowOffsets()[v]; }
-  edge_t getFirstNeighbor(vertex_t v) const override { return _csr.getRowOffsets()[v]; }
+  size_t getDegree(vertex_t v) const override { return getRowOffsets()[v + 1] - getRowOffsets()[v]; }
+  edge_t getFirstNeighbor(vertex_t v) const override { return getRowOffsets()[v]; }
   vertex_t getSourceVertex(edge_t edge) const override {
-    const auto& off = _csr.getRowOffsets();
-    size_t lo = 0, hi = off.size() - 1;
+    const OffsetT* off = getRowOffsets();
+    size_t lo = 0, hi = getVertexCount();
     while (lo < hi) {
       const size_t mid = lo + (hi - lo) / 2;
       if (off[mid + 1] <= edge) lo = mid + 1;
@@ -169,24 +177,31 @@ public:
     }
     return static_cast<vertex_t>(lo);
   }
-  vertex_t getDestinationVertex(edge_t edge) const override { return _csr.getColumnIndices()[edge]; }
-  weight_t getEdgeWeight(edge_t edge) const override { return _csr.getValues()[edge]; }
+  vertex_t getDestinationVertex(edge_t edge) const override { return getColumnIndices()[edge]; }
+  weight_t getEdgeWeight(edge_t edge) const override { return getValues()[edge]; }
 
   IndexT getOffsetsSize() const { return static_cast<IndexT>(getVertexCount() + 1); }
   OffsetT getValuesSize() const { return static_cast<OffsetT>(getEdgeCount()); }
 
-  /* host copies of the CSR arrays */
-  IndexT* getColumnIndices() { return _csr.getColumnIndices().data(); }
-  const IndexT* getColumnIndices() const { return _csr.getColumnIndices().data(); }
-  OffsetT* getRowOffsets() { return _csr.getRowOffsets().data(); }
-  const OffsetT* getRowOffsets() const { return _csr.getRowOffsets().data(); }
-  ValueT* getValues() { return _csr.getValues().data(); }
-  const ValueT* getValues() const { return _csr.getValues().data(); }
+  /* host-readable CSR arrays: the host copy (device space) or the graph's own
+   * managed / pinned arrays (shared, host space) */
+  IndexT* getColumnIndices() const {
+    if constexpr (Space == memory::space::device) return const_cast<IndexT*>(_csr.getColumnIndices().data());
+    else return _device_graph.getColumnIndices();
+  }
+  OffsetT* getRowOffsets() const {
+    if constexpr (Space == memory::space::device) return const_cast<OffsetT*>(_csr.getRowOffsets().data());
+    else return _device_graph.getRowOffsets();
+  }
+  ValueT* getValues() const {
+    if constexpr (Space == memory::space::device) return const_cast<ValueT*>(_csr.getValues().data());
+    else return _device_graph.getValues();
+  }
 
   size_t getIntersectionCount(const vertex_t& src, const vertex_t& dst) const {
     size_t count = 0;
-    const auto& off = _csr.getRowOffsets();
-    const auto& col = _csr.getColumnIndices();
+    const OffsetT* off = getRowOffsets();
+    const IndexT* col = getColumnIndices();
     OffsetT a = off[src], ae = off[src + 1], b = off[dst], be = off[dst + 1];
     while (a < ae && b < be) {
       if (col[a] == col[b]) {
@@ -213,11 +228,12 @@ private:
     if (std::is_floating_point_v<ValueT>) flags |= SG_GRAPH_WEIGHT_FLOAT;
     if constexpr (Space == memory::space::shared) flags |= SG_GRAPH_SPACE_SHARED;
     if constexpr (Space == memory::space::host) flags |= SG_GRAPH_SPACE_HOST;
-    const bool has_values = _csr.getValues().size() == m && m > 0;
+    if (_csr.getValues().size() != m) _csr.getValues().assign(m, static_cast<ValueT>(1));
     if (_csr.getRowOffsets().empty()) _csr.getRowOffsets().push_back(0);
+    flags |= SG_GRAPH_WEIGHTED;
     int st = sg_graph_create(_queue.handle(), (int64_t)n, (int64_t)m, _csr.getRowOffsets().data(), (int)sizeof(OffsetT),
                              reinterpret_cast<const int32_t*>(_csr.getColumnIndices().data()),
-                             has_values ? _csr.getValues().data() : nullptr, flags, &_handle);
+                             m > 0 ? _csr.getValues().data() : nullptr, flags, &_handle);
     if (st != SG_OK) throw std::runtime_error(sg_last_error());
     sg_graph_view v{};
     sg_graph_get_view(_handle, &v);

@@ -14,6 +14,7 @@ import ctypes as C
 import os
 import subprocess
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 
@@ -70,7 +71,7 @@ def lib():
         L.or_tc.restype = C.c_uint64
         L.or_tc.argtypes = [C.c_int64, _i64p, _i32p]
         L.or_bc.restype = None
-        L.or_bc.argtypes = [C.c_int64, _i64p, _i32p, C.c_int32, _i32p, _f64p]
+        L.or_bc.argtypes = [C.c_int64, _i64p, _i32p, C.c_void_p, C.c_void_p, C.c_int32, _i32p, _f64p]
         L.or_pull_scan_count.restype = C.c_int64
         L.or_pull_scan_count.argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int32, C.POINTER(C.c_int64)]
         L.or_bfs_plain_omp.restype = C.c_int32
@@ -103,6 +104,17 @@ class CSRGraph:
 
     def digest(self) -> tuple[int, int]:
         return digest_i64(self.offsets), digest_i32(self.cols)
+
+
+def transpose(g: CSRGraph) -> CSRGraph:
+    """In-arc CSR of a directed graph: row v lists every u with an arc u -> v,
+    ascending (the inverse CSR of graph/impls/graph_csr.hpp:423-425)."""
+    src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.offsets))
+    order = np.lexsort((src, g.cols.astype(np.int64)))
+    off = np.zeros(g.n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(g.cols, minlength=g.n), out=off[1:])
+    w = g.weights[order] if g.weights is not None else None
+    return CSRGraph(g.n, off, np.ascontiguousarray(src[order], dtype=np.int32), w)
 
 
 def csr_from_pairs(n: int, u: np.ndarray, v: np.ndarray) -> CSRGraph:
@@ -218,11 +230,17 @@ def tc(g: CSRGraph) -> int:
     return int(lib().or_tc(g.n, g.offsets, g.cols))
 
 
-def bc(g: CSRGraph, sources) -> np.ndarray:
-    """FP64 Brandes summed over sources, source excluded, no /2 (algorithms/bc.hpp:118-200)."""
+def bc(g: CSRGraph, sources, inverse: Optional[CSRGraph] = None) -> np.ndarray:
+    """FP64 Brandes summed over sources, source excluded, no /2 (algorithms/bc.hpp:118-200).
+    For a directed graph pass its inverse (in-arc CSR): sigma is pulled over in-arcs."""
     s = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
     out = np.empty(g.n, dtype=np.float64)
-    lib().or_bc(g.n, g.offsets, g.cols, len(s), s, out)
+    io = ic = None
+    if inverse is not None:
+        io = np.ascontiguousarray(inverse.offsets, dtype=np.int64)
+        ic = np.ascontiguousarray(inverse.cols, dtype=np.int32)
+    lib().or_bc(g.n, g.offsets, g.cols, io.ctypes.data if io is not None else None,
+                ic.ctypes.data if ic is not None else None, len(s), s, out)
     return out
 
 

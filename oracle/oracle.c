@@ -413,7 +413,13 @@ uint64_t or_tc(int64_t n, const int64_t* offsets, const int32_t* cols) {
 /* scaling is exact, so this equals unscaled FP64 Brandes wherever the latter  */
 /* does not overflow, and stays finite on high-diameter grids where it would. */
 /* ------------------------------------------------------------------------- */
-void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, int32_t nsrc, const int32_t* srcs, double* bc) {
+void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t* in_offsets, const int32_t* in_cols,
+           int32_t nsrc, const int32_t* srcs, double* bc) {
+  /* sigma is pulled over in-arcs; an undirected graph is its own inverse */
+  if (!in_offsets) {
+    in_offsets = offsets;
+    in_cols = cols;
+  }
   int32_t* depth = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
   int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
   int64_t* lvl_start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 2));
@@ -454,8 +460,8 @@ void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, int32_t nsrc,
         for (int64_t i = lend; i < tail; ++i) {
           int32_t y = order[i];
           double acc = 0.0;
-          for (int64_t e = offsets[y]; e < offsets[y + 1]; ++e)
-            if (depth[cols[e]] == depth[y] - 1) acc += sig[cols[e]];
+          for (int64_t e = in_offsets[y]; e < in_offsets[y + 1]; ++e)
+            if (depth[in_cols[e]] == depth[y] - 1) acc += sig[in_cols[e]];
           sig[y] = acc;
           if (acc > mx) mx = acc;
         }

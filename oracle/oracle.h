@@ -54,7 +54,8 @@ void or_cc(int64_t n, const int64_t* offsets, const int32_t* cols, uint32_t* lab
 uint64_t or_tc(int64_t n, const int64_t* offsets, const int32_t* cols);
 /* Brandes betweenness, FP64 with exact power-of-two per-level sigma scaling,
  * summed over nsrc sources, source excluded, no /2 (algorithms/bc.hpp:118-200). */
-void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, int32_t nsrc, const int32_t* srcs, double* bc);
+void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t* in_offsets, const int32_t* in_cols,
+           int32_t nsrc, const int32_t* srcs, double* bc);
 
 /* Pull-level scan cost for the byte model: for every vertex u with depth[u] > level
  * (unvisited at that level) and degree > 0, k_u = 1 + position of the first neighbour

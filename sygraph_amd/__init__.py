@@ -613,3 +613,35 @@ class TC:
         if self.count is None:
             raise SygraphError(SG_ERR_STATE, "TC instance not initialized")
         return self.count
+
+
+class BC:
+    """algorithms::BC (algorithms/bc.hpp:24-224): Brandes dependencies summed
+    over the given sources into a float vector (source excluded, no halving)."""
+
+    def __init__(self, g: Graph):
+        self.g = g
+        self.values: Optional[DeviceBuffer] = None
+        self.sources: list[int] = []
+
+    def init(self, sources):
+        self.sources = [int(s) for s in (sources if hasattr(sources, "__len__") else [sources])]
+        self.values = DeviceBuffer(self.g.dev, 4 * max(self.g.n, 1), dtype=np.float32, shape=(self.g.n,))
+        self.values.fill(0, 4, max(self.g.n, 1))
+
+    def run(self):
+        if self.values is None:
+            raise SygraphError(SG_ERR_STATE, "BC instance not initialized")
+        s = np.ascontiguousarray(np.asarray(self.sources, dtype=np.int32))
+        check(lib().sg_bc(self.g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(self.values.ptr)))
+
+    def get_values(self) -> np.ndarray:
+        return self.values.download(np.float32, self.g.n)
+
+
+def bfs_multi(g: Graph, sources) -> np.ndarray:
+    """Depths from several sources, row i = sources[i] (unreached = n + 1)."""
+    s = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+    out = DeviceBuffer(g.dev, 4 * max(len(s) * g.n, 1), dtype=np.int32)
+    check(lib().sg_bfs_multi(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr)))
+    return out.download(np.int32, len(s) * g.n).reshape(len(s), g.n)

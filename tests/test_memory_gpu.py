@@ -1,0 +1,39 @@
+"""Memory spaces of the C ABI on the MI355X: device, shared (managed) and
+host (pinned) allocations are written by a kernel and read by the host, and
+the reverse (the USM spaces of utils/memory.hpp:12-39 that the reference's
+tests and examples use: tests/operators/advance.cpp:19-20 shares `visited`
+between host loops and device functors)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _host_view(buf, n, dtype=np.uint32):
+    ctype = {np.uint32: C.c_uint32, np.uint64: C.c_uint64}[dtype]
+    return np.ctypeslib.as_array((ctype * n).from_address(buf.ptr))
+
+
+@pytest.mark.parametrize("space_name", ["SPACE_SHARED", "SPACE_HOST"])
+def test_host_accessible_spaces(sg, device, space_name):
+    n = 100_003
+    space = getattr(sg, space_name)
+    buf = device.alloc(4 * n, space)
+    buf.fill(0xDEADBEEF, 4, n)  # device kernel writes (sg_fill synchronizes)
+    view = _host_view(buf, n)
+    assert (view == 0xDEADBEEF).all()
+    view[:] = np.arange(n, dtype=np.uint32)  # host writes, device reads
+    dev_copy = device.alloc(4 * n, sg.SPACE_DEVICE)
+    sg.lib().sg_memcpy(device.h, C.c_void_p(dev_copy.ptr), C.c_void_p(buf.ptr), 4 * n)
+    assert np.array_equal(dev_copy.download(np.uint32), np.arange(n, dtype=np.uint32))
+
+
+def test_device_space_roundtrip(sg, device):
+    x = np.random.default_rng(1).integers(0, 2**63, size=77_777, dtype=np.uint64)
+    b = device.to_device(x)
+    assert np.array_equal(b.download(np.uint64), x)
+    b.fill(7, 8, 10)
+    y = b.download(np.uint64)
+    assert (y[:10] == 7).all() and np.array_equal(y[10:], x[10:])

@@ -214,6 +214,38 @@ def test_bc_matches_networkx(oracle_mod):
     assert np.allclose(b, ref, rtol=1e-12, atol=1e-9)
 
 
+def _random_digraph(oracle_mod, n, m, seed):
+    rng = np.random.default_rng(seed)
+    u, v = rng.integers(0, n, m), rng.integers(0, n, m)
+    keep = u != v
+    pairs = np.unique(np.stack([u[keep], v[keep]], 1), axis=0)
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(pairs[:, 0], minlength=n), out=off[1:])
+    return oracle_mod.CSRGraph(n, off, np.ascontiguousarray(pairs[:, 1], dtype=np.int32)), pairs
+
+
+def test_transpose_is_inverse_csr(oracle_mod):
+    g, pairs = _random_digraph(oracle_mod, 200, 900, seed=2)
+    t = oracle_mod.transpose(g)
+    back = oracle_mod.transpose(t)
+    assert np.array_equal(back.offsets, g.offsets) and np.array_equal(back.cols, g.cols)
+    for v in (0, 17, 199):
+        assert t.cols[t.offsets[v]:t.offsets[v + 1]].tolist() == sorted(pairs[pairs[:, 1] == v, 0].tolist())
+
+
+def test_bc_directed_matches_networkx(oracle_mod):
+    # directed: sigma over in-arcs, dependencies over out-arcs, no halving
+    n = 300
+    g, pairs = _random_digraph(oracle_mod, n, 1500, seed=5)
+    srcs = [0, 7, 42]
+    G = nx.DiGraph()
+    G.add_nodes_from(range(n))
+    G.add_edges_from(pairs.tolist())
+    ref = nx.betweenness_centrality_subset(G, sources=srcs, targets=list(G.nodes()), normalized=False)
+    ref = np.array([ref[i] for i in range(n)])
+    assert np.allclose(oracle_mod.bc(g, srcs, inverse=oracle_mod.transpose(g)), ref, rtol=1e-12, atol=1e-9)
+
+
 def test_bc_grid_scaling_stays_finite(oracle_mod):
     # path counts on a 200x200 grid exceed 1e116 at the far corner; the scaled
     # oracle must stay finite and symmetric under the grid's reflection.
