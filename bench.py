@@ -69,6 +69,55 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect):
     return edges_total, wall, kernel_ms_total, per_kernel
 
 
+def copy_ceiling(sg, dev, mib: int = 1024, reps: int = 5):
+    """Read+write bandwidth of a plain 16-byte streaming copy (sg_copy_device)
+    of `mib` MiB, best of `reps`, timed with hipEvents on the library stream."""
+    import ctypes as C
+
+    nbytes = mib << 20
+    src, dst = dev.alloc(nbytes), dev.alloc(nbytes)
+    src.fill(0x5A, 1, nbytes)
+    lib = sg.lib()
+    e0, e1 = dev.event(), dev.event()
+    best = None
+    for _ in range(reps):
+        e0.record()
+        sg.check(lib.sg_copy_device(dev.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes))
+        e1.record()
+        dev.synchronize()
+        ms = e0.elapsed_ms(e1)
+        best = ms if best is None else min(best, ms)
+    src.free()
+    dst.free()
+    return 2.0 * nbytes / (best / 1e3) / 1e9
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter read on
+    the calibration copy kernel), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "bfs_hybrid_rmat24_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
+def run_sssp(sg, G, sources):
+    s = sg.SSSP(G)
+    edges = 0
+    G.dev.synchronize()
+    t0 = time.perf_counter()
+    for src in sources:
+        s.init(src)
+        st = s.run()
+        edges += st.reached_edges
+    G.dev.synchronize()
+    return edges, time.perf_counter() - t0
+
+
 def cpu_baseline(oracle, G, sources, budget_s: float):
     off, cols, _ = G.download()
     g = oracle.CSRGraph(G.n, off, cols)
@@ -112,7 +161,7 @@ def main():
     info = dev.info()
     log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
     t0 = time.perf_counter()
-    G = sg.Graph.rmat(dev, a.scale, 16, 0.57, 0.19, 0.19, seed=2)
+    G = sg.Graph.rmat(dev, a.scale, 16, 0.57, 0.19, 0.19, seed=2, weight_seed=3)
     log(f"graph: RMAT s{a.scale} n={G.n} m={G.m} generated on device in {time.perf_counter() - t0:.2f}s")
     golden = os.path.join(ROOT, "tests", "golden", "rmat24_seed2.json")
     if a.scale == 24 and os.path.exists(golden):
@@ -135,7 +184,8 @@ def main():
     achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     kernel_names = {0: "push_kernel", 1: "pull_kernel"}
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel_names[dom],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_names[dom]),
+            "kernel": kernel_names[dom],
             "launches": launches, "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
             "alg_bytes_per_launch": round(byts / max(launches, 1))}
     for k, name in kernel_names.items():
@@ -144,7 +194,13 @@ def main():
             log(f"  {name}: {n_} launches, avg {m_ / n_ * 1e3:.1f} us, {b_ / (m_ / 1e3) / 1e9:.0f} GB/s algorithmic")
 
     extra = {}
+    cc = copy_ceiling(sg, dev)
+    roof["copy_ceiling_gbs"] = round(cc, 1)
+    log(f"copy kernel ceiling: {cc:.0f} GB/s (read + write)")
     if not a.no_extra:
+        e2, w2 = run_sssp(sg, G, sources)
+        extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
+        log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3)")
         for name, p in [("push", sg.POLICY_PUSH), ("pull", sg.POLICY_PULL), ("hybrid", sg.POLICY_HYBRID)]:
             if name == a.policy:
                 continue

@@ -94,6 +94,11 @@ int sg_alloc(sg_device_t dev, size_t bytes, int space, void** out);
 int sg_free(sg_device_t dev, void* ptr);
 /* queue.copy (e.g. graph_csr.hpp:401-403); direction inferred by HIP. */
 int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte accesses); dst,
+ * src and bytes must be multiples of 16.  No counterpart in the reference:
+ * it is the copy kernel against which HBM bandwidth and the FETCH_SIZE /
+ * WRITE_SIZE counters are calibrated. */
+int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes);
 /* queue.fill for 1/4/8-byte patterns (e.g. algorithms/bfs.hpp:62-66). */
 int sg_fill(sg_device_t dev, void* dst, uint64_t pattern, int pattern_bytes, size_t count);
 
@@ -279,12 +284,13 @@ int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);
  * (device uint64[n] or NULL) gets the triangles attributed to each DAG source. */
 int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex);
 /* BC::run (algorithms/bc.hpp:118-200) for many sources: bc (device
- * float[n]) += sum_s delta_s(v), v != s, no /2.  Sources run 64 per batch,
- * one bit-lane each; sigma is FP32 with per-(lane, level) power-of-two
- * scaling. */
+ * float[n]) += sum_s delta_s(v), v != s, no /2.  Per source: fused BFS for
+ * depths, vertices grouped by level, level-by-level pulls of sigma (FP64,
+ * each level rescaled by a power of two) and of delta (FP64); the FP64 total
+ * is added to bc once.  Sources are processed one after another. */
 int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
-/* Multi-source BFS, 64 sources per batch packed per bit-lane: depth is device
- * int32[nsrc][n] (row s = source s), unreached = n+1. */
+/* Multi-source BFS: depth is device int32[nsrc][n] (row i = sources_host[i]),
+ * unreached = n+1; one fused hybrid BFS per source. */
 int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth);
 
 #ifdef __cplusplus

@@ -107,6 +107,7 @@ _SIGS = {
     "sg_free": (_I, [_P, _P]),
     "sg_memcpy": (_I, [_P, _P, _P, C.c_size_t]),
     "sg_fill": (_I, [_P, _P, C.c_uint64, _I, C.c_size_t]),
+    "sg_copy_device": (_I, [_P, _P, _P, C.c_size_t]),
     "sg_event_create": (_I, [_P, C.POINTER(_P)]),
     "sg_event_destroy": (_I, [_P]),
     "sg_event_record": (_I, [_P]),

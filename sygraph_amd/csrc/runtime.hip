@@ -13,6 +13,12 @@
 namespace {
 thread_local std::string g_last_error = "no error";
 
+/* Plain streaming copy, 16 bytes per lane per step: the reference point for
+ * HBM bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters. */
+__global__ void __launch_bounds__(256) copy_kernel_u128(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 __global__ void fill_kernel_u8(uint8_t* p, uint8_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -188,6 +194,19 @@ int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return SG_OK;
   SG_CHECK(dst && src, SG_ERR_INVALID, "sg_memcpy: null pointer");
   SG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, dev->stream));
+  SG_HIP(hipStreamSynchronize(dev->stream));
+  return SG_OK;
+}
+
+int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes) {
+  SG_CHECK(dev, SG_ERR_INVALID, "sg_copy_device: null device");
+  if (bytes == 0) return SG_OK;
+  SG_CHECK(dst && src, SG_ERR_INVALID, "sg_copy_device: null pointer");
+  SG_CHECK(((uintptr_t)dst | (uintptr_t)src | bytes) % 16 == 0, SG_ERR_INVALID,
+           "sg_copy_device: pointers and size must be multiples of 16 bytes");
+  const size_t n16 = bytes / 16;
+  copy_kernel_u128<<<sg::grid_for(dev, (int64_t)n16, 256, 8), 256, 0, dev->stream>>>((const uint4*)src, (uint4*)dst, n16);
+  SG_HIP(hipGetLastError());
   SG_HIP(hipStreamSynchronize(dev->stream));
   return SG_OK;
 }

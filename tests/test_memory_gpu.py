@@ -37,3 +37,13 @@ def test_device_space_roundtrip(sg, device):
     b.fill(7, 8, 10)
     y = b.download(np.uint64)
     assert (y[:10] == 7).all() and np.array_equal(y[10:], x[10:])
+
+
+def test_copy_device_kernel(sg, device):
+    x = np.random.default_rng(3).integers(0, 2**32, size=(1 << 22) + 4, dtype=np.uint32)
+    a = device.to_device(x)
+    b = device.alloc(x.nbytes)
+    sg.check(sg.lib().sg_copy_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), x.nbytes))
+    assert np.array_equal(b.download(np.uint32), x)
+    # sizes and pointers must be 16-byte multiples
+    assert sg.lib().sg_copy_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), 20) == sg.SG_ERR_INVALID
