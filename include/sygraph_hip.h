@@ -241,8 +241,10 @@ typedef struct sg_level_stat {
   int64_t frontier_vertices; /* |F| entering the level */
   int64_t frontier_edges;    /* sum deg over F */
   int64_t discovered;        /* vertices discovered at this level */
-  int64_t edges_inspected;   /* arcs read (push: sum deg F; pull: sum k_u) */
+  int64_t edges_inspected;   /* bitmap probes (push: sum deg F; pull: one hint probe per scanned
+                                vertex plus k_u for each hint miss) */
   int64_t vertices_scanned;  /* pull: unvisited vertices with degree > 0 */
+  int64_t hint_misses;       /* pull: scanned vertices whose hint in-neighbour was not in the frontier */
   int64_t unexplored_edges;  /* sum deg over unvisited after the level */
   int64_t max_degree;        /* largest degree among the discovered vertices */
   float kernel_ms;           /* device time of the level's advance kernel (record_levels) */

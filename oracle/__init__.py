@@ -73,7 +73,9 @@ def lib():
         L.or_bc.restype = None
         L.or_bc.argtypes = [C.c_int64, _i64p, _i32p, C.c_void_p, C.c_void_p, C.c_int32, _i32p, _f64p]
         L.or_pull_scan_count.restype = C.c_int64
-        L.or_pull_scan_count.argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int32, C.POINTER(C.c_int64)]
+        L.or_pull_scan_count.argtypes = [C.c_int64, _i64p, _i32p, _i32p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_pull_hint.restype = None
+        L.or_pull_hint.argtypes = [C.c_int64, _i64p, _i32p, _i64p, _i32p]
         L.or_bfs_plain_omp.restype = C.c_int32
         L.or_bfs_plain_omp.argtypes = [C.c_int64, _i64p, _i32p, C.c_int32, C.c_int, C.c_float, C.c_float, C.c_int,
                                        _i32p, _i32p, C.c_int32]
@@ -244,11 +246,21 @@ def bc(g: CSRGraph, sources, inverse: Optional[CSRGraph] = None) -> np.ndarray:
     return out
 
 
-def pull_scan_count(g: CSRGraph, depth: np.ndarray, level: int) -> tuple[int, int]:
-    cnt = C.c_int64(0)
+def pull_hint(g: CSRGraph, inverse: Optional[CSRGraph] = None) -> np.ndarray:
+    """Highest out-degree in-neighbour of every vertex (lowest id on ties, -1 if none)."""
+    inv = inverse if inverse is not None else g
+    out = np.empty(g.n, dtype=np.int32)
+    lib().or_pull_hint(g.n, inv.offsets, inv.cols, g.offsets, out)
+    return out
+
+
+def pull_scan_count(g: CSRGraph, depth: np.ndarray, level: int, hint: Optional[np.ndarray] = None):
+    """(probes, scanned vertices, hint misses) of one pull level; see or_pull_scan_count."""
+    cnt, missed = C.c_int64(0), C.c_int64(0)
+    h = np.ascontiguousarray(hint, dtype=np.int32) if hint is not None else None
     tot = lib().or_pull_scan_count(g.n, g.offsets, g.cols, np.ascontiguousarray(depth, dtype=np.int32), level,
-                                   C.byref(cnt))
-    return int(tot), int(cnt.value)
+                                   h.ctypes.data if h is not None else None, C.byref(cnt), C.byref(missed))
+    return int(tot), int(cnt.value), int(missed.value)
 
 
 BFS_PUSH, BFS_PULL, BFS_HYBRID = 0, 1, 2

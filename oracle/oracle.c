@@ -496,14 +496,24 @@ void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t
   free(scale);
 }
 
+/* Bitmap probes of one pull level (the byte model of the pull kernel).  Each
+ * unvisited vertex u with in-arcs first probes hint[u] (when hint != NULL);
+ * on a miss it scans its in-list in CSR order up to the first in-neighbour in
+ * the frontier (depth == level), k_u probes, or the whole list. */
 int64_t or_pull_scan_count(int64_t n, const int64_t* offsets, const int32_t* cols, const int32_t* depth, int32_t level,
-                           int64_t* n_scanned) {
-  int64_t total = 0, cnt = 0;
-#pragma omp parallel for reduction(+ : total, cnt) schedule(static)
+                           const int32_t* hint, int64_t* n_scanned, int64_t* n_missed) {
+  int64_t total = 0, cnt = 0, missed = 0;
+#pragma omp parallel for reduction(+ : total, cnt, missed) schedule(static)
   for (int64_t u = 0; u < n; ++u) {
     if (depth[u] <= level) continue;
     int64_t b = offsets[u], e = offsets[u + 1];
     if (e == b) continue;
+    cnt += 1;
+    if (hint) {
+      total += 1;
+      if (depth[hint[u]] == level) continue;
+      missed += 1;
+    }
     int64_t k = e - b;
     for (int64_t j = b; j < e; ++j)
       if (depth[cols[j]] == level) {
@@ -511,8 +521,28 @@ int64_t or_pull_scan_count(int64_t n, const int64_t* offsets, const int32_t* col
         break;
       }
     total += k;
-    cnt += 1;
   }
   if (n_scanned) *n_scanned = cnt;
+  if (n_missed) *n_missed = missed;
   return total;
+}
+
+/* hint[v] = the in-neighbour of v with the largest out-degree, the lowest id
+ * on ties; -1 without in-arcs. */
+void or_pull_hint(int64_t n, const int64_t* in_offsets, const int32_t* in_cols, const int64_t* out_offsets,
+                  int32_t* hint) {
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t v = 0; v < n; ++v) {
+    int32_t best = -1;
+    int64_t best_deg = -1;
+    for (int64_t j = in_offsets[v]; j < in_offsets[v + 1]; ++j) {
+      const int32_t u = in_cols[j];
+      const int64_t d = out_offsets[u + 1] - out_offsets[u];
+      if (d > best_deg || (d == best_deg && u < best)) {
+        best = u;
+        best_deg = d;
+      }
+    }
+    hint[v] = best;
+  }
 }

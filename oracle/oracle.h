@@ -61,7 +61,9 @@ void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t
  * (unvisited at that level) and degree > 0, k_u = 1 + position of the first neighbour
  * with depth == level, or deg(u) if none.  Returns sum k_u; *n_scanned = #such u. */
 int64_t or_pull_scan_count(int64_t n, const int64_t* offsets, const int32_t* cols, const int32_t* depth, int32_t level,
-                           int64_t* n_scanned);
+                           const int32_t* hint, int64_t* n_scanned, int64_t* n_missed);
+void or_pull_hint(int64_t n, const int64_t* in_offsets, const int32_t* in_cols, const int64_t* out_offsets,
+                  int32_t* hint);
 
 /* ---- CPU baseline: OpenMP restatement of the reference's plain method ------- */
 /* 2-level bitmap frontier, level-synchronous push (mode 0), pull (1) or the

@@ -73,7 +73,8 @@ class BfsOptions(C.Structure):
 class LevelStat(C.Structure):
     _fields_ = [("direction", C.c_int32), ("load_balancer", C.c_int32), ("frontier_vertices", C.c_int64),
                 ("frontier_edges", C.c_int64), ("discovered", C.c_int64), ("edges_inspected", C.c_int64),
-                ("vertices_scanned", C.c_int64), ("unexplored_edges", C.c_int64), ("max_degree", C.c_int64),
+                ("vertices_scanned", C.c_int64), ("hint_misses", C.c_int64), ("unexplored_edges", C.c_int64),
+                ("max_degree", C.c_int64),
                 ("kernel_ms", C.c_float), ("convert_ms", C.c_float)]
 
 

@@ -46,6 +46,7 @@ struct BfsArgs {
   const int32_t* cols;
   const OffT* in_off; /* inverse CSR (pull); aliases off when undirected */
   const int32_t* in_cols;
+  const int32_t* hint; /* per vertex: in-neighbour of largest out-degree */
   int32_t directed;
   int64_t n;
   int64_t w0;
@@ -81,7 +82,7 @@ template<typename OffT>
 __global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
   zero_slots(a.stats_zero);
   __shared__ unsigned long long lds_new[kBlock];
-  uint64_t st_disc = 0, st_deg = 0, st_edges = 0, st_scanned = 0, st_max = 0;
+  uint64_t st_disc = 0, st_deg = 0, st_edges = 0, st_scanned = 0, st_max = 0, st_miss = 0;
   const int lane = lane_id();
   unsigned long long* wnew = lds_new + (threadIdx.x & ~63);
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -111,14 +112,22 @@ __global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
       const bool active = t < total;
       const uint32_t bit = active ? select_bit(un, t - before) : 0u;
       const int64_t v = (base + owner) * 64 + bit;
-      bool found = false;
+      bool found = false, miss = false;
       OffT b = 0, e = 0, j = 0;
       int64_t kk = 0;
       if (active) {
         b = a.in_off[v];
         e = a.in_off[v + 1];
-        j = b;
-        kk = (int64_t)(e - b);
+        /* first probe: the in-neighbour of largest out-degree, read from a
+         * contiguous array instead of the adjacency list's cache line */
+        if (test_bit(a.cur_bm, (uint32_t)a.hint[v])) {
+          found = true;
+          j = e;
+        } else {
+          miss = true;
+          j = b;
+          kk = (int64_t)(e - b);
+        }
         /* lane-serial head: four independent id loads then four independent
          * bitmap probes per step */
         for (int step = 0; step < kPullHead / 4 && j < e; ++step, j += 4) {
@@ -163,7 +172,8 @@ __global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
       if (active) {
         const int64_t deg = (int64_t)(e - b);
         if (deg > 0) ++st_scanned;
-        st_edges += kk;
+        st_edges += 1 + kk; /* hint probe + list probes after a miss */
+        st_miss += miss ? 1 : 0;
         if (found) {
           a.depth[v] = a.level + 1;
           atomicOr(&wnew[owner], 1ull << bit);
@@ -182,7 +192,7 @@ __global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   }
-  flush_block_stats(a.stats, st_disc, st_deg, st_edges, st_scanned, st_max);
+  flush_block_stats(a.stats, st_disc, st_deg, st_edges, st_scanned, st_max, st_miss);
 }
 
 /* Level-0 setup: depth[src] = 0, visited = prev = isolated | {src},
@@ -267,7 +277,7 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
 }
 
 struct HostStats {
-  uint64_t disc = 0, disc_deg = 0, edges = 0, scanned = 0, max_deg = 0;
+  uint64_t disc = 0, disc_deg = 0, edges = 0, scanned = 0, max_deg = 0, misses = 0;
 };
 
 } // namespace
@@ -288,6 +298,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   BfsWorkspace* ws = nullptr;
   SG_TRY(workspace_for(g, &ws));
   SG_TRY(sg::graph_ensure_isolated(g));
+  if (opt.policy != SG_POLICY_PUSH) SG_TRY(sg::graph_ensure_pull_hint(g));
   const int64_t n = g->n, w0 = ws->w0;
   const int grid_main = dev->cus * 8;
   const int grid_small = std::max(1, std::min(dev->cus * 4, (int)((w0 + 255) / 256)));
@@ -300,6 +311,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   a.cols = g->cols;
   a.in_off = (const OffT*)g->inv_offsets;
   a.in_cols = g->inv_cols;
+  a.hint = g->pull_hint;
   a.directed = (g->flags & SG_GRAPH_DIRECTED) ? 1 : 0;
   a.n = n;
   a.w0 = w0;
@@ -371,6 +383,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
         c.disc_deg += hslots[i].disc_deg;
         c.edges += hslots[i].edges;
         c.scanned += hslots[i].scanned;
+        c.misses += hslots[i].misses;
         c.max_deg = std::max<uint64_t>(c.max_deg, hslots[i].max_deg);
       }
       ++n_pull;
@@ -386,6 +399,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       ls.discovered = (int64_t)c.disc;
       ls.edges_inspected = (int64_t)c.edges;
       ls.vertices_scanned = (int64_t)c.scanned;
+      ls.hint_misses = (int64_t)c.misses;
       ls.max_degree = (int64_t)c.max_deg;
       ls.kernel_ms = ms;
     }

@@ -216,6 +216,51 @@ __global__ void isolated_kernel(const OffT* offsets, int64_t n, int64_t w0, uint
   }
 }
 
+/* hint[v] = in-neighbour of v with the largest out-degree (lowest id on
+ * ties), -1 without in-arcs.  A wave takes 64 consecutive vertices; lists
+ * longer than 64 arcs are reduced by the whole wave.  Candidates are ranked
+ * by the key (degree << 32) | ~id, so the maximum key is the answer. */
+template<typename OffT>
+__global__ void __launch_bounds__(256) pull_hint_kernel(const OffT* in_off, const int32_t* in_cols, const OffT* out_off,
+                                                        int64_t n, int32_t* hint) {
+  const int lane = lane_id();
+  const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  auto key_of = [&](int32_t u) -> uint64_t {
+    const uint64_t d = (uint64_t)(out_off[u + 1] - out_off[u]);
+    return (d << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)u);
+  };
+  for (int64_t base = gwave * 64; base < n; base += nwaves * 64) {
+    const int64_t v = base + lane;
+    int64_t b = 0, e = 0;
+    if (v < n) {
+      b = (int64_t)in_off[v];
+      e = (int64_t)in_off[v + 1];
+    }
+    const bool coop = e - b > 64;
+    uint64_t best = 0;
+    if (!coop)
+      for (int64_t j = b; j < e; ++j) {
+        const uint64_t k = key_of(in_cols[j]);
+        best = k > best ? k : best;
+      }
+    uint64_t heavy = ballot(coop);
+    while (heavy) {
+      const int k2 = __ffsll((unsigned long long)heavy) - 1;
+      heavy &= heavy - 1;
+      const int64_t kb = (int64_t)__shfl((long long)b, k2, 64), ke = (int64_t)__shfl((long long)e, k2, 64);
+      uint64_t part = 0;
+      for (int64_t j = kb + lane; j < ke; j += 64) {
+        const uint64_t k = key_of(in_cols[j]);
+        part = k > part ? k : part;
+      }
+      part = wave_max_u64(part);
+      if (lane == k2) best = part;
+    }
+    if (v < n) hint[v] = e > b ? (int32_t)(0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFull)) : -1;
+  }
+}
+
 template<typename T>
 __global__ void digest_kernel(const T* x, int64_t n, unsigned long long* out) {
   uint64_t acc = 0;
@@ -261,6 +306,8 @@ void free_graph_arrays(sg_graph_t g) {
     (void)hipFree(g->inv_weights);
   }
   (void)hipFree(g->isolated);
+  (void)hipFree(g->pull_hint);
+  g->pull_hint = nullptr;
   g->offsets = g->cols = nullptr;
   g->weights = g->inv_offsets = g->inv_weights = g->isolated = nullptr;
   g->inv_cols = nullptr;
@@ -401,6 +448,21 @@ int graph_ensure_isolated(sg_graph_t g) {
     isolated_kernel<int64_t><<<grid, 256, 0, g->dev->stream>>>((const int64_t*)g->inv_offsets, g->n, w0, g->isolated);
   else
     isolated_kernel<uint32_t><<<grid, 256, 0, g->dev->stream>>>((const uint32_t*)g->inv_offsets, g->n, w0, g->isolated);
+  SG_HIP(hipGetLastError());
+  return SG_OK;
+}
+int graph_ensure_pull_hint(sg_graph_t g) {
+  if (g->pull_hint) return SG_OK;
+  SG_TRY(alloc_device(g->dev, (void**)&g->pull_hint, 4 * (g->n + 1)));
+  const int grid = grid_for(g->dev, (g->n + 3) / 4, 256);
+  const void* in_off = g->inv_offsets ? g->inv_offsets : g->offsets;
+  const int32_t* in_cols = g->inv_cols ? g->inv_cols : g->cols;
+  if (g->offset_bytes == 8)
+    pull_hint_kernel<int64_t><<<grid, 256, 0, g->dev->stream>>>((const int64_t*)in_off, in_cols,
+                                                                (const int64_t*)g->offsets, g->n, g->pull_hint);
+  else
+    pull_hint_kernel<uint32_t><<<grid, 256, 0, g->dev->stream>>>((const uint32_t*)in_off, in_cols,
+                                                                 (const uint32_t*)g->offsets, g->n, g->pull_hint);
   SG_HIP(hipGetLastError());
   return SG_OK;
 }

@@ -106,6 +106,7 @@ struct sg_graph_s {
   bool owns_inverse = false;
   /* lazily built helpers */
   uint64_t* isolated = nullptr; /* bit set for vertices with in-degree 0 */
+  int32_t* pull_hint = nullptr;  /* per vertex: in-neighbour of largest out-degree (pull probes it first) */
   int64_t total_degree = 0;
   void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */
   void* sssp_ws = nullptr;
@@ -136,6 +137,7 @@ int zero_async(sg_device_t dev, void* p, size_t bytes);
 
 /* graph helpers */
 int graph_ensure_isolated(sg_graph_t g);
+int graph_ensure_pull_hint(sg_graph_t g);
 void bfs_workspace_release(sg_graph_t g);
 void sssp_workspace_release(sg_graph_t g);
 

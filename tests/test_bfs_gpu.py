@@ -116,14 +116,15 @@ def test_rmat14_level_counters_match_oracle(sg, oracle_mod, rmat14):
     g = oracle_mod.CSRGraph(G.n, off, cols)
     src = int(z["source"])
     ref = z["depth"]
+    hint = oracle_mod.pull_hint(g)
     for policy in ["push", "pull"]:
         _, st, _ = _run(sg, G, src, policy)
         for L in range(st.levels):
             ls = st.level[L]
             assert ls.discovered == int((ref == L + 1).sum())
             if policy == "pull":
-                tot, cnt = oracle_mod.pull_scan_count(g, ref, L)
-                assert (ls.edges_inspected, ls.vertices_scanned) == (tot, cnt)
+                tot, cnt, miss = oracle_mod.pull_scan_count(g, ref, L, hint)
+                assert (ls.edges_inspected, ls.vertices_scanned, ls.hint_misses) == (tot, cnt, miss)
             else:
                 fr = ref == L
                 assert ls.frontier_vertices == int(fr.sum())

@@ -287,22 +287,46 @@ def test_hybrid_schedule_line5(oracle_mod):
     assert d.tolist() == [0, 1, 2, 3, 4] and lv == 5 and len(dirs) == 5
 
 
+def test_pull_hint_brute_force(oracle_mod):
+    g = oracle_mod.rmat(9, 8, seed=3)
+    deg = g.degrees()
+    h = oracle_mod.pull_hint(g)
+    for v in range(g.n):
+        nb = g.cols[g.offsets[v]:g.offsets[v + 1]]
+        if len(nb) == 0:
+            assert h[v] == -1
+            continue
+        best = max(nb.tolist(), key=lambda u: (deg[u], -u))  # largest degree, lowest id on ties
+        assert h[v] == best
+    # directed: hints come from in-arcs and rank by out-degree
+    t = oracle_mod.transpose(oracle_mod.CSRGraph(3, np.array([0, 2, 3, 3]), np.array([1, 2, 2], dtype=np.int32)))
+    hd = oracle_mod.pull_hint(oracle_mod.CSRGraph(3, np.array([0, 2, 3, 3]), np.array([1, 2, 2], dtype=np.int32)), t)
+    assert hd.tolist() == [-1, 0, 0]
+
+
 def test_pull_scan_count_brute_force(oracle_mod):
     g = oracle_mod.rmat(8, 8, seed=3)
     d, lv = oracle_mod.bfs(g, int(np.argmax(g.degrees())))
+    hint = oracle_mod.pull_hint(g)
     for level in range(lv):
-        tot, cnt = oracle_mod.pull_scan_count(g, d, level)
-        t2 = c2 = 0
-        for u in range(g.n):
-            if d[u] <= level:
-                continue
-            nb = g.cols[g.offsets[u]:g.offsets[u + 1]]
-            if len(nb) == 0:
-                continue
-            hit = np.nonzero(d[nb] == level)[0]
-            t2 += (hit[0] + 1) if len(hit) else len(nb)
-            c2 += 1
-        assert (tot, cnt) == (t2, c2)
+        for use_hint in (False, True):
+            tot, cnt, miss = oracle_mod.pull_scan_count(g, d, level, hint if use_hint else None)
+            t2 = c2 = m2 = 0
+            for u in range(g.n):
+                if d[u] <= level:
+                    continue
+                nb = g.cols[g.offsets[u]:g.offsets[u + 1]]
+                if len(nb) == 0:
+                    continue
+                c2 += 1
+                if use_hint:
+                    t2 += 1
+                    if d[hint[u]] == level:
+                        continue
+                    m2 += 1
+                hit = np.nonzero(d[nb] == level)[0]
+                t2 += (hit[0] + 1) if len(hit) else len(nb)
+            assert (tot, cnt, miss) == (t2, c2, m2)
 
 
 # ----------------------------------------------------- committed golden vectors
