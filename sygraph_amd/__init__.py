@@ -484,7 +484,7 @@ def swap(a: Frontier, b: Frontier):
 
 # -------------------------------------------------------------- algorithms --
 def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, alpha: float = 15.0,
-                beta: float = 24.0, record_levels: bool = False) -> BfsOptions:
+                beta: float = 24.0, record_levels: bool = False, sync_per_level: bool = False) -> BfsOptions:
     o = BfsOptions()
     lib().sg_bfs_options_default(C.byref(o))
     o.policy = policy
@@ -492,6 +492,7 @@ def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, al
     o.alpha = alpha
     o.beta = beta
     o.record_levels = 1 if record_levels else 0
+    o.sync_per_level = 1 if sync_per_level else 0
     return o
 
 
@@ -514,10 +515,10 @@ class BFS:
         self.source = None
 
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
-            load_balancer: int = LB_BUCKETING) -> BfsStats:
+            load_balancer: int = LB_BUCKETING, sync_per_level: bool = False) -> BfsStats:
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
-        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True)
+        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level)
         check(lib().sg_bfs(self.g.h, self.source, C.byref(o), C.c_void_p(self.depth.ptr), C.byref(self.stats)))
         return self.stats
 

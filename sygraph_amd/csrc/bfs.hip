@@ -69,7 +69,7 @@ __device__ __forceinline__ void push_visit(const BfsArgs<OffT>& a, uint32_t y) {
 /* Push level: every arc of the class queues; discovery is the test-and-set of
  * the visited bit. */
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) push_kernel(BfsArgs<OffT> a) {
+__device__ __forceinline__ void push_level(const BfsArgs<OffT>& a) {
   zero_slots(a.stats_zero);
   for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols,
                             [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y); });
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock) push_kernel(BfsArgs<OffT> a) {
  * unvisited vertices in registers (popcount scan + bit select), so all 64 lanes
  * pull for real vertices; new frontier bits are gathered per word in LDS. */
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
+__device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a) {
   zero_slots(a.stats_zero);
   __shared__ unsigned long long lds_new[kBlock];
   uint64_t st_disc = 0, st_deg = 0, st_edges = 0, st_scanned = 0, st_max = 0, st_miss = 0;
@@ -195,11 +195,55 @@ __global__ void __launch_bounds__(kBlock) pull_kernel(BfsArgs<OffT> a) {
   flush_block_stats(a.stats, st_disc, st_deg, st_edges, st_scanned, st_max, st_miss);
 }
 
+/* Device-resident control block of the level loop.  Every kernel of a level
+ * reads it (kernel boundaries order the accesses); only decide_kernel and
+ * finish_kernel write it. */
+struct BfsCtl {
+  int32_t dir;      /* direction of the level about to run: 0 push, 1 pull */
+  int32_t done;     /* a level discovered nothing: later levels are no-ops */
+  int32_t mode;     /* count pass of this level: 0 none, 1 visited & ~prev, 2 frontier bitmap + prev sync */
+  int32_t write;    /* queues for the next (push) level must be written */
+  int32_t next_dir; /* decided after a pull level by decide_kernel */
+  int32_t pad;
+  int64_t f_vertices, f_edges; /* frontier entering the level */
+  unsigned long long visited_deg, reached;
+  unsigned long long disc, disc_deg, edges, scanned, misses, max_deg; /* staged pull-level statistics */
+};
+
+/* One level's record, written by finish_kernel into a ring in pinned host
+ * memory; the host reads it after the level's completion event. */
+struct alignas(64) LevelLog {
+  int32_t level, dir, done, next_dir;
+  int64_t f_vertices, f_edges, disc, disc_deg, edges, scanned, misses, max_deg, unexplored;
+};
+
+constexpr int kRing = 8; /* levels in the log / event rings (> lookahead + 1) */
+
+struct DecideParams {
+  int32_t policy;
+  float alpha, beta;
+  int64_t n;
+  unsigned long long total_degree;
+};
+
+/* evaluateHeuristic (algorithms/bfs.hpp:341-350): push -> pull when the new
+ * frontier's arcs exceed the unexplored arcs / alpha (both cast to uint32 and
+ * compared as float); pull -> push when the frontier is smaller than n / beta. */
+__device__ __forceinline__ int32_t next_direction(const DecideParams& p, int32_t dir, unsigned long long disc,
+                                                  unsigned long long disc_deg, unsigned long long unexplored) {
+  if (p.policy != SG_POLICY_HYBRID) return dir;
+  if (dir == 0) {
+    const uint32_t fdeg = (uint32_t)disc_deg, udeg = (uint32_t)unexplored;
+    return ((float)fdeg > ((float)udeg / p.alpha)) ? 1 : 0;
+  }
+  return ((float)disc < ((float)p.n / p.beta)) ? 0 : 1;
+}
+
 /* Level-0 setup: depth[src] = 0, visited = prev = isolated | {src},
- * cur_bm = {src}, F0 queue = {src}. */
+ * cur_bm = {src}, F0 queue = {src}, control block for level 0. */
 template<typename OffT>
 __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, QEntry* q1, QEntry* q2, QueueHdr* hdr,
-                                   Tuning t, int32_t src) {
+                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t bit = 1ull << (src & 63);
   a.depth[src] = 0;
@@ -208,6 +252,121 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
   a.cur_bm[src >> 6] |= bit;
   QEntry* const q[3] = {q0, q1, q2};
   enqueue_single(a.off, t, q, hdr, src);
+  const unsigned long long d = (unsigned long long)(a.off[src + 1] - a.off[src]);
+  BfsCtl c{};
+  c.dir = first_dir;
+  c.f_vertices = 1;
+  c.f_edges = (int64_t)d;
+  c.visited_deg = d; /* the source's arcs count as explored */
+  c.reached = 1;
+  *ctl = c;
+}
+
+/* The traversal of one level, push or pull as the control block says. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
+  if (ctl->done) return;
+  if (ctl->dir == 0) push_level(a);
+  else pull_level(a);
+}
+
+/* After the traversal: a pull level's statistics are reduced from its slots and
+ * the next direction decided now, because the count pass depends on it (queues
+ * are only built when the next level pushes); a push level is counted first. */
+__global__ void decide_kernel(BfsCtl* ctl, const StatSlot* slots, DecideParams p) {
+  if (ctl->done) return;
+  if (ctl->dir == 0) {
+    if (threadIdx.x == 0) ctl->mode = 1;
+    return;
+  }
+  const StatSlot& x = slots[threadIdx.x]; /* 64 threads, one slot each */
+  const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
+  const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
+  const unsigned long long misses = wave_sum_u64(x.misses), max_deg = wave_max_u64(x.max_deg);
+  if (threadIdx.x == 0) {
+    const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
+    const int32_t next = next_direction(p, 1, disc, disc_deg, unexplored);
+    ctl->disc = disc;
+    ctl->disc_deg = disc_deg;
+    ctl->edges = edges;
+    ctl->scanned = scanned;
+    ctl->misses = misses;
+    ctl->max_deg = max_deg;
+    ctl->next_dir = next;
+    ctl->mode = (disc > 0 && next == 0) ? 2 : 0;
+  }
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, const BfsCtl* ctl) {
+  if (ctl->done) return;
+  const int32_t mode = ctl->mode;
+  if (mode == 1) compact_count_body<OffT, kFromDiff>(a);
+  else if (mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
+}
+
+/* Scan of the count pass, the push-level decision, the level record, and the
+ * control block for the next level. */
+__global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, int64_t ntiles, QueueHdr* hdr_out,
+                                                              BfsCtl* ctl, DecideParams p, int32_t level,
+                                                              LevelLog* log) {
+  __shared__ QueueHdr tot;
+  if (ctl->done) return;
+  const int32_t mode = ctl->mode;
+  if (mode != 0) compact_scan_body(tiles, ntiles, hdr_out, &tot);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int32_t dir = ctl->dir;
+  unsigned long long disc, disc_deg, edges, scanned, misses, max_deg;
+  int32_t next;
+  const unsigned long long vdeg_before = ctl->visited_deg;
+  if (dir == 0) {
+    disc = tot.nvert;
+    disc_deg = tot.deg;
+    max_deg = tot.maxdeg;
+    edges = (unsigned long long)ctl->f_edges;
+    scanned = (unsigned long long)ctl->f_vertices;
+    misses = 0;
+    next = next_direction(p, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg));
+  } else {
+    disc = ctl->disc;
+    disc_deg = ctl->disc_deg;
+    max_deg = ctl->max_deg;
+    edges = ctl->edges;
+    scanned = ctl->scanned;
+    misses = ctl->misses;
+    next = ctl->next_dir;
+  }
+  const unsigned long long vdeg = vdeg_before + disc_deg;
+  LevelLog e;
+  e.level = level;
+  e.dir = dir;
+  e.done = disc == 0 ? 1 : 0;
+  e.next_dir = next;
+  e.f_vertices = ctl->f_vertices;
+  e.f_edges = ctl->f_edges;
+  e.disc = (int64_t)disc;
+  e.disc_deg = (int64_t)disc_deg;
+  e.edges = (int64_t)edges;
+  e.scanned = (int64_t)scanned;
+  e.misses = (int64_t)misses;
+  e.max_deg = (int64_t)max_deg;
+  e.unexplored = (int64_t)(p.total_degree - vdeg);
+  log[level % kRing] = e;
+  __threadfence_system();
+  ctl->visited_deg = vdeg;
+  ctl->reached += disc;
+  ctl->f_vertices = (int64_t)disc;
+  ctl->f_edges = (int64_t)disc_deg;
+  ctl->write = (disc > 0 && next == 0) ? 1 : 0;
+  ctl->done = disc == 0 ? 1 : 0;
+  ctl->dir = next;
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) write_dyn_kernel(CompactArgs<OffT> a, const BfsCtl* ctl) {
+  if (!ctl->write) return;
+  compact_write_body<OffT>(a);
 }
 
 __global__ void copy_words_kernel(uint64_t* dst, uint64_t* dst2, const uint64_t* src, int64_t nw) {
@@ -230,7 +389,10 @@ struct BfsWorkspace {
   uint64_t* bm[2] = {nullptr, nullptr};
   QueueSet qs;
   StatSlot* stats = nullptr; /* 2 areas of kSlots */
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  BfsCtl* ctl = nullptr;
+  LevelLog* log = nullptr; /* kRing entries, pinned coherent host memory */
+  hipEvent_t ev_start[kRing] = {}, ev_mid[kRing] = {}, ev_done[kRing] = {};
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
 void ws_free(BfsWorkspace* w) {
@@ -241,8 +403,13 @@ void ws_free(BfsWorkspace* w) {
   (void)hipFree(w->bm[1]);
   w->qs.release();
   (void)hipFree(w->stats);
-  for (auto e : w->ev)
-    if (e) (void)hipEventDestroy(e);
+  (void)hipFree(w->ctl);
+  if (w->log) (void)hipHostFree(w->log);
+  for (int r = 0; r < kRing; ++r)
+    for (hipEvent_t e : {w->ev_start[r], w->ev_mid[r], w->ev_done[r]})
+      if (e) (void)hipEventDestroy(e);
+  if (w->ev_begin) (void)hipEventDestroy(w->ev_begin);
+  if (w->ev_end) (void)hipEventDestroy(w->ev_end);
   delete w;
 }
 
@@ -260,15 +427,19 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
       (st = sg::alloc_device(dev, (void**)&w->bm[0], 8 * (w->w0 + 1))) ||
       (st = sg::alloc_device(dev, (void**)&w->bm[1], 8 * (w->w0 + 1))) ||
       (st = sg::alloc_device(dev, (void**)&w->stats, sizeof(StatSlot) * kSlots * 2)) ||
+      (st = sg::alloc_device(dev, (void**)&w->ctl, sizeof(BfsCtl))) ||
       (st = w->qs.init(dev, g->n, g->m, default_tuning()))) {
     ws_free(w);
     return st;
   }
-  bool ok = true;
-  for (auto& e : w->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  bool ok = hipHostMalloc((void**)&w->log, sizeof(LevelLog) * kRing, hipHostMallocCoherent) == hipSuccess;
+  for (int r = 0; r < kRing && ok; ++r)
+    ok = hipEventCreate(&w->ev_start[r]) == hipSuccess && hipEventCreate(&w->ev_mid[r]) == hipSuccess &&
+         hipEventCreate(&w->ev_done[r]) == hipSuccess;
+  ok = ok && hipEventCreate(&w->ev_begin) == hipSuccess && hipEventCreate(&w->ev_end) == hipSuccess;
   if (!ok) {
     ws_free(w);
-    sg::set_error("bfs workspace: event creation failed");
+    sg::set_error("bfs workspace: event or pinned log allocation failed");
     return SG_ERR_HIP;
   }
   g->bfs_ws = w;
@@ -276,9 +447,6 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
   return SG_OK;
 }
 
-struct HostStats {
-  uint64_t disc = 0, disc_deg = 0, edges = 0, scanned = 0, max_deg = 0, misses = 0;
-};
 
 } // namespace
 
@@ -291,6 +459,13 @@ void bfs_workspace_release(sg_graph_t g) {
 }
 } // namespace sg
 
+/* Host side of the level loop.  Each level is a fixed sequence of five
+ * launches (traverse, decide, count, scan+finish, write) whose behaviour the
+ * device-resident control block selects, so the host never has to decide
+ * anything between levels: it keeps `lookahead` levels enqueued beyond the one
+ * it waits for and only reads each level's record (pinned host memory) after
+ * that level's completion event, to stop after the first empty level.  Levels
+ * enqueued beyond the last one find `done` set and return at once. */
 template<typename OffT>
 int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats) {
   sg_device_t dev = g->dev;
@@ -317,148 +492,109 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   a.w0 = w0;
   a.visited = ws->visited;
   a.depth = depth;
-  int cb = 0; /* bitmap holding the current frontier */
-  int qs = 0; /* queue set holding the current push frontier */
+  a.cur_bm = ws->bm[0];
 
   fill_i32_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(depth, (int32_t)(n + 1), n);
   copy_words_kernel<<<grid_small, 256, 0, s>>>(ws->visited, ws->prev, g->isolated, w0);
   SG_TRY(sg::zero_async(dev, ws->bm[0], 8 * w0));
   SG_TRY(sg::zero_async(dev, ws->stats, sizeof(StatSlot) * kSlots * 2));
-  a.cur_bm = ws->bm[0];
-  init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src);
+  init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
+                                            opt.policy == SG_POLICY_PULL ? 1 : 0);
   SG_HIP(hipGetLastError());
 
-  const bool want_events = opt.record_levels != 0 || stats != nullptr;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-  if (want_events) {
-    SG_HIP(hipEventCreate(&ev_begin));
-    SG_HIP(hipEventCreate(&ev_end));
-    SG_HIP(hipEventRecord(ev_begin, s));
-  }
+  const bool timed = opt.record_levels && stats;
+  const bool want_total = stats != nullptr;
+  const int lookahead = opt.sync_per_level ? 0 : 1;
+  DecideParams P;
+  P.policy = opt.policy;
+  P.alpha = opt.alpha;
+  P.beta = opt.beta;
+  P.n = n;
+  P.total_degree = (unsigned long long)g->total_degree;
+  if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 
-  bool push = opt.policy != SG_POLICY_PULL;
-  int32_t level = 0;
-  int32_t n_push = 0, n_pull = 0;
-  QueueHdr* hh = (QueueHdr*)dev->host_scratch;
-  StatSlot* hslots = (StatSlot*)((char*)dev->host_scratch + 4096);
-  SG_HIP(hipMemcpyAsync(hh, Q.hdr, sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
-  SG_HIP(hipStreamSynchronize(s));
-  uint64_t visited_deg = hh->deg; /* the source's arcs count as explored */
-  int64_t f_vertices = 1, f_edges = (int64_t)hh->deg;
-  int64_t reached_v = 1;
-  for (;; ++level) {
-    a.level = level;
-    a.stats = ws->stats + kSlots * (level & 1);
-    a.stats_zero = ws->stats + kSlots * ((level + 1) & 1);
-    for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[qs][k];
-    a.qin_hdr = Q.hdr + qs;
-    a.cur_bm = ws->bm[cb];
-    a.next_bm = ws->bm[cb ^ 1];
-    const bool timed = opt.record_levels && stats && level < SG_MAX_LEVELS_RECORDED;
-    if (timed) SG_HIP(hipEventRecord(ws->ev[0], s));
-    HostStats c;
-    if (push) {
-      push_kernel<OffT><<<grid_main, kBlock, 0, s>>>(a);
-      if (timed) SG_HIP(hipEventRecord(ws->ev[1], s));
-      /* new frontier = visited & ~prev into bm[cb^1]; per-class counts and the
-       * degree statistics land in header qs^1 (queues written later if the
-       * next level pushes) */
-      SG_TRY((compact_to_queues<OffT, kFromDiff>(s, Q, qs ^ 1, off, ws->visited, ws->prev, a.next_bm, tune, true)));
-      SG_HIP(hipMemcpyAsync(hh, Q.hdr + (qs ^ 1), sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
+  auto enqueue = [&](int32_t L) -> int {
+    BfsArgs<OffT> la = a;
+    const int cur = L & 1, nxt = (L + 1) & 1, r = L % kRing;
+    la.level = L;
+    la.stats = ws->stats + kSlots * cur;
+    la.stats_zero = ws->stats + kSlots * nxt;
+    for (int k = 0; k < 3; ++k) la.qin[k] = Q.q[cur][k];
+    la.qin_hdr = Q.hdr + cur;
+    la.cur_bm = ws->bm[cur];
+    la.next_bm = ws->bm[nxt];
+    const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
+    if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
+    level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
+    if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
+    decide_kernel<<<1, 64, 0, s>>>(ws->ctl, la.stats, P);
+    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.tiles, Q.ntiles, Q.hdr + nxt, ws->ctl, P, L, ws->log);
+    write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipEventRecord(ws->ev_done[r], s));
+    return SG_OK;
+  };
+
+  SG_TRY(enqueue(0));
+  for (int k = 1; k <= lookahead; ++k) SG_TRY(enqueue(k));
+  int32_t levels = 0, n_push = 0, n_pull = 0;
+  int64_t reached_v = 1, reached_e = 0;
+  for (int32_t L = 0;; ++L) {
+    const int r = L % kRing;
+    SG_HIP(hipEventSynchronize(ws->ev_done[r]));
+    LevelLog e;
+    std::memcpy(&e, (const void*)&ws->log[r], sizeof(e));
+    if (e.level != L) {
       SG_HIP(hipStreamSynchronize(s));
-      c.disc = hh->nvert;
-      c.disc_deg = hh->deg;
-      c.max_deg = hh->maxdeg;
-      c.edges = (uint64_t)f_edges;
-      c.scanned = (uint64_t)f_vertices;
-      ++n_push;
-    } else {
-      pull_kernel<OffT><<<grid_main, kBlock, 0, s>>>(a);
-      if (timed) SG_HIP(hipEventRecord(ws->ev[1], s));
-      SG_HIP(hipGetLastError());
-      SG_HIP(hipMemcpyAsync(hslots, a.stats, sizeof(StatSlot) * kSlots, hipMemcpyDeviceToHost, s));
-      SG_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i < kSlots; ++i) {
-        c.disc += hslots[i].disc;
-        c.disc_deg += hslots[i].disc_deg;
-        c.edges += hslots[i].edges;
-        c.scanned += hslots[i].scanned;
-        c.misses += hslots[i].misses;
-        c.max_deg = std::max<uint64_t>(c.max_deg, hslots[i].max_deg);
-      }
-      ++n_pull;
+      sg::set_error("sg_bfs: level record %d out of order (found %d)", L, e.level);
+      return SG_ERR_STATE;
     }
-    if (timed) {
-      float ms = 0.f;
-      SG_HIP(hipEventElapsedTime(&ms, ws->ev[0], ws->ev[1]));
-      sg_level_stat& ls = stats->level[level];
-      ls.direction = push ? 0 : 1;
+    if (L == 0) reached_e = e.f_edges;
+    reached_v += e.disc;
+    reached_e += e.disc_deg;
+    if (e.dir == 0) ++n_push;
+    else ++n_pull;
+    if (stats && L < SG_MAX_LEVELS_RECORDED) {
+      sg_level_stat& ls = stats->level[L];
+      std::memset(&ls, 0, sizeof(ls));
+      ls.direction = e.dir;
       ls.load_balancer = opt.load_balancer;
-      ls.frontier_vertices = f_vertices;
-      ls.frontier_edges = f_edges;
-      ls.discovered = (int64_t)c.disc;
-      ls.edges_inspected = (int64_t)c.edges;
-      ls.vertices_scanned = (int64_t)c.scanned;
-      ls.hint_misses = (int64_t)c.misses;
-      ls.max_degree = (int64_t)c.max_deg;
-      ls.kernel_ms = ms;
+      ls.frontier_vertices = e.f_vertices;
+      ls.frontier_edges = e.f_edges;
+      ls.discovered = e.disc;
+      ls.edges_inspected = e.edges;
+      ls.vertices_scanned = e.scanned;
+      ls.hint_misses = e.misses;
+      ls.max_degree = e.max_deg;
+      ls.unexplored_edges = e.unexplored;
+      if (timed) {
+        SG_HIP(hipEventElapsedTime(&ls.kernel_ms, ws->ev_start[r], ws->ev_mid[r]));
+        SG_HIP(hipEventElapsedTime(&ls.convert_ms, ws->ev_mid[r], ws->ev_done[r]));
+      }
     }
-    visited_deg += c.disc_deg;
-    reached_v += (int64_t)c.disc;
-    const uint64_t unexplored = (uint64_t)g->total_degree - visited_deg;
-    if (timed) stats->level[level].unexplored_edges = (int64_t)unexplored;
-    f_vertices = (int64_t)c.disc;
-    f_edges = (int64_t)c.disc_deg;
-    if (c.disc == 0) {
-      ++level;
+    if (e.done) {
+      levels = L + 1;
       break;
     }
-    /* direction of the next level (evaluateHeuristic, bfs.hpp:341-350) */
-    bool next_push = push;
-    if (opt.policy == SG_POLICY_HYBRID) {
-      if (push) {
-        const uint32_t fdeg = (uint32_t)c.disc_deg, udeg = (uint32_t)unexplored;
-        if ((float)fdeg > ((float)udeg / opt.alpha)) next_push = false;
-      } else {
-        if ((float)c.disc < ((float)n / opt.beta)) next_push = true;
-      }
+    if (L + 1 >= n + 1) { /* more levels than vertices: the control block is corrupt */
+      SG_HIP(hipStreamSynchronize(s));
+      sg::set_error("sg_bfs: no termination after %lld levels", (long long)(L + 1));
+      return SG_ERR_STATE;
     }
-    if (timed) SG_HIP(hipEventRecord(ws->ev[2], s));
-    cb ^= 1; /* the new frontier is in bm[cb] now, after either direction */
-    if (next_push) {
-      if (push) {
-        SG_TRY(compact_write_only<OffT>(s, Q, qs ^ 1, off, ws->bm[cb], tune));
-      } else {
-        SG_TRY((compact_to_queues<OffT, kFromBitmapSyncPrev>(s, Q, qs ^ 1, off, ws->visited, ws->prev, ws->bm[cb],
-                                                              tune)));
-      }
-      qs ^= 1;
-    }
-    if (timed) {
-      SG_HIP(hipEventRecord(ws->ev[3], s));
-      SG_HIP(hipEventSynchronize(ws->ev[3]));
-      float ms = 0.f;
-      SG_HIP(hipEventElapsedTime(&ms, ws->ev[2], ws->ev[3]));
-      stats->level[level].convert_ms = ms;
-    }
-    push = next_push;
+    SG_TRY(enqueue(L + 1 + lookahead));
   }
-  if (want_events) {
-    SG_HIP(hipEventRecord(ev_end, s));
-    SG_HIP(hipEventSynchronize(ev_end));
-    float ms = 0.f;
-    SG_HIP(hipEventElapsedTime(&ms, ev_begin, ev_end));
-    (void)hipEventDestroy(ev_begin);
-    (void)hipEventDestroy(ev_end);
-    if (stats) stats->kernel_ms = ms;
-  }
+  if (want_total) SG_HIP(hipEventRecord(ws->ev_end, s));
   SG_HIP(hipStreamSynchronize(s));
   if (stats) {
-    stats->levels = level;
+    float ms = 0.f;
+    SG_HIP(hipEventElapsedTime(&ms, ws->ev_begin, ws->ev_end));
+    stats->kernel_ms = ms;
+    stats->levels = levels;
     stats->n_push = n_push;
     stats->n_pull = n_pull;
     stats->reached_vertices = reached_v;
-    stats->reached_edges = (int64_t)visited_deg;
+    stats->reached_edges = reached_e;
   }
   return SG_OK;
 }
@@ -487,7 +623,11 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
            "sg_bfs: policy %d not supported", opt.policy);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
-  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (stats) {
+    stats->levels = stats->n_push = stats->n_pull = 0;
+    stats->reached_vertices = stats->reached_edges = 0;
+    stats->kernel_ms = 0.f;
+  }
   SG_HIP(hipSetDevice(g->dev->ordinal));
   if (g->offset_bytes == 8) return run_bfs<int64_t>(g, source, opt, depth, stats);
   return run_bfs<uint32_t>(g, source, opt, depth, stats);

@@ -178,7 +178,7 @@ struct CompactArgs {
 
 /* pass 1: one wave per 64-word tile */
 template<typename OffT, int kMode>
-__global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT> a) {
+__device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a) {
   const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (tile >= a.ntiles) return; /* whole wave leaves together */
   const int lane = lane_id();
@@ -238,6 +238,12 @@ __global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT>
   }
 }
 
+template<typename OffT, int kMode>
+__global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT> a) {
+  compact_count_body<OffT, kMode>(a);
+}
+
+
 /* pass 2: one 1024-thread workgroup scans the per-tile class counts (in
  * place, exclusive) and writes the totals into the queue header.  Each thread
  * owns kScanPer consecutive tiles, so one workgroup scan covers 8,192 tiles
@@ -245,8 +251,9 @@ __global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT>
 constexpr int kScanThreads = 1024;
 constexpr int kScanPer = 8;
 
-static __global__ void __launch_bounds__(kScanThreads) compact_scan_kernel(TileCount* tiles, int64_t ntiles,
-                                                                           QueueHdr* hdr) {
+/* Must be called by all kScanThreads threads; thread 0 gets the totals in
+ * *out (also stored to *hdr). */
+__device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntiles, QueueHdr* hdr, QueueHdr* out) {
   __shared__ uint32_t wave_tot[kScanThreads / 64][3];
   __shared__ unsigned long long red[kScanThreads / 64][3];
   uint32_t carry[3] = {0, 0, 0};
@@ -313,12 +320,18 @@ static __global__ void __launch_bounds__(kScanThreads) compact_scan_kernel(TileC
       h.maxdeg = red[w][2] > h.maxdeg ? red[w][2] : h.maxdeg;
     }
     *hdr = h;
+    if (out) *out = h;
   }
+}
+
+static __global__ void __launch_bounds__(kScanThreads) compact_scan_kernel(TileCount* tiles, int64_t ntiles,
+                                                                           QueueHdr* hdr) {
+  compact_scan_body(tiles, ntiles, hdr, nullptr);
 }
 
 /* pass 3: emit the class queues in vertex order at the scanned offsets */
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT> a) {
+__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a) {
   const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (tile >= a.ntiles) return;
   const int lane = lane_id();
@@ -364,6 +377,12 @@ __global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT>
     pos2 += __shfl(incl, 63, 64);
   }
 }
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT> a) {
+  compact_write_body<OffT>(a);
+}
+
 
 /* Single-vertex queue (level-0 source) written by one thread. */
 template<typename OffT>
@@ -482,6 +501,23 @@ struct QueueSet {
   }
   int grid_tiles() const { return (int)((ntiles * 64 + kBlock - 1) / kBlock); }
 };
+
+template<typename OffT>
+inline CompactArgs<OffT> make_compact_args(QueueSet& qs, int dst, const OffT* off, const uint64_t* visited,
+                                           uint64_t* prev, uint64_t* frontier, const Tuning& t) {
+  CompactArgs<OffT> a{};
+  a.off = off;
+  a.w0 = qs.w0;
+  a.ntiles = qs.ntiles;
+  a.visited = visited;
+  a.prev = prev;
+  a.frontier = frontier;
+  a.tiles = qs.tiles;
+  for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
+  a.hdr = qs.hdr + dst;
+  a.tune = t;
+  return a;
+}
 
 /* Launch the three compaction passes into queue set `dst`.  count_only skips
  * the write pass (totals still land in hdr[dst]). */

@@ -205,3 +205,23 @@ def test_rmat24_full_size(sg, device, oracle_mod):
             assert st.levels == rec["levels"]
             e, v = G.reached_edges(b.depth)
             assert (e, v) == (rec["reached_edges"], rec["reached_vertices"])
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_lookahead_matches_level_by_level(sg, device, oracle_mod, policy):
+    """Levels enqueued ahead of the host (device-side decisions) give the same
+    depths, schedule and per-level counters as waiting for every level."""
+    g = oracle_mod.rmat(13, 16, seed=17)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols)
+    pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[policy]
+    src = int(np.argmax(g.degrees()))
+    runs = []
+    for sync in (False, True):
+        b = sg.BFS(G)
+        b.init(src)
+        st = b.run(pol, 15.0, 24.0, sync_per_level=sync)
+        lv = [(st.level[i].direction, st.level[i].discovered, st.level[i].edges_inspected,
+               st.level[i].vertices_scanned, st.level[i].unexplored_edges) for i in range(st.levels)]
+        runs.append((b.get_distances(), st.levels, st.reached_edges, lv))
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1:] == runs[1][1:]
+    assert np.array_equal(runs[0][0], oracle_mod.bfs(g, src)[0])
