@@ -529,7 +529,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
     decide_kernel<<<1, 64, 0, s>>>(ws->ctl, la.stats, P);
     count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
-    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.tiles, Q.ntiles, Q.hdr + nxt, ws->ctl, P, L, ws->log);
+    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, L, ws->log);
     write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev_done[r], s));

@@ -170,17 +170,19 @@ struct CompactArgs {
   const uint64_t* visited; /* kFromDiff / kFromBitmapSyncPrev */
   uint64_t* prev;          /* kFromDiff / kFromBitmapSyncPrev */
   uint64_t* frontier;      /* bitmap: written (kFromDiff) or read */
-  TileCount* tiles;
+  TileCount* tiles;        /* per wave (64-word tile): class counts */
+  TileCount* blocks;       /* per workgroup (4 tiles): sums, then exclusive offsets after the scan */
   QEntry* qout[3];
   QueueHdr* hdr;
   Tuning tune;
 };
 
-/* pass 1: one wave per 64-word tile */
+/* pass 1: one wave per 64-word tile; the workgroup also sums its four tiles
+ * (all threads of the workgroup must call) */
 template<typename OffT, int kMode>
 __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a) {
-  const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (tile >= a.ntiles) return; /* whole wave leaves together */
+  __shared__ TileCount s_wave[kBlock / 64];
+  const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; /* tiles >= ntiles see only words >= w0 */
   const int lane = lane_id();
   const int64_t myw = tile * 64 + lane;
   uint64_t mynew = 0;
@@ -234,7 +236,19 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a) {
     t.nvert = nv;
     t.deg = deg;
     t.maxdeg = mx;
-    a.tiles[tile] = t;
+    if (tile < a.ntiles) a.tiles[tile] = t;
+    s_wave[threadIdx.x >> 6] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    TileCount b{};
+    for (int w = 0; w < kBlock / 64; ++w) {
+      for (int c = 0; c < 3; ++c) b.c[c] += s_wave[w].c[c];
+      b.nvert += s_wave[w].nvert;
+      b.deg += s_wave[w].deg;
+      b.maxdeg = s_wave[w].maxdeg > b.maxdeg ? s_wave[w].maxdeg : b.maxdeg;
+    }
+    a.blocks[blockIdx.x] = b;
   }
 }
 
@@ -244,12 +258,13 @@ __global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT>
 }
 
 
-/* pass 2: one 1024-thread workgroup scans the per-tile class counts (in
- * place, exclusive) and writes the totals into the queue header.  Each thread
- * owns kScanPer consecutive tiles, so one workgroup scan covers 8,192 tiles
- * (33.5 M vertices); larger graphs loop with a carry. */
+/* pass 2: one 1024-thread workgroup scans the per-workgroup class counts of
+ * pass 1 (in place, exclusive) and writes the totals into the queue header.
+ * Each thread owns kScanPer consecutive entries, so one workgroup scan covers
+ * 2,048 entries (8,192 tiles, 33.5 M vertices); larger graphs loop with a
+ * carry. */
 constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 8;
+constexpr int kScanPer = 2;
 
 /* Must be called by all kScanThreads threads; thread 0 gets the totals in
  * *out (also stored to *hdr). */
@@ -337,8 +352,16 @@ __device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a) {
   const int lane = lane_id();
   const int64_t myw = tile * 64 + lane;
   const uint64_t mynew = myw < a.w0 ? a.frontier[myw] : 0ull;
-  const TileCount tc = a.tiles[tile];
-  uint32_t pos0 = tc.c[0], pos1 = tc.c[1], pos2 = tc.c[2];
+  /* this tile's offsets: its workgroup's exclusive offsets plus the counts of
+   * the workgroup's earlier tiles */
+  const TileCount bc = a.blocks[blockIdx.x];
+  uint32_t pos0 = bc.c[0], pos1 = bc.c[1], pos2 = bc.c[2];
+  for (int64_t k = (int64_t)blockIdx.x * (kBlock / 64); k < tile; ++k) {
+    const TileCount tc = a.tiles[k];
+    pos0 += tc.c[0];
+    pos1 += tc.c[1];
+    pos2 += tc.c[2];
+  }
   uint64_t todo = ballot(mynew != 0);
   while (todo) {
     const int k = __ffsll((unsigned long long)todo) - 1;
@@ -474,14 +497,18 @@ struct QueueSet {
   QEntry* q[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   QueueHdr* hdr = nullptr; /* 2 headers */
   TileCount* tiles = nullptr;
+  TileCount* blocks = nullptr;
+  int64_t nblocks = 0;
 
   int init(sg_device_t dev, int64_t n, int64_t m, const Tuning& t) {
     w0 = (n + 63) / 64;
     ntiles = (w0 + 63) / 64;
+    nblocks = (ntiles + kBlock / 64 - 1) / (kBlock / 64);
     cap_large = m / t.chunk + m / t.t_large + 64;
     int st = SG_OK;
     if ((st = alloc_device(dev, (void**)&hdr, sizeof(QueueHdr) * 2)) ||
-        (st = alloc_device(dev, (void**)&tiles, sizeof(TileCount) * (ntiles + 1))))
+        (st = alloc_device(dev, (void**)&tiles, sizeof(TileCount) * (ntiles + 1))) ||
+        (st = alloc_device(dev, (void**)&blocks, sizeof(TileCount) * (nblocks + 1))))
       return st;
     for (int s = 0; s < 2; ++s)
       for (int k = 0; k < 3; ++k)
@@ -496,10 +523,12 @@ struct QueueSet {
       }
     (void)hipFree(hdr);
     (void)hipFree(tiles);
+    (void)hipFree(blocks);
     hdr = nullptr;
     tiles = nullptr;
+    blocks = nullptr;
   }
-  int grid_tiles() const { return (int)((ntiles * 64 + kBlock - 1) / kBlock); }
+  int grid_tiles() const { return (int)nblocks; }
 };
 
 template<typename OffT>
@@ -513,6 +542,7 @@ inline CompactArgs<OffT> make_compact_args(QueueSet& qs, int dst, const OffT* of
   a.prev = prev;
   a.frontier = frontier;
   a.tiles = qs.tiles;
+  a.blocks = qs.blocks;
   for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
   a.hdr = qs.hdr + dst;
   a.tune = t;
@@ -532,11 +562,12 @@ inline int compact_to_queues(hipStream_t s, QueueSet& qs, int dst, const OffT* o
   a.prev = prev;
   a.frontier = frontier;
   a.tiles = qs.tiles;
+  a.blocks = qs.blocks;
   for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
   a.hdr = qs.hdr + dst;
   a.tune = t;
   compact_count_kernel<OffT, kMode><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
-  compact_scan_kernel<<<1, kScanThreads, 0, s>>>(qs.tiles, qs.ntiles, qs.hdr + dst);
+  compact_scan_kernel<<<1, kScanThreads, 0, s>>>(qs.blocks, qs.nblocks, qs.hdr + dst);
   if (!count_only) compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
   SG_HIP(hipGetLastError());
   return SG_OK;
@@ -553,6 +584,7 @@ inline int compact_write_only(hipStream_t s, QueueSet& qs, int dst, const OffT* 
   a.ntiles = qs.ntiles;
   a.frontier = frontier;
   a.tiles = qs.tiles;
+  a.blocks = qs.blocks;
   for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
   a.hdr = qs.hdr + dst;
   a.tune = t;
