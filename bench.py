@@ -108,6 +108,47 @@ def measured_traffic(kernel: str):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
+def batched_vs_plain(sg, G, sources16):
+    """64 sources (the 16 committed ones + 48 seeded non-isolated vertices):
+    batched multi-source BFS (64 bit-lanes) against one plain push BFS per
+    source and one hybrid BFS per source.  Depths must agree exactly."""
+    off, _, _ = G.download()
+    rng = np.random.default_rng(64)
+    cand = np.nonzero(np.diff(off) > 0)[0]
+    srcs = list(sources16) + [int(x) for x in rng.choice(cand, 48, replace=False)]
+    b = sg.BFS(G)
+    out = {}
+    edges = 0
+    rows = {}
+    for name, pol in [("plain_push", sg.POLICY_PUSH), ("hybrid", sg.POLICY_HYBRID)]:
+        G.dev.synchronize()
+        t0 = time.perf_counter()
+        e = 0
+        for s in srcs:
+            b.init(s)
+            st = b.run(pol, 15.0, 24.0)
+            e += st.reached_edges
+            if name == "plain_push" and s in (srcs[0], srcs[-1]):
+                rows[s] = b.get_distances()
+        G.dev.synchronize()
+        out[name] = e / (time.perf_counter() - t0) / 1e9
+        edges = e
+    import ctypes as C
+
+    buf = G.dev.alloc(4 * len(srcs) * G.n)  # depth rows stay on the device
+    sa = np.ascontiguousarray(np.asarray(srcs, dtype=np.int32))
+    G.dev.synchronize()
+    t0 = time.perf_counter()
+    sg.check(sg.lib().sg_bfs_multi(G.h, len(srcs), sa.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr)))
+    G.dev.synchronize()
+    t = time.perf_counter() - t0
+    out["batched"] = edges / t / 1e9
+    d = buf.download(np.int32, len(srcs) * G.n).reshape(len(srcs), G.n)
+    assert np.array_equal(d[0], rows[srcs[0]]) and np.array_equal(d[-1], rows[srcs[-1]])
+    buf.free()
+    return {k: round(v, 3) for k, v in out.items()}
+
+
 def run_sssp(sg, G, sources):
     s = sg.SSSP(G)
     edges = 0
@@ -204,6 +245,10 @@ def main():
     roof["copy_ceiling_gbs"] = round(cc, 1)
     log(f"copy kernel ceiling: {cc:.0f} GB/s (read + write)")
     if not a.no_extra:
+        bp = batched_vs_plain(sg, G, sources)
+        extra["bfs_64_sources_gteps"] = bp
+        log(f"64 sources: batched {bp['batched']:.1f}, plain push {bp['plain_push']:.1f}, "
+            f"hybrid {bp['hybrid']:.1f} GTEPS")
         e2, w2 = run_sssp(sg, G, sources)
         extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
         log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3)")

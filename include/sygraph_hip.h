@@ -292,7 +292,10 @@ int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex);
  * is added to bc once.  Sources are processed one after another. */
 int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
 /* Multi-source BFS: depth is device int32[nsrc][n] (row i = sources_host[i]),
- * unreached = n+1; one fused hybrid BFS per source. */
+ * unreached = n+1.  Sources run 64 per batch, one bit-lane each: every vertex
+ * holds 64-bit visited / frontier / next masks, so one pass over the arcs
+ * advances all 64 traversals by a level.  No counterpart in the reference
+ * (its BFS takes one source). */
 int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth);
 
 #ifdef __cplusplus

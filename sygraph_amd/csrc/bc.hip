@@ -1,6 +1,6 @@
 /*
- * bc.hip -- betweenness centrality (BC::run, algorithms/bc.hpp:118-200) and
- * multi-source BFS for gfx950.
+ * bc.hip -- betweenness centrality (BC::run, algorithms/bc.hpp:118-200) for
+ * gfx950.
  *
  * Per source s:
  *  1. depths from the fused BFS (sg_bfs, hybrid push/pull);
@@ -266,16 +266,4 @@ extern "C" int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, fl
   SG_HIP(hipSetDevice(g->dev->ordinal));
   if (g->offset_bytes == 8) return run_bc<int64_t>(g, nsrc, sources_host, bc);
   return run_bc<uint32_t>(g, nsrc, sources_host, bc);
-}
-
-/* One BFS per source into row s of depth (int32[nsrc][n]). */
-extern "C" int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth) {
-  SG_CHECK(g && depth && (nsrc == 0 || sources_host), SG_ERR_INVALID, "sg_bfs_multi: null argument");
-  SG_CHECK(nsrc >= 0, SG_ERR_INVALID, "sg_bfs_multi: negative source count");
-  sg_bfs_options opt;
-  sg_bfs_options_default(&opt);
-  opt.policy = SG_POLICY_HYBRID;
-  auto stats = std::make_unique<sg_bfs_stats>();
-  for (int32_t i = 0; i < nsrc; ++i) SG_TRY(sg_bfs(g, sources_host[i], &opt, depth + (int64_t)i * g->n, stats.get()));
-  return SG_OK;
 }
