@@ -46,6 +46,7 @@ typedef struct sg_device_s* sg_device_t;
 typedef struct sg_graph_s* sg_graph_t;
 typedef struct sg_frontier_s* sg_frontier_t;
 typedef struct sg_event_s* sg_event_t;
+typedef struct sg_policy_s* sg_policy_t;
 
 /* ------------------------------------------------------------------ errors */
 /* Thread-local text of the last error (reference: the what() of the
@@ -229,10 +230,18 @@ enum sg_load_balancer {
 #define SG_MAX_LEVELS_RECORDED 4096
 typedef struct sg_bfs_options {
   int32_t policy;          /* sg_direction_policy */
-  int32_t load_balancer;   /* sg_load_balancer for push levels */
+  int32_t load_balancer;   /* sg_load_balancer: the work mapping of every level under the push, pull and
+                              hybrid policies.  Push levels: BUCKETING (and WORKITEM) = lane / wave /
+                              workgroup bins; WORKGROUP = the arcs of the lane and wave bins laid end to
+                              end (edge-balanced); SUBGROUP = large lists split over waves instead of
+                              workgroups.  Pull levels: lane-serial head of 16 / 4 / 64 in-arcs
+                              (BUCKETING / WORKGROUP / SUBGROUP) before the wave joins. */
   float alpha, beta;       /* hybrid thresholds (algorithms/bfs.hpp:153) */
   int32_t record_levels;   /* fill per-level stats in sg_bfs_stats */
-  int32_t sync_per_level;  /* 1: host reads stats every level (default 0) */
+  int32_t sync_per_level;  /* 1: the host waits for every level before enqueuing the next (default 0:
+                              one level ahead) */
+  sg_policy_t learned;     /* SG_POLICY_LEARNED: the trained policy choosing direction and work mapping
+                              per level */
 } sg_bfs_options;
 
 typedef struct sg_level_stat {
@@ -247,6 +256,8 @@ typedef struct sg_level_stat {
   int64_t hint_misses;       /* pull: scanned vertices whose hint in-neighbour was not in the frontier */
   int64_t unexplored_edges;  /* sum deg over unvisited after the level */
   int64_t max_degree;        /* largest degree among the discovered vertices */
+  int32_t action;            /* direction * 3 + work-mapping variant (0 bucketing, 1 edge-balanced /
+                                short pull head, 2 wave-split / long pull head) of this level */
   float kernel_ms;           /* device time of the level's advance kernel (record_levels) */
   float convert_ms;          /* device time of a queue<->bitmap conversion after it */
 } sg_level_stat;
@@ -264,6 +275,29 @@ void sg_bfs_options_default(sg_bfs_options* opt);
 /* BFS::run (algorithms/bfs.hpp:153-238): depth (device int32[n]) receives
  * hop counts, unreached = n+1 (bfs.hpp:62-67).  stats may be NULL. */
 int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt, int32_t* depth, sg_bfs_stats* stats);
+
+/* ---- learned per-level policy (no counterpart in the reference, whose only
+ * policy is the alpha/beta rule of algorithms/bfs.hpp:325-350) -------------
+ * An 8 -> 32 -> 32 -> 6 ReLU MLP in FP32 (1,542 parameters, layout W1[32][8]
+ * b1[32] W2[32][32] b2[32] W3[6][32] b3[6]) scoring action = direction * 3 +
+ * work-mapping variant from eight features of the frontier entering a level.
+ * Features, scores and labels live in device memory. */
+#define SG_POLICY_PARAMS 1542
+#define SG_POLICY_FEATURES 8
+#define SG_POLICY_ACTIONS 6
+int sg_policy_create(sg_device_t dev, sg_policy_t* out); /* parameters start at zero */
+int sg_policy_destroy(sg_policy_t p);
+int sg_policy_set_params(sg_policy_t p, const float* host_params); /* also resets the optimizer */
+int sg_policy_get_params(sg_policy_t p, float* host_params);
+/* scores[batch][6] for features[batch][8] */
+int sg_policy_forward(sg_policy_t p, const float* features, int32_t batch, float* scores);
+/* gradient (device float[1542]) of the mean softmax cross-entropy of labels[batch];
+ * loss (host float, may be NULL) receives the mean loss */
+int sg_policy_gradient(sg_policy_t p, const float* features, const int32_t* labels, int32_t batch, float* grad,
+                       float* loss);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss */
+int sg_policy_train_step(sg_policy_t p, const float* features, const int32_t* labels, int32_t batch, float lr,
+                         float* loss);
 
 typedef struct sg_sssp_stats {
   int32_t rounds;            /* frontier rounds until no distance changed */

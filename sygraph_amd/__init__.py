@@ -67,14 +67,14 @@ class FrontierView(C.Structure):
 
 class BfsOptions(C.Structure):
     _fields_ = [("policy", C.c_int32), ("load_balancer", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
-                ("record_levels", C.c_int32), ("sync_per_level", C.c_int32)]
+                ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p)]
 
 
 class LevelStat(C.Structure):
     _fields_ = [("direction", C.c_int32), ("load_balancer", C.c_int32), ("frontier_vertices", C.c_int64),
                 ("frontier_edges", C.c_int64), ("discovered", C.c_int64), ("edges_inspected", C.c_int64),
                 ("vertices_scanned", C.c_int64), ("hint_misses", C.c_int64), ("unexplored_edges", C.c_int64),
-                ("max_degree", C.c_int64),
+                ("max_degree", C.c_int64), ("action", C.c_int32),
                 ("kernel_ms", C.c_float), ("convert_ms", C.c_float)]
 
 
@@ -109,6 +109,13 @@ _SIGS = {
     "sg_memcpy": (_I, [_P, _P, _P, C.c_size_t]),
     "sg_fill": (_I, [_P, _P, C.c_uint64, _I, C.c_size_t]),
     "sg_copy_device": (_I, [_P, _P, _P, C.c_size_t]),
+    "sg_policy_create": (_I, [_P, _P]),
+    "sg_policy_destroy": (_I, [_P]),
+    "sg_policy_set_params": (_I, [_P, _P]),
+    "sg_policy_get_params": (_I, [_P, _P]),
+    "sg_policy_forward": (_I, [_P, _P, C.c_int32, _P]),
+    "sg_policy_gradient": (_I, [_P, _P, _P, C.c_int32, _P, _P]),
+    "sg_policy_train_step": (_I, [_P, _P, _P, C.c_int32, C.c_float, _P]),
     "sg_event_create": (_I, [_P, C.POINTER(_P)]),
     "sg_event_destroy": (_I, [_P]),
     "sg_event_record": (_I, [_P]),
@@ -484,7 +491,8 @@ def swap(a: Frontier, b: Frontier):
 
 # -------------------------------------------------------------- algorithms --
 def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, alpha: float = 15.0,
-                beta: float = 24.0, record_levels: bool = False, sync_per_level: bool = False) -> BfsOptions:
+                beta: float = 24.0, record_levels: bool = False, sync_per_level: bool = False,
+                learned: Optional["Policy"] = None) -> BfsOptions:
     o = BfsOptions()
     lib().sg_bfs_options_default(C.byref(o))
     o.policy = policy
@@ -493,6 +501,7 @@ def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, al
     o.beta = beta
     o.record_levels = 1 if record_levels else 0
     o.sync_per_level = 1 if sync_per_level else 0
+    o.learned = learned.h if learned is not None else None
     return o
 
 
@@ -515,10 +524,12 @@ class BFS:
         self.source = None
 
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
-            load_balancer: int = LB_BUCKETING, sync_per_level: bool = False) -> BfsStats:
+            load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
+            learned: Optional["Policy"] = None) -> BfsStats:
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
-        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level)
+        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,
+                        learned=learned)
         check(lib().sg_bfs(self.g.h, self.source, C.byref(o), C.c_void_p(self.depth.ptr), C.byref(self.stats)))
         return self.stats
 
@@ -648,3 +659,64 @@ def bfs_multi(g: Graph, sources) -> np.ndarray:
     out = DeviceBuffer(g.dev, 4 * max(len(s) * g.n, 1), dtype=np.int32)
     check(lib().sg_bfs_multi(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr)))
     return out.download(np.int32, len(s) * g.n).reshape(len(s), g.n)
+
+
+# ------------------------------------------------------------ learned policy --
+POLICY_PARAMS, POLICY_FEATURES, POLICY_ACTIONS = 1542, 8, 6
+
+
+class Policy:
+    """The learned per-level BFS policy (8-32-32-6 MLP) on a device; training
+    and inference run in HIP kernels."""
+
+    def __init__(self, dev: Device, params: Optional[np.ndarray] = None):
+        self.dev = dev
+        h = C.c_void_p()
+        check(lib().sg_policy_create(dev.h, C.byref(h)))
+        self.h = h
+        self._grad = DeviceBuffer(dev, 4 * POLICY_PARAMS, dtype=np.float32)
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == POLICY_PARAMS
+        check(lib().sg_policy_set_params(self.h, p.ctypes.data_as(C.c_void_p)))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(POLICY_PARAMS, dtype=np.float32)
+        check(lib().sg_policy_get_params(self.h, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        xb = self.dev.to_device(X)
+        out = DeviceBuffer(self.dev, 4 * len(X) * POLICY_ACTIONS, dtype=np.float32)
+        check(lib().sg_policy_forward(self.h, C.c_void_p(xb.ptr), len(X), C.c_void_p(out.ptr)))
+        return out.download(np.float32).reshape(len(X), POLICY_ACTIONS)
+
+    def gradient(self, X: np.ndarray, y: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        xb, yb = self.dev.to_device(X), self.dev.to_device(np.ascontiguousarray(y, dtype=np.int32))
+        loss = C.c_float()
+        check(lib().sg_policy_gradient(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), len(X),
+                                       C.c_void_p(self._grad.ptr), C.byref(loss)))
+        return float(loss.value), self._grad.download(np.float32)
+
+    def train_step(self, xb: DeviceBuffer, yb: DeviceBuffer, batch: int, lr: float) -> float:
+        loss = C.c_float()
+        check(lib().sg_policy_train_step(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), batch, lr, C.byref(loss)))
+        return float(loss.value)
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            lib().sg_policy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _finalizing():
+            return
+        try:
+            self.destroy()
+        except Exception:
+            pass

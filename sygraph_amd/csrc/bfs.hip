@@ -32,13 +32,14 @@
 
 using namespace sygraph::hipdev;
 
+#include "policy.hpp"
 #include "traverse.hpp"
 
 using namespace sg::trav;
 
 namespace {
 
-constexpr int kPullHead = 16; /* ids a lane scans alone before the wave helps */
+constexpr int kPullHead = 16; /* steps of four ids in the longest lane-serial pull head */
 
 template<typename OffT>
 struct BfsArgs {
@@ -69,18 +70,20 @@ __device__ __forceinline__ void push_visit(const BfsArgs<OffT>& a, uint32_t y) {
 /* Push level: every arc of the class queues; discovery is the test-and-set of
  * the visited bit. */
 template<typename OffT>
-__device__ __forceinline__ void push_level(const BfsArgs<OffT>& a) {
+__device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant) {
   zero_slots(a.stats_zero);
-  for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols,
-                            [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y); });
+  for_each_queued_arc<OffT>(
+      a.qin, a.qin_hdr, a.cols, [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y); }, variant);
 }
 
 /* Pull level.  A wave loads 64 visited words at once and compacts their
  * unvisited vertices in registers (popcount scan + bit select), so all 64 lanes
  * pull for real vertices; new frontier bits are gathered per word in LDS. */
 template<typename OffT>
-__device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a) {
+__device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) {
   zero_slots(a.stats_zero);
+  /* arcs a lane scans alone before the wave helps: 16 / 4 / 64 */
+  const int head_steps = variant == 1 ? 1 : (variant == 2 ? kPullHead : kPullHead / 4);
   __shared__ unsigned long long lds_new[kBlock];
   uint64_t st_disc = 0, st_deg = 0, st_edges = 0, st_scanned = 0, st_max = 0, st_miss = 0;
   const int lane = lane_id();
@@ -130,7 +133,7 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a) {
         }
         /* lane-serial head: four independent id loads then four independent
          * bitmap probes per step */
-        for (int step = 0; step < kPullHead / 4 && j < e; ++step, j += 4) {
+        for (int step = 0; step < head_steps && j < e; ++step, j += 4) {
           const uint32_t u0 = (uint32_t)a.in_cols[j];
           const uint32_t u1 = j + 1 < e ? (uint32_t)a.in_cols[j + 1] : u0;
           const uint32_t u2 = j + 2 < e ? (uint32_t)a.in_cols[j + 2] : u0;
@@ -203,8 +206,8 @@ struct BfsCtl {
   int32_t done;     /* a level discovered nothing: later levels are no-ops */
   int32_t mode;     /* count pass of this level: 0 none, 1 visited & ~prev, 2 frontier bitmap + prev sync */
   int32_t write;    /* queues for the next (push) level must be written */
-  int32_t next_dir; /* decided after a pull level by decide_kernel */
-  int32_t pad;
+  int32_t next_act; /* decided after a pull level by decide_kernel: direction * 3 + variant */
+  int32_t lb;       /* work-mapping variant of the level about to run */
   int64_t f_vertices, f_edges; /* frontier entering the level */
   unsigned long long visited_deg, reached;
   unsigned long long disc, disc_deg, edges, scanned, misses, max_deg; /* staged pull-level statistics */
@@ -213,7 +216,7 @@ struct BfsCtl {
 /* One level's record, written by finish_kernel into a ring in pinned host
  * memory; the host reads it after the level's completion event. */
 struct alignas(64) LevelLog {
-  int32_t level, dir, done, next_dir;
+  int32_t level, dir, done, next_dir, action;
   int64_t f_vertices, f_edges, disc, disc_deg, edges, scanned, misses, max_deg, unexplored;
 };
 
@@ -224,6 +227,9 @@ struct DecideParams {
   float alpha, beta;
   int64_t n;
   unsigned long long total_degree;
+  int32_t lb_fixed;     /* variant under the push / pull / hybrid policies */
+  const float* mlp;     /* learned policy parameters (SG_POLICY_LEARNED) */
+  double n_d, m_d;      /* vertex and arc counts for the features */
 };
 
 /* evaluateHeuristic (algorithms/bfs.hpp:341-350): push -> pull when the new
@@ -239,11 +245,25 @@ __device__ __forceinline__ int32_t next_direction(const DecideParams& p, int32_t
   return ((float)disc < ((float)p.n / p.beta)) ? 0 : 1;
 }
 
+/* Action (direction * 3 + variant) of the next level; every lane of one
+ * wavefront calls with the same arguments. */
+__device__ __forceinline__ int32_t decide_action(const DecideParams& p, int32_t dir, unsigned long long disc,
+                                                 unsigned long long disc_deg, unsigned long long unexplored,
+                                                 int32_t next_level, unsigned long long max_deg) {
+  if (p.policy == SG_POLICY_LEARNED) {
+    float f[sg::policy::kIn];
+    sg::policy::features(f, (double)disc, (double)disc_deg, (double)unexplored, next_level, (double)max_deg, p.n_d,
+                         p.m_d);
+    return sg::policy::infer_wave(p.mlp, f);
+  }
+  return next_direction(p, dir, disc, disc_deg, unexplored) * 3 + p.lb_fixed;
+}
+
 /* Level-0 setup: depth[src] = 0, visited = prev = isolated | {src},
  * cur_bm = {src}, F0 queue = {src}, control block for level 0. */
 template<typename OffT>
 __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, QEntry* q1, QEntry* q2, QueueHdr* hdr,
-                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir) {
+                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir, int32_t first_lb) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t bit = 1ull << (src & 63);
   a.depth[src] = 0;
@@ -255,6 +275,7 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
   const unsigned long long d = (unsigned long long)(a.off[src + 1] - a.off[src]);
   BfsCtl c{};
   c.dir = first_dir;
+  c.lb = first_lb;
   c.f_vertices = 1;
   c.f_edges = (int64_t)d;
   c.visited_deg = d; /* the source's arcs count as explored */
@@ -266,14 +287,14 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
   if (ctl->done) return;
-  if (ctl->dir == 0) push_level(a);
-  else pull_level(a);
+  if (ctl->dir == 0) push_level(a, ctl->lb);
+  else pull_level(a, ctl->lb);
 }
 
 /* After the traversal: a pull level's statistics are reduced from its slots and
  * the next direction decided now, because the count pass depends on it (queues
  * are only built when the next level pushes); a push level is counted first. */
-__global__ void decide_kernel(BfsCtl* ctl, const StatSlot* slots, DecideParams p) {
+__global__ void decide_kernel(BfsCtl* ctl, const StatSlot* slots, DecideParams p, int32_t level) {
   if (ctl->done) return;
   if (ctl->dir == 0) {
     if (threadIdx.x == 0) ctl->mode = 1;
@@ -283,16 +304,17 @@ __global__ void decide_kernel(BfsCtl* ctl, const StatSlot* slots, DecideParams p
   const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
   const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
   const unsigned long long misses = wave_sum_u64(x.misses), max_deg = wave_max_u64(x.max_deg);
+  const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
+  const int32_t act = decide_action(p, 1, disc, disc_deg, unexplored, level + 1, max_deg);
+  const int32_t next = act / 3;
   if (threadIdx.x == 0) {
-    const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
-    const int32_t next = next_direction(p, 1, disc, disc_deg, unexplored);
     ctl->disc = disc;
     ctl->disc_deg = disc_deg;
     ctl->edges = edges;
     ctl->scanned = scanned;
     ctl->misses = misses;
     ctl->max_deg = max_deg;
-    ctl->next_dir = next;
+    ctl->next_act = act;
     ctl->mode = (disc > 0 && next == 0) ? 2 : 0;
   }
 }
@@ -305,8 +327,9 @@ __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, 
   else if (mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
 }
 
-/* Scan of the count pass, the push-level decision, the level record, and the
- * control block for the next level. */
+/* Scan of the count pass, the push-level decision (wave 0, so the learned
+ * policy can run on it), the level record, and the control block for the
+ * next level. */
 __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, int64_t ntiles, QueueHdr* hdr_out,
                                                               BfsCtl* ctl, DecideParams p, int32_t level,
                                                               LevelLog* log) {
@@ -315,10 +338,10 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   const int32_t mode = ctl->mode;
   if (mode != 0) compact_scan_body(tiles, ntiles, hdr_out, &tot);
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= 64) return;
   const int32_t dir = ctl->dir;
   unsigned long long disc, disc_deg, edges, scanned, misses, max_deg;
-  int32_t next;
+  int32_t act;
   const unsigned long long vdeg_before = ctl->visited_deg;
   if (dir == 0) {
     disc = tot.nvert;
@@ -327,7 +350,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     edges = (unsigned long long)ctl->f_edges;
     scanned = (unsigned long long)ctl->f_vertices;
     misses = 0;
-    next = next_direction(p, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg));
+    act = decide_action(p, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1, max_deg);
   } else {
     disc = ctl->disc;
     disc_deg = ctl->disc_deg;
@@ -335,14 +358,17 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     edges = ctl->edges;
     scanned = ctl->scanned;
     misses = ctl->misses;
-    next = ctl->next_dir;
+    act = ctl->next_act;
   }
+  if (threadIdx.x != 0) return;
+  const int32_t next = act / 3;
   const unsigned long long vdeg = vdeg_before + disc_deg;
   LevelLog e;
   e.level = level;
   e.dir = dir;
   e.done = disc == 0 ? 1 : 0;
   e.next_dir = next;
+  e.action = dir * 3 + ctl->lb;
   e.f_vertices = ctl->f_vertices;
   e.f_edges = ctl->f_edges;
   e.disc = (int64_t)disc;
@@ -361,6 +387,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   ctl->write = (disc > 0 && next == 0) ? 1 : 0;
   ctl->done = disc == 0 ? 1 : 0;
   ctl->dir = next;
+  ctl->lb = act % 3;
 }
 
 template<typename OffT>
@@ -498,8 +525,11 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   copy_words_kernel<<<grid_small, 256, 0, s>>>(ws->visited, ws->prev, g->isolated, w0);
   SG_TRY(sg::zero_async(dev, ws->bm[0], 8 * w0));
   SG_TRY(sg::zero_async(dev, ws->stats, sizeof(StatSlot) * kSlots * 2));
+  /* work-mapping variant under the fixed policies (see sg_bfs_options) */
+  const int32_t lb_fixed = opt.load_balancer == SG_LB_WORKGROUP ? 1 : (opt.load_balancer == SG_LB_SUBGROUP ? 2 : 0);
   init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
-                                            opt.policy == SG_POLICY_PULL ? 1 : 0);
+                                            opt.policy == SG_POLICY_PULL ? 1 : 0,
+                                            opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed);
   SG_HIP(hipGetLastError());
 
   const bool timed = opt.record_levels && stats;
@@ -511,6 +541,10 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   P.beta = opt.beta;
   P.n = n;
   P.total_degree = (unsigned long long)g->total_degree;
+  P.lb_fixed = lb_fixed;
+  P.mlp = opt.policy == SG_POLICY_LEARNED ? sg::policy_device_params(opt.learned) : nullptr;
+  P.n_d = (double)n;
+  P.m_d = (double)g->m;
   if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 
   auto enqueue = [&](int32_t L) -> int {
@@ -527,7 +561,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
-    decide_kernel<<<1, 64, 0, s>>>(ws->ctl, la.stats, P);
+    decide_kernel<<<1, 64, 0, s>>>(ws->ctl, la.stats, P, L);
     count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, L, ws->log);
     write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
@@ -568,6 +602,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       ls.hint_misses = e.misses;
       ls.max_degree = e.max_deg;
       ls.unexplored_edges = e.unexplored;
+      ls.action = e.action;
       if (timed) {
         SG_HIP(hipEventElapsedTime(&ls.kernel_ms, ws->ev_start[r], ws->ev_mid[r]));
         SG_HIP(hipEventElapsedTime(&ls.convert_ms, ws->ev_mid[r], ws->ev_done[r]));
@@ -619,8 +654,12 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
     opt = *opt_in;
   else
     sg_bfs_options_default(&opt);
-  SG_CHECK(opt.policy >= SG_POLICY_PUSH && opt.policy <= SG_POLICY_HYBRID, SG_ERR_UNSUPPORTED,
+  SG_CHECK(opt.policy >= SG_POLICY_PUSH && opt.policy <= SG_POLICY_LEARNED, SG_ERR_UNSUPPORTED,
            "sg_bfs: policy %d not supported", opt.policy);
+  SG_CHECK(opt.policy != SG_POLICY_LEARNED || opt.learned, SG_ERR_INVALID,
+           "sg_bfs: the learned policy needs a trained sg_policy_t in options.learned");
+  SG_CHECK(opt.load_balancer >= SG_LB_WORKITEM && opt.load_balancer <= SG_LB_BUCKETING, SG_ERR_INVALID,
+           "sg_bfs: load balancer %d out of range", opt.load_balancer);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
   if (stats) {

@@ -138,6 +138,7 @@ int zero_async(sg_device_t dev, void* p, size_t bytes);
 /* graph helpers */
 int graph_ensure_isolated(sg_graph_t g);
 int graph_ensure_pull_hint(sg_graph_t g);
+const float* policy_device_params(sg_policy_t p); /* device parameters of a learned policy */
 void bfs_workspace_release(sg_graph_t g);
 void sssp_workspace_release(sg_graph_t g);
 

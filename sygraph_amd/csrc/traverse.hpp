@@ -441,50 +441,71 @@ __device__ __forceinline__ void enqueue_single(const OffT* off, const Tuning& t,
   *hdr = h;
 }
 
-/* Visit every queued arc with `visit(src, dst, arc_index)`: class 2 entries
- * per workgroup, class 1 per wave, class 0 as a wave-level merge path (the
- * arcs of 64 small vertices laid end to end, 64 per step).  All threads of
- * the workgroup must call. */
+/* Visit every queued arc with `visit(src, dst, arc_index)`.  All threads of
+ * the workgroup must call.  `variant` picks the work mapping (the result is
+ * the same set of visits):
+ *   0  class 2 chunks per workgroup, class 1 a wave per vertex, class 0 as a
+ *      wave-level merge path (the arcs of 64 small vertices laid end to end,
+ *      64 per step) -- the bucketing balancer;
+ *   1  classes 0 and 1 both through the merge path (edge-balanced);
+ *   2  class 2 chunks per wave instead of per workgroup. */
 template<typename OffT, typename Visit>
 __device__ __forceinline__ void for_each_queued_arc(QEntry* const q[3], const QueueHdr* hdr, const int32_t* cols,
-                                                    Visit&& visit) {
+                                                    Visit&& visit, int variant = 0) {
   const uint32_t n_small = hdr->count[0], n_med = hdr->count[1], n_large = hdr->count[2];
   const int lane = lane_id();
   const int wave_in_block = threadIdx.x >> 6;
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
-  for (uint32_t i = blockIdx.x; i < n_large; i += gridDim.x) {
-    const QEntry qe = q[2][i];
-    for (int64_t j = wave_in_block * 64 + lane; j < qe.len; j += kBlock) {
-      const int64_t e = qe.start + j;
-      visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+  if (variant == 2) {
+    for (int64_t i = gwave; i < n_large; i += nwaves) {
+      const QEntry qe = q[2][i];
+      for (int64_t j = lane; j < qe.len; j += 64) {
+        const int64_t e = qe.start + j;
+        visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+      }
+    }
+  } else {
+    for (uint32_t i = blockIdx.x; i < n_large; i += gridDim.x) {
+      const QEntry qe = q[2][i];
+      for (int64_t j = wave_in_block * 64 + lane; j < qe.len; j += kBlock) {
+        const int64_t e = qe.start + j;
+        visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+      }
     }
   }
-  for (int64_t i = gwave; i < n_med; i += nwaves) {
-    const QEntry qe = q[1][i];
-    for (int64_t j = lane; j < qe.len; j += 64) {
-      const int64_t e = qe.start + j;
-      visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+  if (variant != 1) {
+    for (int64_t i = gwave; i < n_med; i += nwaves) {
+      const QEntry qe = q[1][i];
+      for (int64_t j = lane; j < qe.len; j += 64) {
+        const int64_t e = qe.start + j;
+        visit((uint32_t)qe.v, (uint32_t)cols[e], e);
+      }
     }
   }
-  for (int64_t base = gwave * 64; base < n_small; base += nwaves * 64) {
-    const int64_t i = base + lane;
-    QEntry qe;
-    qe.start = 0;
-    qe.v = 0;
-    qe.len = 0;
-    if (i < n_small) qe = q[0][i];
-    const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
-    const uint32_t total = __shfl(incl, 63, 64);
-    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      const int owner = scan_owner(incl, t);
-      const uint32_t before = __shfl(incl - (uint32_t)qe.len, owner, 64);
-      const int64_t st = shfl_i64(qe.start, owner);
-      const int32_t src = __shfl(qe.v, owner, 64);
-      if (t < total) {
-        const int64_t e = st + (t - before);
-        visit((uint32_t)src, (uint32_t)cols[e], e);
+  /* merge path over class 0 (and class 1 for variant 1) */
+  for (int cls = 0; cls < (variant == 1 ? 2 : 1); ++cls) {
+    const QEntry* qq = q[cls];
+    const uint32_t cnt = cls == 0 ? n_small : n_med;
+    for (int64_t base = gwave * 64; base < cnt; base += nwaves * 64) {
+      const int64_t i = base + lane;
+      QEntry qe;
+      qe.start = 0;
+      qe.v = 0;
+      qe.len = 0;
+      if (i < cnt) qe = qq[i];
+      const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
+      const uint32_t total = __shfl(incl, 63, 64);
+      for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const int owner = scan_owner(incl, t);
+        const uint32_t before = __shfl(incl - (uint32_t)qe.len, owner, 64);
+        const int64_t st = shfl_i64(qe.start, owner);
+        const int32_t src = __shfl(qe.v, owner, 64);
+        if (t < total) {
+          const int64_t e = st + (t - before);
+          visit((uint32_t)src, (uint32_t)cols[e], e);
+        }
       }
     }
   }
