@@ -42,7 +42,7 @@ def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
     return (2 * off_bytes * ls.vertices_scanned + 8 * ls.edges_inspected + 4 * ls.discovered + 3 * w0_bytes)
 
 
-def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect):
+def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, learned=None):
     b = sg.BFS(G)
     edges_total = 0
     per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0]}  # dir -> [bytes, ms, launches]
@@ -51,13 +51,13 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect):
     for _ in range(warmup):
         for s in sources:
             b.init(s)
-            b.run(policy, alpha, beta)
+            b.run(policy, alpha, beta, learned=learned)
     G.dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         for s in sources:
             b.init(s)
-            st = b.run(policy, alpha, beta)
+            st = b.run(policy, alpha, beta, learned=learned)
             edges_total += st.reached_edges
             kernel_ms_total += st.kernel_ms
             if collect:
@@ -258,6 +258,12 @@ def main():
             e2, w2, k2, _ = run_policy(sg, G, sources, p, 15.0, 24.0, 1, 0, False)
             extra[f"gteps_{name}"] = round(e2 / w2 / 1e9, 3)
             log(f"{name}: {e2 / w2 / 1e9:.2f} GTEPS wall")
+        weights = os.path.join(ROOT, "sygraph_amd", "data", "bfs_policy.f32")
+        if os.path.exists(weights):
+            pol = sg.Policy(dev, np.fromfile(weights, dtype=np.float32))
+            e2, w2, _, _ = run_policy(sg, G, sources, sg.POLICY_LEARNED, 15.0, 24.0, a.steps, 1, False, learned=pol)
+            extra["gteps_learned"] = round(e2 / w2 / 1e9, 3)
+            log(f"learned: {e2 / w2 / 1e9:.2f} GTEPS wall (sygraph_amd/data/bfs_policy.f32)")
 
     cpu = None
     if not a.no_cpu_baseline:

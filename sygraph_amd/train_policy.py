@@ -1,0 +1,206 @@
+"""Train the learned per-level BFS policy on the device and evaluate it.
+
+    python -m sygraph_amd.train_policy [--out sygraph_amd/data] [--quick]
+
+1. Graph sets.  Training: seeds 100-299; held-out: seeds 300-349.  The seed
+   fixes the family and size: RMAT (scale 16-22, edge factor 8/16/32,
+   a in [0.45, 0.60], b = c = (1 - a) / 2.6), 4-neighbour grids (side
+   256-1024, 0-30 % of edges deleted) and Erdos-Renyi graphs (2^16-2^21
+   vertices, average degree 4-32).  All are generated on the device.
+2. Per-level costs.  From four non-isolated sources per graph, the BFS runs
+   once per action with that action forced at every level (push or pull x
+   work mapping variant 0/1/2).  The frontier sequence of a BFS does not
+   depend on the actions, so level L of every run sees the same state and
+   cost(L, a) = traversal + conversion time of level L in the run forced to a.
+   Each level L >= 1 becomes one sample: the features the device computes
+   when it decides level L (policy.hpp), labelled with argmin_a cost(L, a).
+3. Training: full-batch Adam on the device (sg_policy_train_step) on
+   samples resampled in proportion to the level's stake (slowest minus fastest
+   action time), so the levels that dominate traversal time dominate the loss.
+4. Held-out evaluation: the summed device time of the BFS under the plain
+   policy (push, bucketing), the reference heuristic (alpha 15, beta 24), the
+   learned policy, and the per-level best action (a lower bound).
+
+Writes <out>/bfs_policy.f32 (1,542 float32 parameters) and
+<out>/bfs_policy.json (sets, sample counts, training loss, held-out times).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+import sygraph_amd as sg  # noqa: E402
+
+ACTIONS = [(sg.POLICY_PUSH, sg.LB_BUCKETING), (sg.POLICY_PUSH, sg.LB_WORKGROUP), (sg.POLICY_PUSH, sg.LB_SUBGROUP),
+           (sg.POLICY_PULL, sg.LB_BUCKETING), (sg.POLICY_PULL, sg.LB_WORKGROUP), (sg.POLICY_PULL, sg.LB_SUBGROUP)]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_graph(dev, seed: int, quick: bool):
+    """The graph of a seed (family and size drawn from the seed)."""
+    rng = np.random.default_rng(seed)
+    kind = rng.choice(["rmat", "rmat", "rmat", "grid", "er"])
+    top = 19 if quick else 22
+    if kind == "rmat":
+        scale = int(rng.integers(16, top + 1))
+        ef = int(rng.choice([8, 16, 32]))
+        a = float(rng.uniform(0.45, 0.60))
+        bc = (1.0 - a) / 2.6
+        return f"rmat s{scale} ef{ef} a{a:.3f}", sg.Graph.rmat(dev, scale, ef, a, bc, bc, seed=seed)
+    if kind == "grid":
+        side = int(rng.choice([128, 256, 512] if quick else [256, 512, 1024]))
+        p = float(rng.uniform(0.0, 0.3))
+        return f"grid {side}x{side} p{p:.2f}", sg.Graph.grid(dev, side, side, p, seed=seed)
+    lg = int(rng.integers(16, (19 if quick else 21) + 1))
+    deg = int(rng.choice([4, 8, 16, 32]))
+    n = 1 << lg
+    return f"er 2^{lg} d{deg}", sg.Graph.erdos_renyi(dev, n, n * deg // 2, seed=seed)
+
+
+def sources_of(G, seed: int, k: int = 4):
+    off, _, _ = G.download()
+    cand = np.nonzero(np.diff(off) > 0)[0]
+    rng = np.random.default_rng(seed + 7)
+    return [int(x) for x in rng.choice(cand, min(k, len(cand)), replace=False)]
+
+
+def level_costs(G, src: int):
+    """cost[L][a] for every action a forced at every level, and the stats of
+    the first run (the frontier sequence is the same in all runs)."""
+    b = sg.BFS(G)
+    costs, stats0 = [], None
+    for pol, lb in ACTIONS:
+        b.init(src)
+        st = b.run(pol, 15.0, 24.0, load_balancer=lb)
+        c = [st.level[i].kernel_ms + st.level[i].convert_ms for i in range(st.levels)]
+        costs.append(c)
+        if stats0 is None:
+            stats0 = [(st.level[i].frontier_vertices, st.level[i].frontier_edges, st.level[i].unexplored_edges,
+                       st.level[i].max_degree) for i in range(st.levels)]
+    L = min(len(c) for c in costs)
+    return np.array([[costs[a][i] for a in range(len(ACTIONS))] for i in range(L)]), stats0[:L]
+
+
+def level_features(stats0, n: int, m: int):
+    """Features of the decision for level L (L >= 1), as policy.hpp computes
+    them on the device: the frontier entering L, the arcs still unexplored
+    after L-1, the largest degree discovered at L-1."""
+    out = []
+    for L in range(1, len(stats0)):
+        fv, fe, _, _ = stats0[L]
+        _, _, unexplored, maxdeg = stats0[L - 1]
+        out.append([np.log2(1 + fv) / 24, np.log2(1 + fe) / 32, np.log2(1 + unexplored) / 32, L / 32,
+                    np.log2(1 + fe / max(fv, 1)) / 16, np.log2(1 + maxdeg) / 24, np.log2(1 + m / max(n, 1)) / 8,
+                    np.log2(max(n, 1)) / 32])
+    return np.array(out, dtype=np.float32).reshape(-1, 8)
+
+
+def collect(dev, seeds, quick):
+    X, Y, C = [], [], []
+    for seed in seeds:
+        name, G = make_graph(dev, seed, quick)
+        for src in sources_of(G, seed):
+            cost, stats0 = level_costs(G, src)
+            if len(stats0) < 2:
+                continue
+            X.append(level_features(stats0, G.n, G.m))
+            Y.append(np.argmin(cost[1:], axis=1).astype(np.int32))
+            C.append(cost[1:])
+        log(f"  seed {seed}: {name} n={G.n} m={G.m}")
+        G.destroy()
+    return np.concatenate(X), np.concatenate(Y), np.concatenate(C)
+
+
+def train(dev, X, Y, C, epochs: int, lr: float, seed: int = 0):
+    """Cost-sensitive training: samples are drawn in proportion to how much
+    the choice matters at that level (slowest minus fastest action), so the
+    loss is dominated by the levels that dominate the traversal time."""
+    rng = np.random.default_rng(seed)
+    stake = C.max(axis=1) - C.min(axis=1)
+    prob = (stake + 1e-4) / (stake + 1e-4).sum()
+    idx = rng.choice(len(Y), size=4 * len(Y), replace=True, p=prob)
+    Xb, Yb = dev.to_device(X[idx]), dev.to_device(Y[idx])
+    scale = np.concatenate([np.full(256, np.sqrt(2 / 8)), np.zeros(32), np.full(1024, np.sqrt(2 / 32)), np.zeros(32),
+                            np.full(192, np.sqrt(1 / 32)), np.zeros(6)])
+    pol = sg.Policy(dev, (rng.normal(0, 1, sg.POLICY_PARAMS) * scale).astype(np.float32))
+    losses = []
+    for ep in range(epochs):
+        losses.append(pol.train_step(Xb, Yb, len(idx), lr))
+    return pol, losses
+
+
+def evaluate(dev, pol, seeds, quick):
+    rows = []
+    for seed in seeds:
+        name, G = make_graph(dev, seed, quick)
+        b = sg.BFS(G)
+        for src in sources_of(G, seed):
+            t = {}
+            for key, kw in [("plain", dict(policy=sg.POLICY_PUSH)), ("heuristic", dict(policy=sg.POLICY_HYBRID)),
+                            ("learned", dict(policy=sg.POLICY_LEARNED, learned=pol))]:
+                best = None
+                for _ in range(3):  # best of three, device time of the whole traversal
+                    b.init(src)
+                    st = b.run(alpha=15.0, beta=24.0, **kw)
+                    best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+                t[key] = best
+            cost, _ = level_costs(G, src)
+            t["per_level_best"] = float(cost.min(axis=1).sum())
+            rows.append({"seed": seed, "graph": name, "source": src, **t})
+        G.destroy()
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "data"))
+    ap.add_argument("--quick", action="store_true", help="smaller graphs and sets (smoke run)")
+    ap.add_argument("--epochs", type=int, default=1500)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    a = ap.parse_args()
+    dev = sg.Device(0)
+    train_seeds = list(range(100, 300)) if not a.quick else list(range(100, 116))
+    test_seeds = list(range(300, 350)) if not a.quick else list(range(300, 306))
+    t0 = time.time()
+    log("collecting training levels ...")
+    X, Y, C = collect(dev, train_seeds, a.quick)
+    log(f"{len(Y)} samples, class counts {np.bincount(Y, minlength=6).tolist()} ({time.time() - t0:.1f}s)")
+    pol, losses = train(dev, X, Y, C, a.epochs, a.lr)
+    pred = np.argmax(pol.forward(X), axis=1)
+    acc = float((pred == Y).mean())
+    regret = float((C[np.arange(len(Y)), pred] - C.min(axis=1)).sum() / C.min(axis=1).sum())
+    log(f"train: loss {losses[0]:.3f} -> {losses[-1]:.3f}, accuracy {acc:.3f}, time regret {regret:.3%}")
+    rows = evaluate(dev, pol, test_seeds, a.quick)
+    tot = {k: float(sum(r[k] for r in rows)) for k in ["plain", "heuristic", "learned", "per_level_best"]}
+    geo = lambda num, den: float(np.exp(np.mean([np.log(r[num] / r[den]) for r in rows])))  # noqa: E731
+    summary = {"held_out_total_ms": tot,
+               "learned_speedup_vs_plain_total": tot["plain"] / tot["learned"],
+               "learned_speedup_vs_heuristic_total": tot["heuristic"] / tot["learned"],
+               "learned_speedup_vs_plain_geomean": geo("plain", "learned"),
+               "learned_speedup_vs_heuristic_geomean": geo("heuristic", "learned")}
+    log(json.dumps(summary, indent=1))
+    os.makedirs(a.out, exist_ok=True)
+    pol.get_params().astype(np.float32).tofile(os.path.join(a.out, "bfs_policy.f32"))
+    meta = {"train_seeds": [train_seeds[0], train_seeds[-1]], "held_out_seeds": [test_seeds[0], test_seeds[-1]],
+            "quick": a.quick, "samples": int(len(Y)), "class_counts": np.bincount(Y, minlength=6).tolist(),
+            "epochs": a.epochs, "lr": a.lr, "loss_first": losses[0], "loss_last": losses[-1],
+            "train_accuracy": acc, "train_time_regret": regret, "held_out": summary, "held_out_runs": rows}
+    with open(os.path.join(a.out, "bfs_policy.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()
