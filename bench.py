@@ -149,14 +149,16 @@ def batched_vs_plain(sg, G, sources16):
     return {k: round(v, 3) for k, v in out.items()}
 
 
-def run_sssp(sg, G, sources):
+def run_sssp(sg, G, sources, delta=0.0):
     s = sg.SSSP(G)
+    s.init(sources[0])
+    s.run(delta=delta)  # workspace and mean weight outside the timed loop
     edges = 0
     G.dev.synchronize()
     t0 = time.perf_counter()
     for src in sources:
         s.init(src)
-        st = s.run()
+        st = s.run(delta=delta)
         edges += st.reached_edges
     G.dev.synchronize()
     return edges, time.perf_counter() - t0
@@ -251,7 +253,13 @@ def main():
             f"hybrid {bp['hybrid']:.1f} GTEPS")
         e2, w2 = run_sssp(sg, G, sources)
         extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
-        log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3)")
+        log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3, automatic delta)")
+        sweep = {}
+        for d in [-1.0, 4.0, 8.0, 16.0, 64.0]:
+            e3, w3 = run_sssp(sg, G, sources[:4], d)
+            sweep["bellman_ford" if d < 0 else f"delta_{int(d)}"] = round(e3 / w3 / 1e9, 3)
+        extra["gteps_sssp_by_delta"] = sweep
+        log(f"sssp by delta (4 sources): {sweep}")
         for name, p in [("push", sg.POLICY_PUSH), ("pull", sg.POLICY_PULL), ("hybrid", sg.POLICY_HYBRID)]:
             if name == a.policy:
                 continue

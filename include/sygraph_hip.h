@@ -242,6 +242,11 @@ typedef struct sg_bfs_options {
                               one level ahead) */
   sg_policy_t learned;     /* SG_POLICY_LEARNED: the trained policy choosing direction and work mapping
                               per level */
+  /* tuning knobs (0 = default); the depths never depend on them */
+  int32_t t_medium;        /* degree from which a push vertex gets a wave (default 64) */
+  int32_t t_large;         /* degree from which its arcs are split into chunks (default 1024) */
+  int32_t chunk;           /* arcs per chunk of a large vertex (default 1024, at least 64) */
+  int32_t blocks_per_cu;   /* workgroups per compute unit of the traversal kernel (default 8) */
 } sg_bfs_options;
 
 typedef struct sg_level_stat {
@@ -312,6 +317,16 @@ typedef struct sg_sssp_stats {
  * min; dist (device, 4-byte, same type as the weights) unreached = n+1.
  * stats may be NULL. */
 int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats);
+
+typedef struct sg_sssp_options {
+  int32_t load_balancer; /* sg_load_balancer (the bucketing bins in every case) */
+  double delta;          /* bucket width of the near/far split: 0 = automatic (1/4 of the mean arc
+                            weight), negative or infinite = none, i.e. every dirty vertex is relaxed
+                            each round (the reference's frontier Bellman-Ford, sssp.hpp:152-210) */
+} sg_sssp_options;
+void sg_sssp_options_default(sg_sssp_options* opt);
+/* sg_sssp with options; the distances do not depend on delta. */
+int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt, void* dist, sg_sssp_stats* stats);
 /* CC::run (algorithms/cc.hpp:103-169): labels (device uint32[n]) converge to
  * the largest vertex id of each component. */
 int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);

@@ -67,7 +67,8 @@ class FrontierView(C.Structure):
 
 class BfsOptions(C.Structure):
     _fields_ = [("policy", C.c_int32), ("load_balancer", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
-                ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p)]
+                ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p),
+                ("t_medium", C.c_int32), ("t_large", C.c_int32), ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32)]
 
 
 class LevelStat(C.Structure):
@@ -82,6 +83,10 @@ class SsspStats(C.Structure):
     _fields_ = [("rounds", C.c_int32), ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
                 ("relaxations", C.c_int64), ("kernel_ms", C.c_float),
                 ("round_vertices", C.c_int64 * MAX_LEVELS_RECORDED), ("round_edges", C.c_int64 * MAX_LEVELS_RECORDED)]
+
+
+class SsspOptions(C.Structure):
+    _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double)]
 
 
 class BfsStats(C.Structure):
@@ -109,6 +114,8 @@ _SIGS = {
     "sg_memcpy": (_I, [_P, _P, _P, C.c_size_t]),
     "sg_fill": (_I, [_P, _P, C.c_uint64, _I, C.c_size_t]),
     "sg_copy_device": (_I, [_P, _P, _P, C.c_size_t]),
+    "sg_sssp_options_default": (None, [_P]),
+    "sg_sssp_ex": (_I, [_P, C.c_int32, _P, _P, _P]),
     "sg_policy_create": (_I, [_P, _P]),
     "sg_policy_destroy": (_I, [_P]),
     "sg_policy_set_params": (_I, [_P, _P]),
@@ -492,7 +499,7 @@ def swap(a: Frontier, b: Frontier):
 # -------------------------------------------------------------- algorithms --
 def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, alpha: float = 15.0,
                 beta: float = 24.0, record_levels: bool = False, sync_per_level: bool = False,
-                learned: Optional["Policy"] = None) -> BfsOptions:
+                learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsOptions:
     o = BfsOptions()
     lib().sg_bfs_options_default(C.byref(o))
     o.policy = policy
@@ -502,6 +509,8 @@ def bfs_options(policy: int = POLICY_PUSH, load_balancer: int = LB_BUCKETING, al
     o.record_levels = 1 if record_levels else 0
     o.sync_per_level = 1 if sync_per_level else 0
     o.learned = learned.h if learned is not None else None
+    for k, v in (tuning or {}).items():
+        setattr(o, k, int(v))
     return o
 
 
@@ -525,11 +534,12 @@ class BFS:
 
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
             load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
-            learned: Optional["Policy"] = None) -> BfsStats:
+            learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsStats:
+        """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
         o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,
-                        learned=learned)
+                        learned=learned, tuning=tuning)
         check(lib().sg_bfs(self.g.h, self.source, C.byref(o), C.c_void_p(self.depth.ptr), C.byref(self.stats)))
         return self.stats
 
@@ -564,10 +574,16 @@ class SSSP:
     def reset(self):
         self.source = None
 
-    def run(self, load_balancer: int = LB_BUCKETING) -> SsspStats:
+    def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0) -> SsspStats:
+        """delta: bucket width of the near/far split (0 automatic, negative or
+        inf for the reference's frontier Bellman-Ford)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
-        check(lib().sg_sssp(self.g.h, self.source, load_balancer, C.c_void_p(self.dist.ptr), C.byref(self.stats)))
+        o = SsspOptions()
+        lib().sg_sssp_options_default(C.byref(o))
+        o.load_balancer = load_balancer
+        o.delta = float(delta)
+        check(lib().sg_sssp_ex(self.g.h, self.source, C.byref(o), C.c_void_p(self.dist.ptr), C.byref(self.stats)))
         return self.stats
 
     def get_distances(self) -> np.ndarray:

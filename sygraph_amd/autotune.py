@@ -1,0 +1,138 @@
+"""Autotuner for the traversal knobs of one graph on one device.
+
+    from sygraph_amd import autotune
+    best = autotune.tune_bfs(G, sources)          # {"t_medium":..., "t_large":..., "chunk":..., "blocks_per_cu":...}
+    bfs.run(sg.POLICY_HYBRID, 15, 24, tuning=best)
+    delta = autotune.tune_sssp(G, sources)        # bucket width for SSSP.run(delta=...)
+
+Each candidate point is timed by the device time of whole traversals (best of
+`repeats` over the given sources).  Every point must reproduce the outputs of
+the default point exactly -- depths for BFS, distances for SSSP -- otherwise
+tuning stops with an error: tuning may change speed, never results.  The
+choice is the smallest time, ties going to the earlier point of the grid, so
+it is a pure function of the recorded timings (select()).  Results are cached
+in a JSON file keyed by the graph (vertex and arc counts and the CSR digests)
+and the device architecture; a cached graph is not re-measured.
+"""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+import sygraph_amd as sg
+
+BFS_GRID = {
+    "t_medium": [32, 64, 128],
+    "t_large": [512, 1024, 4096],
+    "chunk": [512, 1024, 2048],
+    "blocks_per_cu": [4, 8, 16],
+}
+SSSP_DELTAS = [-1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0, 128.0]
+
+
+def bfs_points(grid: Optional[dict] = None) -> list[dict]:
+    g = grid or BFS_GRID
+    keys = list(g)
+    return [dict(zip(keys, vals)) for vals in itertools.product(*(g[k] for k in keys))]
+
+
+def select(timings: Sequence[float]) -> int:
+    """Index of the fastest point; ties go to the earliest (deterministic)."""
+    best = 0
+    for i, t in enumerate(timings):
+        if t < timings[best]:
+            best = i
+    return best
+
+
+def graph_key(G) -> str:
+    a, b = G.digest()
+    arch = G.dev.info().arch.decode(errors="replace")
+    return f"{arch}:n{G.n}:m{G.m}:{a:016x}{b:016x}"
+
+
+def _load(cache: Optional[str]) -> dict:
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            return json.load(f)
+    return {}
+
+
+def _store(cache: Optional[str], data: dict):
+    if not cache:
+        return
+    tmp = cache + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, cache)
+
+
+def _digest(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def tune_bfs(G, sources: Sequence[int], grid: Optional[dict] = None, repeats: int = 3,
+             policy: int = sg.POLICY_HYBRID, cache: Optional[str] = None) -> dict:
+    key = f"bfs:{graph_key(G)}:p{policy}:s{','.join(map(str, sources))}"
+    data = _load(cache)
+    if key in data:
+        return data[key]["choice"]
+    points = bfs_points(grid)
+    b = sg.BFS(G)
+    ref = None
+    times = []
+    for pt in [{}] + points:  # the default point first, as the output reference
+        digests, total = [], 0.0
+        for src in sources:
+            best = None
+            for _ in range(repeats):
+                b.init(src)
+                st = b.run(policy, 15.0, 24.0, tuning=pt)
+                best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+            total += best
+            digests.append(_digest(b.get_distances()))
+        if ref is None:
+            ref = digests
+            continue
+        if digests != ref:
+            raise sg.SygraphError(sg.SG_ERR_STATE, f"tuning point {pt} changed the BFS depths")
+        times.append(total)
+    choice = points[select(times)]
+    data[key] = {"choice": choice, "points": points, "ms": times}
+    _store(cache, data)
+    return choice
+
+
+def tune_sssp(G, sources: Sequence[int], deltas: Sequence[float] = SSSP_DELTAS, repeats: int = 2,
+              cache: Optional[str] = None) -> float:
+    key = f"sssp:{graph_key(G)}:s{','.join(map(str, sources))}"
+    data = _load(cache)
+    if key in data:
+        return data[key]["choice"]
+    s = sg.SSSP(G)
+    ref = None
+    times = []
+    for d in list(deltas):
+        digests, total = [], 0.0
+        for src in sources:
+            best = None
+            for _ in range(repeats):
+                s.init(src)
+                st = s.run(delta=d)
+                best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+            total += best
+            digests.append(_digest(s.get_distances()))
+        if ref is None:
+            ref = digests
+        elif digests != ref:
+            raise sg.SygraphError(sg.SG_ERR_STATE, f"delta {d} changed the SSSP distances")
+        times.append(total)
+    choice = float(list(deltas)[select(times)])
+    data[key] = {"choice": choice, "points": list(deltas), "ms": times}
+    _store(cache, data)
+    return choice

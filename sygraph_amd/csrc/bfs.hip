@@ -502,9 +502,17 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   SG_TRY(sg::graph_ensure_isolated(g));
   if (opt.policy != SG_POLICY_PUSH) SG_TRY(sg::graph_ensure_pull_hint(g));
   const int64_t n = g->n, w0 = ws->w0;
-  const int grid_main = dev->cus * 8;
+  const int grid_main = dev->cus * (opt.blocks_per_cu > 0 ? opt.blocks_per_cu : 8);
   const int grid_small = std::max(1, std::min(dev->cus * 4, (int)((w0 + 255) / 256)));
-  const Tuning tune = default_tuning();
+  Tuning tune = default_tuning();
+  if (opt.t_medium > 0) tune.t_medium = opt.t_medium;
+  if (opt.t_large > 0) tune.t_large = opt.t_large;
+  if (opt.chunk > 0) tune.chunk = opt.chunk;
+  if (!ws->qs.fits(g->m, tune)) { /* queues sized for a coarser tuning: grow them */
+    SG_HIP(hipStreamSynchronize(s));
+    ws->qs.release();
+    SG_TRY(ws->qs.init(dev, n, g->m, tune));
+  }
   QueueSet& Q = ws->qs;
   const OffT* off = (const OffT*)g->offsets;
 
@@ -660,6 +668,12 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
            "sg_bfs: the learned policy needs a trained sg_policy_t in options.learned");
   SG_CHECK(opt.load_balancer >= SG_LB_WORKITEM && opt.load_balancer <= SG_LB_BUCKETING, SG_ERR_INVALID,
            "sg_bfs: load balancer %d out of range", opt.load_balancer);
+  SG_CHECK(opt.t_medium >= 0 && opt.t_large >= 0 && opt.chunk >= 0 && opt.blocks_per_cu >= 0 &&
+               opt.blocks_per_cu <= 64 && (opt.chunk == 0 || opt.chunk >= 64) &&
+               (opt.t_medium == 0 || opt.t_large == 0 || opt.t_large > opt.t_medium) &&
+               opt.t_medium <= (1 << 30) && opt.t_large <= (1 << 30) && opt.chunk <= (1 << 30),
+           SG_ERR_INVALID, "sg_bfs: tuning out of range (t_medium %d, t_large %d, chunk %d, blocks_per_cu %d)",
+           opt.t_medium, opt.t_large, opt.chunk, opt.blocks_per_cu);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
   if (stats) {

@@ -2,21 +2,29 @@
  * sssp.hip -- single-source shortest paths for gfx950 (SSSP::run,
  * algorithms/sssp.hpp:152-210).
  *
- * Frontier Bellman-Ford, as the reference: every round relaxes all arcs of
- * the frontier with an atomic minimum on the tentative distance
- * (sssp.hpp:180-189, sync/atomics.hpp:76-81); vertices whose distance dropped
- * form the next frontier (the filter of sssp.hpp:192-196 only deduplicates
- * them, which the bitmap does by construction).  Each round reads the
- * frontier through the degree-class queues (thread / wave / workgroup bins),
- * relaxes with a plain-load pre-check before the atomic, and marks improved
- * vertices in the next-frontier bitmap with a test-then-OR.  Distances are
- * unique, so the result is independent of the relaxation order.
+ * Label-correcting relaxation, as the reference: a round relaxes all arcs of
+ * its frontier with an atomic minimum on the tentative distance
+ * (sssp.hpp:180-189, sync/atomics.hpp:76-81) and vertices whose distance
+ * dropped become dirty again (the filter of sssp.hpp:192-196 only
+ * deduplicates them, which a bitmap does by construction).  The reference
+ * relaxes every dirty vertex each round (frontier Bellman-Ford); here a round
+ * takes only the dirty vertices with distance below a threshold T (the near
+ * set) and defers the rest, raising T to (smallest deferred distance) + delta
+ * when the near set runs dry -- the bucket order of delta-stepping, which
+ * relaxes far fewer arcs on weighted graphs.  delta = infinity is exactly the
+ * reference's schedule.  Each round reads its frontier through the
+ * degree-class queues (thread / wave / workgroup bins), relaxes with a
+ * plain-load pre-check before the atomic, and marks improved vertices dirty
+ * with a test-then-OR.  Shortest distances are unique, so the result does not
+ * depend on delta or on the relaxation order.
  *
  * Distances have the weights' 4-byte type: uint32 weights use integer min,
  * float weights (non-negative) compare through their bit patterns.  The
  * unreached value is n+1 in that type (sssp.hpp:43).
  */
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstring>
 
 #include "traverse.hpp"
@@ -29,11 +37,13 @@ template<typename W>
 struct DistOps;
 template<>
 struct DistOps<uint32_t> {
+  static constexpr bool is_float = false;
   __host__ __device__ static uint32_t bits(uint32_t x) { return x; }
   __host__ __device__ static uint32_t value(uint32_t b) { return b; }
 };
 template<>
 struct DistOps<float> {
+  static constexpr bool is_float = true;
   __host__ __device__ static uint32_t bits(float x) { return __builtin_bit_cast(uint32_t, x); }
   __host__ __device__ static float value(uint32_t b) { return __builtin_bit_cast(float, b); }
 };
@@ -52,8 +62,6 @@ struct SsspArgs {
 
 template<typename OffT, typename W>
 __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < a.w0; i += (int64_t)gridDim.x * kBlock)
-    a.zero_bm[i] = 0ull;
   for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
     const W du = DistOps<W>::value(atomic_load(&a.dist[u]));
     const W nd = du + a.w[e];
@@ -68,6 +76,52 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a) {
   });
 }
 
+/* per-block counters of the split, one 128-byte line per slot */
+struct alignas(128) SplitSlot {
+  unsigned long long far_count;
+  unsigned int far_min; /* distance bits; 4-byte distances order like their bits */
+  unsigned int pad0;
+  unsigned long long pad[14];
+};
+
+/* near = dirty vertices with distance < T (moved out of the dirty bitmap into
+ * the round's frontier bitmap); count and smallest distance of the rest */
+template<typename W>
+__global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, const uint32_t* dist,
+                                                       W T, int64_t w0, SplitSlot* slots) {
+  unsigned long long far = 0;
+  unsigned int fmin = 0xFFFFFFFFu;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
+    const uint64_t f = dirty[i];
+    uint64_t nb = 0;
+    for (uint64_t rest = f; rest; rest &= rest - 1) {
+      const int b = __ffsll((unsigned long long)rest) - 1;
+      const uint32_t db = dist[i * 64 + b];
+      if (DistOps<W>::value(db) < T) nb |= 1ull << b;
+      else fmin = db < fmin ? db : fmin;
+    }
+    near_bm[i] = nb;
+    if (nb) dirty[i] = f & ~nb;
+    far += (unsigned long long)__popcll(f & ~nb);
+  }
+  far = wave_sum_u64(far);
+  unsigned long long m = wave_max_u64(~(unsigned long long)fmin); /* min via max of complement */
+  fmin = ~(unsigned int)m;
+  if (lane_id() == 0 && (far || fmin != 0xFFFFFFFFu)) {
+    SplitSlot* sl = slots + (blockIdx.x % kSlots);
+    if (far) atomic_add(&sl->far_count, far);
+    atomic_min(&sl->far_min, fmin);
+  }
+}
+
+__global__ void split_slots_reset_kernel(SplitSlot* slots) {
+  const int i = threadIdx.x;
+  if (i < kSlots) {
+    slots[i].far_count = 0;
+    slots[i].far_min = 0xFFFFFFFFu;
+  }
+}
+
 template<typename OffT>
 __global__ void sssp_init_kernel(const OffT* off, uint32_t* dist, uint32_t unreached, int64_t n, int32_t src,
                                  QEntry* q0, QEntry* q1, QEntry* q2, QueueHdr* hdr, Tuning t) {
@@ -77,6 +131,18 @@ __global__ void sssp_init_kernel(const OffT* off, uint32_t* dist, uint32_t unrea
     QEntry* const q[3] = {q0, q1, q2};
     enqueue_single(off, t, q, hdr, src);
   }
+}
+
+__global__ void set_bit_kernel(uint64_t* bm, int32_t v) { bm[v >> 6] |= 1ull << (v & 63); }
+
+template<typename W>
+__global__ void weight_sum_kernel(const W* w, int64_t m, double* out) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)w[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane_id() == 0) atomicAdd(out, acc);
 }
 
 template<typename OffT>
@@ -97,8 +163,9 @@ __global__ void reached_kernel(const OffT* off, const uint32_t* dist, uint32_t u
 }
 
 struct SsspWorkspace {
-  uint64_t* bm[2] = {nullptr, nullptr};
+  uint64_t* bm[2] = {nullptr, nullptr}; /* dirty vertices, near set of the round */
   unsigned long long* acc = nullptr;
+  SplitSlot* slots = nullptr;
   QueueSet qs;
   int64_t w0 = 0;
 };
@@ -108,29 +175,43 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->bm[0]);
   (void)hipFree(w->bm[1]);
   (void)hipFree(w->acc);
+  (void)hipFree(w->slots);
   w->qs.release();
   delete w;
 }
 
-template<typename OffT, typename W>
-int run_sssp(sg_graph_t g, int32_t src, W* dist_out, sg_sssp_stats* stats) {
-  sg_device_t dev = g->dev;
-  hipStream_t s = dev->stream;
-  const int64_t n = g->n;
-  const Tuning tune = default_tuning();
+constexpr double kAutoDeltaFactor = 0.25; /* automatic delta = 1/4 of the mean arc weight */
+
+int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
   auto* ws = static_cast<SsspWorkspace*>(g->sssp_ws);
   if (!ws) {
+    sg_device_t dev = g->dev;
+    const int64_t n = g->n;
     ws = new SsspWorkspace();
     ws->w0 = (n + 63) / 64;
     int st = SG_OK;
     if ((st = sg::alloc_device(dev, (void**)&ws->bm[0], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->bm[1], 8 * (ws->w0 + 1))) ||
-        (st = sg::alloc_device(dev, (void**)&ws->acc, 16)) || (st = ws->qs.init(dev, n, g->m, tune))) {
+        (st = sg::alloc_device(dev, (void**)&ws->acc, 16)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
+        (st = ws->qs.init(dev, n, g->m, default_tuning()))) {
       sssp_ws_free(ws);
       return st;
     }
     g->sssp_ws = ws;
   }
+  *out = ws;
+  return SG_OK;
+}
+
+template<typename OffT, typename W>
+int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats* stats) {
+  sg_device_t dev = g->dev;
+  hipStream_t s = dev->stream;
+  const int64_t n = g->n;
+  const Tuning tune = default_tuning();
+  SsspWorkspace* ws = nullptr;
+  SG_TRY(sssp_workspace_for(g, &ws));
   QueueSet& Q = ws->qs;
   const OffT* off = (const OffT*)g->offsets;
   uint32_t* dist = reinterpret_cast<uint32_t*>(dist_out);
@@ -143,22 +224,52 @@ int run_sssp(sg_graph_t g, int32_t src, W* dist_out, sg_sssp_stats* stats) {
     SG_HIP(hipEventCreate(&e1));
     SG_HIP(hipEventRecord(e0, s));
   }
-  SG_TRY(sg::zero_async(dev, ws->bm[0], 8 * ws->w0));
-  SG_TRY(sg::zero_async(dev, ws->bm[1], 8 * ws->w0));
+  uint64_t* dirty = ws->bm[0];
+  uint64_t* near_bm = ws->bm[1];
+  SG_TRY(sg::zero_async(dev, dirty, 8 * ws->w0));
   sssp_init_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(off, dist, unreached, n, src, Q.q[0][0],
                                                                        Q.q[0][1], Q.q[0][2], Q.hdr, tune);
+  set_bit_kernel<<<1, 1, 0, s>>>(dirty, src);
   SG_HIP(hipGetLastError());
+  const bool bellman = !(delta > 0.0) || !std::isfinite(delta);
+  const double w_max = DistOps<W>::is_float ? 3.0e38 : 4294967295.0;
+  double T = bellman ? w_max : std::min(delta, w_max);
   QueueHdr* hh = (QueueHdr*)dev->host_scratch;
-  int qs = 0;
+  SplitSlot* hslots = (SplitSlot*)((char*)dev->host_scratch + 4096);
+  const int grid_w = sg::grid_for(dev, ws->w0, kBlock, 4);
   int32_t round = 0;
-  int64_t fv = 1, fe = 0;
-  {
+  int64_t relaxed = 0, dry_steps = 0;
+  for (;;) {
+    split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots);
+    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, dist, (W)T, ws->w0, ws->slots);
+    SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, Q, 0, off, nullptr, nullptr, near_bm, tune)));
     SG_HIP(hipMemcpyAsync(hh, Q.hdr, sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipMemcpyAsync(hslots, ws->slots, sizeof(SplitSlot) * kSlots, hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
-    fe = (int64_t)hh->deg;
-  }
-  int64_t relaxed = 0;
-  for (;; ++round) {
+    const int64_t fv = (int64_t)hh->nvert, fe = (int64_t)hh->deg;
+    if (fv == 0) {
+      unsigned long long far = 0;
+      uint32_t fmin = 0xFFFFFFFFu;
+      for (int k = 0; k < kSlots; ++k) {
+        far += hslots[k].far_count;
+        fmin = std::min(fmin, hslots[k].far_min);
+      }
+      if (far == 0) break;
+      /* the near set ran dry: the next bucket starts at the smallest deferred
+       * distance and must include it (at least one representable step above) */
+      const W dmin = DistOps<W>::value(fmin);
+      W t_next = (W)std::min((double)dmin + delta, w_max);
+      if (!(t_next > dmin)) {
+        if constexpr (DistOps<W>::is_float) t_next = std::nextafter(dmin, std::numeric_limits<float>::infinity());
+        else t_next = dmin + 1u;
+      }
+      T = (double)t_next;
+      if (++dry_steps > 2 * n + 64) {
+        sg::set_error("sg_sssp: no progress after %lld empty buckets", (long long)dry_steps);
+        return SG_ERR_STATE;
+      }
+      continue;
+    }
     if (stats && round < SG_MAX_LEVELS_RECORDED) {
       stats->round_vertices[round] = fv;
       stats->round_edges[round] = fe;
@@ -168,23 +279,13 @@ int run_sssp(sg_graph_t g, int32_t src, W* dist_out, sg_sssp_stats* stats) {
     a.cols = g->cols;
     a.w = (const W*)g->weights;
     a.dist = dist;
-    a.next_bm = ws->bm[round & 1];
-    a.zero_bm = ws->bm[(round + 1) & 1];
+    a.next_bm = dirty;
     a.w0 = ws->w0;
-    for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[qs][k];
-    a.qin_hdr = Q.hdr + qs;
+    for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[0][k];
+    a.qin_hdr = Q.hdr;
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a);
     SG_HIP(hipGetLastError());
-    SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, Q, qs ^ 1, off, nullptr, nullptr, ws->bm[round & 1], tune)));
-    SG_HIP(hipMemcpyAsync(hh, Q.hdr + (qs ^ 1), sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));
-    qs ^= 1;
-    fv = (int64_t)hh->nvert;
-    fe = (int64_t)hh->deg;
-    if (fv == 0) {
-      ++round;
-      break;
-    }
+    ++round;
   }
   if (stats) {
     SG_HIP(hipEventRecord(e1, s));
@@ -219,19 +320,75 @@ void sssp_workspace_release(sg_graph_t g) {
 }
 } // namespace sg
 
-extern "C" int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats) {
+/* mean arc weight, computed once per graph (automatic delta) */
+template<typename W>
+int mean_weight(sg_graph_t g, double* out) {
+  if (g->mean_weight >= 0.0) {
+    *out = g->mean_weight;
+    return SG_OK;
+  }
+  SsspWorkspace* ws = nullptr;
+  SG_TRY(sssp_workspace_for(g, &ws));
+  double* acc = reinterpret_cast<double*>(ws->acc);
+  SG_TRY(sg::zero_async(g->dev, acc, 16));
+  weight_sum_kernel<W><<<sg::grid_for(g->dev, g->m, 256, 4), 256, 0, g->dev->stream>>>((const W*)g->weights, g->m,
+                                                                                       acc);
+  double h = 0.0;
+  SG_HIP(hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, g->dev->stream));
+  SG_HIP(hipStreamSynchronize(g->dev->stream));
+  g->mean_weight = g->m > 0 ? h / (double)g->m : 1.0;
+  *out = g->mean_weight;
+  return SG_OK;
+}
+
+template<typename OffT, typename W>
+int run_sssp_auto(sg_graph_t g, int32_t source, double delta, W* dist, sg_sssp_stats* stats) {
+  if (delta == 0.0) {
+    double mw = 1.0;
+    SG_TRY(mean_weight<W>(g, &mw));
+    delta = std::max(kAutoDeltaFactor * mw, DistOps<W>::is_float ? 1e-30 : 1.0);
+  }
+  return run_sssp<OffT, W>(g, source, delta, dist, stats);
+}
+
+extern "C" {
+
+void sg_sssp_options_default(sg_sssp_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->load_balancer = SG_LB_BUCKETING;
+  opt->delta = 0.0;
+}
+
+int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void* dist, sg_sssp_stats* stats) {
   SG_CHECK(g && dist, SG_ERR_INVALID, "sg_sssp: null argument");
   SG_CHECK(source >= 0 && source < g->n, SG_ERR_INVALID, "sg_sssp: source %d out of range [0,%lld)", source,
            (long long)g->n);
   SG_CHECK(g->weights, SG_ERR_STATE, "sg_sssp: graph has no edge weights");
-  SG_CHECK(load_balancer >= SG_LB_WORKITEM && load_balancer <= SG_LB_BUCKETING, SG_ERR_INVALID,
-           "sg_sssp: unknown load balancer %d", load_balancer);
-  if (stats) std::memset(stats, 0, sizeof(*stats));
+  sg_sssp_options opt;
+  if (opt_in) opt = *opt_in;
+  else sg_sssp_options_default(&opt);
+  SG_CHECK(opt.load_balancer >= SG_LB_WORKITEM && opt.load_balancer <= SG_LB_BUCKETING, SG_ERR_INVALID,
+           "sg_sssp: unknown load balancer %d", opt.load_balancer);
+  if (stats) {
+    stats->rounds = 0;
+    stats->reached_vertices = stats->reached_edges = stats->relaxations = 0;
+    stats->kernel_ms = 0.f;
+  }
   SG_HIP(hipSetDevice(g->dev->ordinal));
   const bool fw = (g->flags & SG_GRAPH_WEIGHT_FLOAT) != 0;
   if (g->offset_bytes == 8)
-    return fw ? run_sssp<int64_t, float>(g, source, (float*)dist, stats)
-              : run_sssp<int64_t, uint32_t>(g, source, (uint32_t*)dist, stats);
-  return fw ? run_sssp<uint32_t, float>(g, source, (float*)dist, stats)
-            : run_sssp<uint32_t, uint32_t>(g, source, (uint32_t*)dist, stats);
+    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, (float*)dist, stats)
+              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, (uint32_t*)dist, stats);
+  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, (float*)dist, stats)
+            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, (uint32_t*)dist, stats);
 }
+
+int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats) {
+  sg_sssp_options opt;
+  sg_sssp_options_default(&opt);
+  opt.load_balancer = load_balancer;
+  return sg_sssp_ex(g, source, &opt, dist, stats);
+}
+
+} // extern "C"

@@ -521,11 +521,15 @@ struct QueueSet {
   TileCount* blocks = nullptr;
   int64_t nblocks = 0;
 
+  /* largest number of class-2 chunks any frontier can produce under t */
+  static int64_t large_capacity(int64_t m, const Tuning& t) { return m / t.chunk + m / t.t_large + 64; }
+  bool fits(int64_t m, const Tuning& t) const { return large_capacity(m, t) <= cap_large; }
+
   int init(sg_device_t dev, int64_t n, int64_t m, const Tuning& t) {
     w0 = (n + 63) / 64;
     ntiles = (w0 + 63) / 64;
     nblocks = (ntiles + kBlock / 64 - 1) / (kBlock / 64);
-    cap_large = m / t.chunk + m / t.t_large + 64;
+    cap_large = large_capacity(m, t);
     int st = SG_OK;
     if ((st = alloc_device(dev, (void**)&hdr, sizeof(QueueHdr) * 2)) ||
         (st = alloc_device(dev, (void**)&tiles, sizeof(TileCount) * (ntiles + 1))) ||

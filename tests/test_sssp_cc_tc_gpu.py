@@ -65,6 +65,50 @@ def test_sssp_random_graphs(sg, device, oracle_mod, lb):
             assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, int(src)))
 
 
+@pytest.mark.parametrize("delta", [float("inf"), -1.0, 0.0, 0.5, 1.0, 7.0, 40.0, 1000.0])
+def test_sssp_bucket_width_does_not_change_distances(sg, device, oracle_mod, delta):
+    g = oracle_mod.rmat(12, 16, seed=44)
+    oracle_mod.fill_weights(g, 5)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+    s = sg.SSSP(G)
+    for src in (0, int(np.argmax(g.degrees()))):
+        s.init(src)
+        st = s.run(delta=delta)
+        assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src))
+    if delta in (float("inf"), -1.0):  # the reference's schedule: every dirty vertex each round
+        assert st.relaxations >= st.reached_edges
+
+
+def test_sssp_buckets_relax_fewer_arcs(sg, device, oracle_mod):
+    g = oracle_mod.rmat(13, 16, seed=45)
+    oracle_mod.fill_weights(g, 6)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+    s = sg.SSSP(G)
+    src = int(np.argmax(g.degrees()))
+    s.init(src)
+    bf = s.run(delta=-1.0).relaxations
+    s.init(src)
+    auto = s.run().relaxations
+    assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src))
+    assert auto < bf, (auto, bf)
+
+
+@pytest.mark.parametrize("delta", [0.0, 0.3, -1.0])
+def test_sssp_float_weights_buckets(sg, device, oracle_mod, delta):
+    g = oracle_mod.rmat(11, 8, seed=46)
+    w = oracle_mod.fill_weights(g, 5)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=(w / 7.0).astype(np.float32), weight_float=True)
+    s = sg.SSSP(G)
+    s.init(9)
+    s.run(delta=delta)
+    ref = oracle_mod.sssp(g, 9).astype(np.float64)
+    got = s.get_distances().astype(np.float64)
+    reach = ref != g.n + 1
+    # float sums differ from the integer oracle only by rounding (path sums of w/7)
+    assert np.array_equal(got == g.n + 1, ~reach)
+    assert np.allclose(got[reach], ref[reach] / 7.0, rtol=1e-5)
+
+
 def test_sssp_float_weights(sg, device, oracle_mod):
     g = oracle_mod.rmat(11, 8, seed=41)
     w = oracle_mod.fill_weights(g, 5)
