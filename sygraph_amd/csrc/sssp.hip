@@ -60,8 +60,27 @@ struct SsspArgs {
   const QueueHdr* qin_hdr;
 };
 
+/* Device-resident control block of the round loop (written only by
+ * sssp_finish_kernel; kernel boundaries order the accesses). */
+struct SsspCtl {
+  uint32_t T_bits;  /* threshold: near = distance below T */
+  int32_t done;     /* no dirty vertex left: later iterations are no-ops */
+  int32_t relax;    /* the current iteration relaxes (its near set is not empty) */
+  int32_t rounds;   /* iterations that relaxed so far */
+  unsigned long long relaxed; /* arcs relaxed so far */
+};
+
+struct alignas(64) RoundLog {
+  int32_t iter, done, relax, rounds;
+  int64_t near_v, near_e;
+  unsigned long long relaxed;
+};
+
+constexpr int kRing = 8;
+
 template<typename OffT, typename W>
-__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a) {
+__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl) {
+  if (ctl->done || !ctl->relax) return;
   for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
     const W du = DistOps<W>::value(atomic_load(&a.dist[u]));
     const W nd = du + a.w[e];
@@ -86,9 +105,12 @@ struct alignas(128) SplitSlot {
 
 /* near = dirty vertices with distance < T (moved out of the dirty bitmap into
  * the round's frontier bitmap); count and smallest distance of the rest */
+
 template<typename W>
 __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, const uint32_t* dist,
-                                                       W T, int64_t w0, SplitSlot* slots) {
+                                                       const SsspCtl* ctl, int64_t w0, SplitSlot* slots) {
+  if (ctl->done) return;
+  const W T = DistOps<W>::value(ctl->T_bits);
   unsigned long long far = 0;
   unsigned int fmin = 0xFFFFFFFFu;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
@@ -114,12 +136,79 @@ __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t
   }
 }
 
-__global__ void split_slots_reset_kernel(SplitSlot* slots) {
+__global__ void split_slots_reset_kernel(SplitSlot* slots, const SsspCtl* ctl) {
+  if (ctl->done) return;
   const int i = threadIdx.x;
   if (i < kSlots) {
     slots[i].far_count = 0;
     slots[i].far_min = 0xFFFFFFFFu;
   }
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) near_count_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
+  if (ctl->done) return;
+  compact_count_body<OffT, kFromBitmap>(a);
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) near_write_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
+  if (ctl->done || !ctl->relax) return;
+  compact_write_body<OffT>(a);
+}
+
+/* Threshold after a dry iteration: the smallest deferred distance plus
+ * delta, at least one representable step above it. */
+template<typename W>
+__device__ __forceinline__ uint32_t next_threshold(uint32_t fmin_bits, double delta, double w_max) {
+  const W dmin = DistOps<W>::value(fmin_bits);
+  const double t = (double)dmin + delta;
+  W tn = (W)(t < w_max ? t : w_max);
+  if (!(tn > dmin)) {
+    /* non-negative finite floats order like their bit patterns: +1 is the next float up */
+    if constexpr (DistOps<W>::is_float) tn = DistOps<W>::value(fmin_bits + 1u);
+    else tn = dmin + 1u;
+  }
+  return DistOps<W>::bits(tn);
+}
+
+/* Scan of the near set's counts; relax / advance the threshold / stop; the
+ * iteration's record into the pinned ring. */
+template<typename W>
+__global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* blocks, int64_t nblocks, QueueHdr* hdr,
+                                                                   SsspCtl* ctl, const SplitSlot* slots,
+                                                                   double delta, double w_max, int32_t iter,
+                                                                   RoundLog* log) {
+  __shared__ QueueHdr tot;
+  if (ctl->done) return;
+  compact_scan_body(blocks, nblocks, hdr, &tot);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  RoundLog e{};
+  e.iter = iter;
+  e.near_v = (int64_t)tot.nvert;
+  e.near_e = (int64_t)tot.deg;
+  if (tot.nvert > 0) {
+    ctl->relax = 1;
+    ctl->rounds += 1;
+    ctl->relaxed += tot.deg;
+  } else {
+    ctl->relax = 0;
+    unsigned long long far = 0;
+    uint32_t fmin = 0xFFFFFFFFu;
+    for (int k = 0; k < kSlots; ++k) {
+      far += slots[k].far_count;
+      fmin = slots[k].far_min < fmin ? slots[k].far_min : fmin;
+    }
+    if (far == 0) ctl->done = 1;
+    else ctl->T_bits = next_threshold<W>(fmin, delta, w_max);
+  }
+  e.done = ctl->done;
+  e.relax = ctl->relax;
+  e.rounds = ctl->rounds;
+  e.relaxed = ctl->relaxed;
+  log[iter % kRing] = e;
+  __threadfence_system();
 }
 
 template<typename OffT>
@@ -166,6 +255,9 @@ struct SsspWorkspace {
   uint64_t* bm[2] = {nullptr, nullptr}; /* dirty vertices, near set of the round */
   unsigned long long* acc = nullptr;
   SplitSlot* slots = nullptr;
+  SsspCtl* ctl = nullptr;
+  RoundLog* log = nullptr; /* kRing entries, pinned coherent host memory */
+  hipEvent_t ev[kRing] = {};
   QueueSet qs;
   int64_t w0 = 0;
 };
@@ -176,6 +268,10 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->bm[1]);
   (void)hipFree(w->acc);
   (void)hipFree(w->slots);
+  (void)hipFree(w->ctl);
+  if (w->log) (void)hipHostFree(w->log);
+  for (hipEvent_t e : w->ev)
+    if (e) (void)hipEventDestroy(e);
   w->qs.release();
   delete w;
 }
@@ -194,9 +290,17 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
         (st = sg::alloc_device(dev, (void**)&ws->bm[1], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->acc, 16)) ||
         (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->ctl, sizeof(SsspCtl))) ||
         (st = ws->qs.init(dev, n, g->m, default_tuning()))) {
       sssp_ws_free(ws);
       return st;
+    }
+    bool ok = hipHostMalloc((void**)&ws->log, sizeof(RoundLog) * kRing, hipHostMallocCoherent) == hipSuccess;
+    for (int r = 0; r < kRing && ok; ++r) ok = hipEventCreate(&ws->ev[r]) == hipSuccess;
+    if (!ok) {
+      sssp_ws_free(ws);
+      sg::set_error("sssp workspace: event or pinned log allocation failed");
+      return SG_ERR_HIP;
     }
     g->sssp_ws = ws;
   }
@@ -233,59 +337,62 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
   SG_HIP(hipGetLastError());
   const bool bellman = !(delta > 0.0) || !std::isfinite(delta);
   const double w_max = DistOps<W>::is_float ? 3.0e38 : 4294967295.0;
-  double T = bellman ? w_max : std::min(delta, w_max);
-  QueueHdr* hh = (QueueHdr*)dev->host_scratch;
-  SplitSlot* hslots = (SplitSlot*)((char*)dev->host_scratch + 4096);
+  {
+    SsspCtl c{};
+    c.T_bits = DistOps<W>::bits((W)(bellman ? w_max : std::min(delta, w_max)));
+    SG_HIP(hipMemcpyAsync(ws->ctl, &c, sizeof(c), hipMemcpyHostToDevice, s));
+  }
   const int grid_w = sg::grid_for(dev, ws->w0, kBlock, 4);
-  int32_t round = 0;
-  int64_t relaxed = 0, dry_steps = 0;
-  for (;;) {
-    split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots);
-    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, dist, (W)T, ws->w0, ws->slots);
-    SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, Q, 0, off, nullptr, nullptr, near_bm, tune)));
-    SG_HIP(hipMemcpyAsync(hh, Q.hdr, sizeof(QueueHdr), hipMemcpyDeviceToHost, s));
-    SG_HIP(hipMemcpyAsync(hslots, ws->slots, sizeof(SplitSlot) * kSlots, hipMemcpyDeviceToHost, s));
-    SG_HIP(hipStreamSynchronize(s));
-    const int64_t fv = (int64_t)hh->nvert, fe = (int64_t)hh->deg;
-    if (fv == 0) {
-      unsigned long long far = 0;
-      uint32_t fmin = 0xFFFFFFFFu;
-      for (int k = 0; k < kSlots; ++k) {
-        far += hslots[k].far_count;
-        fmin = std::min(fmin, hslots[k].far_min);
-      }
-      if (far == 0) break;
-      /* the near set ran dry: the next bucket starts at the smallest deferred
-       * distance and must include it (at least one representable step above) */
-      const W dmin = DistOps<W>::value(fmin);
-      W t_next = (W)std::min((double)dmin + delta, w_max);
-      if (!(t_next > dmin)) {
-        if constexpr (DistOps<W>::is_float) t_next = std::nextafter(dmin, std::numeric_limits<float>::infinity());
-        else t_next = dmin + 1u;
-      }
-      T = (double)t_next;
-      if (++dry_steps > 2 * n + 64) {
-        sg::set_error("sg_sssp: no progress after %lld empty buckets", (long long)dry_steps);
-        return SG_ERR_STATE;
-      }
-      continue;
-    }
-    if (stats && round < SG_MAX_LEVELS_RECORDED) {
-      stats->round_vertices[round] = fv;
-      stats->round_edges[round] = fe;
-    }
-    relaxed += fe;
-    SsspArgs<OffT, W> a{};
-    a.cols = g->cols;
-    a.w = (const W*)g->weights;
-    a.dist = dist;
-    a.next_bm = dirty;
-    a.w0 = ws->w0;
-    for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[0][k];
-    a.qin_hdr = Q.hdr;
-    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a);
+  const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, 0, off, nullptr, nullptr, near_bm, tune);
+  SsspArgs<OffT, W> a{};
+  a.cols = g->cols;
+  a.w = (const W*)g->weights;
+  a.dist = dist;
+  a.next_bm = dirty;
+  a.w0 = ws->w0;
+  for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[0][k];
+  a.qin_hdr = Q.hdr;
+  /* one iteration: split, near-set compaction, decide (relax / next bucket /
+   * stop), relax; everything the host needs is in the iteration's record */
+  auto enqueue = [&](int32_t it) -> int {
+    split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl);
+    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, dist, ws->ctl, ws->w0, ws->slots);
+    near_count_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    sssp_finish_kernel<W><<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr, ws->ctl, ws->slots,
+                                                     bellman ? 0.0 : delta, w_max, it, ws->log);
+    near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl);
     SG_HIP(hipGetLastError());
-    ++round;
+    SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
+    return SG_OK;
+  };
+  SG_TRY(enqueue(0));
+  SG_TRY(enqueue(1));
+  int32_t round = 0;
+  int64_t relaxed = 0;
+  const int64_t max_iter = 4 * n + 256; /* every iteration relaxes or advances T past a distance */
+  for (int32_t it = 0;; ++it) {
+    SG_HIP(hipEventSynchronize(ws->ev[it % kRing]));
+    RoundLog e;
+    std::memcpy(&e, (const void*)&ws->log[it % kRing], sizeof(e));
+    if (e.iter != it) {
+      SG_HIP(hipStreamSynchronize(s));
+      sg::set_error("sg_sssp: iteration record %d out of order (found %d)", it, e.iter);
+      return SG_ERR_STATE;
+    }
+    if (e.relax && stats && e.rounds - 1 < SG_MAX_LEVELS_RECORDED) {
+      stats->round_vertices[e.rounds - 1] = e.near_v;
+      stats->round_edges[e.rounds - 1] = e.near_e;
+    }
+    round = e.rounds;
+    relaxed = (int64_t)e.relaxed;
+    if (e.done) break;
+    if (it + 2 > max_iter) {
+      SG_HIP(hipStreamSynchronize(s));
+      sg::set_error("sg_sssp: no termination after %lld iterations", (long long)it);
+      return SG_ERR_STATE;
+    }
+    SG_TRY(enqueue(it + 2));
   }
   if (stats) {
     SG_HIP(hipEventRecord(e1, s));
