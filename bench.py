@@ -149,6 +149,52 @@ def batched_vs_plain(sg, G, sources16):
     return {k: round(v, 3) for k, v in out.items()}
 
 
+def secondary(sg, dev):
+    """CC, TC and BC on the other configurations of the path (best of 3,
+    graphs generated on the device): CC on the 4096 x 4096 grid with 10 % of
+    the edges deleted and on RMAT scale 20 edge factor 32; TC on that RMAT;
+    BC from 64 sources on RMAT scale 20 edge factor 48, a = 0.55."""
+    out = {}
+
+    def best(fn, reps=3):
+        t = []
+        for _ in range(reps):
+            dev.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            dev.synchronize()
+            t.append(time.perf_counter() - t0)
+        return min(t)
+
+    G = sg.Graph.grid(dev, 4096, 4096, 0.1, seed=4)
+    cc = sg.CC(G)
+    cc.init()
+    t = best(cc.run)
+    out["cc_grid4096_ms"] = round(t * 1e3, 3)
+    out["cc_grid4096_gteps"] = round(G.m / t / 1e9, 3)
+    G.destroy()
+    G = sg.Graph.rmat(dev, 20, 32, seed=5)
+    cc = sg.CC(G)
+    cc.init()
+    t = best(cc.run)
+    out["cc_rmat20_ms"] = round(t * 1e3, 3)
+    tc = sg.TC(G)
+    t = best(tc.run)
+    out["tc_rmat20_ms"] = round(t * 1e3, 3)
+    out["tc_rmat20_triangles"] = tc.count
+    G.destroy()
+    G = sg.Graph.rmat(dev, 20, 48, 0.55, 0.17, 0.17, seed=6)
+    off, _, _ = G.download()
+    rng = np.random.default_rng(6)
+    srcs = [int(x) for x in rng.choice(np.nonzero(np.diff(off) > 0)[0], 64, replace=False)]
+    bc = sg.BC(G)
+    bc.init(srcs)
+    t = best(bc.run, reps=1)
+    out["bc_rmat20_64src_ms"] = round(t * 1e3, 3)
+    G.destroy()
+    return out
+
+
 def run_sssp(sg, G, sources, delta=0.0):
     s = sg.SSSP(G)
     s.init(sources[0])
@@ -255,7 +301,7 @@ def main():
         extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
         log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3, automatic delta)")
         sweep = {}
-        for d in [-1.0, 4.0, 8.0, 16.0, 64.0]:
+        for d in [-1.0, 4.0, 8.0, 16.0, 32.0, 64.0]:
             e3, w3 = run_sssp(sg, G, sources[:4], d)
             sweep["bellman_ford" if d < 0 else f"delta_{int(d)}"] = round(e3 / w3 / 1e9, 3)
         extra["gteps_sssp_by_delta"] = sweep
@@ -280,6 +326,11 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(oracle, G, sources, a.cpu_budget)
         log(f"cpu: {cpu['value']:.3f} GTEPS on {cpu['cores']} threads")
+
+    if not a.no_extra:
+        G.destroy()  # frees the scale-24 graph before the secondary graphs
+        extra["secondary"] = secondary(sg, dev)
+        log(f"secondary: {extra['secondary']}")
 
     out = {
         "metric": METRIC, "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": a.steps,

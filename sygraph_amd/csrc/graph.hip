@@ -173,21 +173,49 @@ __device__ __forceinline__ uint32_t edge_weight(uint32_t u, uint32_t v, uint64_t
   return 1u + (uint32_t)(((r >> 32) * 255ull) >> 32);
 }
 
+/* f(u, e) for every arc e of every vertex u: a wave takes 64 consecutive
+ * vertices, a lane walks a short list, the whole wave a list of more than 64
+ * arcs (so hubs do not serialise a lane).  All lanes of the wave call. */
+template<typename OffT, typename F>
+__device__ __forceinline__ void for_each_arc_wave(const OffT* off, int64_t n, F&& f) {
+  const int lane = lane_id();
+  const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t base = gwave * 64; base < n; base += nwaves * 64) {
+    const int64_t u = base + lane;
+    int64_t b = 0, e = 0;
+    if (u < n) {
+      b = (int64_t)off[u];
+      e = (int64_t)off[u + 1];
+    }
+    const bool coop = e - b > 64;
+    if (!coop)
+      for (int64_t j = b; j < e; ++j) f(u, j);
+    uint64_t heavy = ballot(coop);
+    while (heavy) {
+      const int k = __ffsll((unsigned long long)heavy) - 1;
+      heavy &= heavy - 1;
+      const int64_t kb = (int64_t)__shfl((long long)b, k, 64), ke = (int64_t)__shfl((long long)e, k, 64);
+      const int64_t ku = base + k;
+      for (int64_t j = kb + lane; j < ke; j += 64) f(ku, j);
+    }
+  }
+}
+
 template<typename OffT>
-__global__ void weights_kernel(const OffT* offsets, const int32_t* cols, int64_t n, uint64_t seed, uint32_t* w) {
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x)
-    for (int64_t e = (int64_t)offsets[u]; e < (int64_t)offsets[u + 1]; ++e) w[e] = edge_weight((uint32_t)u, (uint32_t)cols[e], seed);
+__global__ void __launch_bounds__(256) weights_kernel(const OffT* offsets, const int32_t* cols, int64_t n,
+                                                      uint64_t seed, uint32_t* w) {
+  for_each_arc_wave(offsets, n, [&](int64_t u, int64_t e) { w[e] = edge_weight((uint32_t)u, (uint32_t)cols[e], seed); });
 }
 
 /* Transpose helpers: key = (dst << 32) | src for every arc. */
 template<typename OffT>
-__global__ void transpose_keys_kernel(const OffT* offsets, const int32_t* cols, int64_t n, uint64_t* keys,
-                                      uint64_t* idx) {
-  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x)
-    for (int64_t e = (int64_t)offsets[u]; e < (int64_t)offsets[u + 1]; ++e) {
-      keys[e] = ((uint64_t)(uint32_t)cols[e] << 32) | (uint64_t)u;
-      idx[e] = (uint64_t)e;
-    }
+__global__ void __launch_bounds__(256) transpose_keys_kernel(const OffT* offsets, const int32_t* cols, int64_t n,
+                                                             uint64_t* keys, uint64_t* idx) {
+  for_each_arc_wave(offsets, n, [&](int64_t u, int64_t e) {
+    keys[e] = ((uint64_t)(uint32_t)cols[e] << 32) | (uint64_t)u;
+    idx[e] = (uint64_t)e;
+  });
 }
 
 template<typename OffT>
