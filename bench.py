@@ -241,6 +241,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the push-only / pull-only side measurements")
+    ap.add_argument("--algo", default="bfs", choices=["bfs", "sssp", "secondary"],
+                    help="sssp: time only SSSP from the 16 sources; secondary: only the CC / TC / BC "
+                         "configurations (for profiling those paths)")
+    ap.add_argument("--delta", type=float, default=0.0, help="SSSP bucket width with --algo sssp (0 automatic)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the traversal core runs on a "
@@ -252,6 +256,9 @@ def main():
     dev = sg.Device(0)
     info = dev.info()
     log(f"device: {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+    if a.algo == "secondary":
+        print(json.dumps({"secondary": secondary(sg, dev)}), flush=True)
+        return
     t0 = time.perf_counter()
     G = sg.Graph.rmat(dev, a.scale, 16, 0.57, 0.19, 0.19, seed=2, weight_seed=3)
     log(f"graph: RMAT s{a.scale} n={G.n} m={G.m} generated on device in {time.perf_counter() - t0:.2f}s")
@@ -265,6 +272,16 @@ def main():
         cand = np.nonzero(np.diff(off) > 0)[0]
         sources = [int(x) for x in rng.choice(cand, 16, replace=False)]
     pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[a.policy]
+    if a.algo == "sssp":
+        for _ in range(a.warmup):
+            run_sssp(sg, G, sources, a.delta)
+        e, w = 0, 0.0
+        for _ in range(a.steps):
+            e1, w1 = run_sssp(sg, G, sources, a.delta)
+            e, w = e + e1, w + w1
+        print(json.dumps({"metric": "SSSP GTEPS on 1 MI355X (RMAT s24, 16 sources)", "value": round(e / w / 1e9, 3),
+                          "unit": "GTEPS", "steps": a.steps, "delta": a.delta}), flush=True)
+        return
 
     edges, wall, kms, per_kernel = run_policy(sg, G, sources, pol, 15.0, 24.0, a.steps, a.warmup, True)
     gteps = edges / wall / 1e9
@@ -301,7 +318,7 @@ def main():
         extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
         log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3, automatic delta)")
         sweep = {}
-        for d in [-1.0, 4.0, 8.0, 16.0, 32.0, 64.0]:
+        for d in [-1.0, 16.0, 32.0, 64.0, 128.0, 256.0]:
             e3, w3 = run_sssp(sg, G, sources[:4], d)
             sweep["bellman_ford" if d < 0 else f"delta_{int(d)}"] = round(e3 / w3 / 1e9, 3)
         extra["gteps_sssp_by_delta"] = sweep

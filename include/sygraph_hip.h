@@ -308,7 +308,8 @@ typedef struct sg_sssp_stats {
   int32_t rounds;            /* frontier rounds until no distance changed */
   int64_t reached_vertices;  /* dist != n+1 */
   int64_t reached_edges;     /* TEPS numerator (algorithms/sssp.hpp:66-77) */
-  int64_t relaxations;       /* arcs relaxed over all rounds */
+  int64_t relaxations;       /* arcs relaxed over all rounds (light arcs in light rounds, heavy arcs in a
+                                bucket's heavy round) */
   float kernel_ms;           /* device time of the whole run */
   int64_t round_vertices[SG_MAX_LEVELS_RECORDED]; /* |F| per round */
   int64_t round_edges[SG_MAX_LEVELS_RECORDED];    /* sum deg over F per round */
@@ -320,7 +321,7 @@ int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_
 
 typedef struct sg_sssp_options {
   int32_t load_balancer; /* sg_load_balancer (the bucketing bins in every case) */
-  double delta;          /* bucket width of the near/far split: 0 = automatic (1/4 of the mean arc
+  double delta;          /* bucket width of the near/far split: 0 = automatic (half the mean arc
                             weight), negative or infinite = none, i.e. every dirty vertex is relaxed
                             each round (the reference's frontier Bellman-Ford, sssp.hpp:152-210) */
 } sg_sssp_options;

@@ -47,7 +47,7 @@ struct BfsArgs {
   const int32_t* cols;
   const OffT* in_off; /* inverse CSR (pull); aliases off when undirected */
   const int32_t* in_cols;
-  const int32_t* hint; /* per vertex: in-neighbour of largest out-degree */
+  const int2* hint; /* per vertex: {in-neighbour of largest out-degree, out-degree} */
   int32_t directed;
   int64_t n;
   int64_t w0;
@@ -118,16 +118,19 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) 
       bool found = false, miss = false;
       OffT b = 0, e = 0, j = 0;
       int64_t kk = 0;
+      int32_t odeg = 0;
       if (active) {
-        b = a.in_off[v];
-        e = a.in_off[v + 1];
-        /* first probe: the in-neighbour of largest out-degree, read from a
-         * contiguous array instead of the adjacency list's cache line */
-        if (test_bit(a.cur_bm, (uint32_t)a.hint[v])) {
+        /* first probe: the in-neighbour of largest out-degree, read with the
+         * vertex's out-degree from one 8-byte entry; the offsets and the
+         * adjacency list are read only on a miss */
+        const int2 hd = a.hint[v];
+        odeg = hd.y;
+        if (test_bit(a.cur_bm, (uint32_t)hd.x)) {
           found = true;
-          j = e;
         } else {
           miss = true;
+          b = a.in_off[v];
+          e = a.in_off[v + 1];
           j = b;
           kk = (int64_t)(e - b);
         }
@@ -173,14 +176,12 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) 
         }
       }
       if (active) {
-        const int64_t deg = (int64_t)(e - b);
-        if (deg > 0) ++st_scanned;
+        ++st_scanned; /* active vertices have in-arcs: in-degree 0 is pre-marked visited */
         st_edges += 1 + kk; /* hint probe + list probes after a miss */
         st_miss += miss ? 1 : 0;
         if (found) {
           a.depth[v] = a.level + 1;
           atomicOr(&wnew[owner], 1ull << bit);
-          const int64_t odeg = a.directed ? (int64_t)(a.off[v + 1] - a.off[v]) : deg;
           ++st_disc;
           st_deg += odeg;
           st_max = odeg > (int64_t)st_max ? odeg : st_max;
@@ -199,14 +200,14 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) 
 }
 
 /* Device-resident control block of the level loop.  Every kernel of a level
- * reads it (kernel boundaries order the accesses); only decide_kernel and
+ * reads it (kernel boundaries order the accesses); only count_dyn_kernel and
  * finish_kernel write it. */
 struct BfsCtl {
   int32_t dir;      /* direction of the level about to run: 0 push, 1 pull */
   int32_t done;     /* a level discovered nothing: later levels are no-ops */
   int32_t mode;     /* count pass of this level: 0 none, 1 visited & ~prev, 2 frontier bitmap + prev sync */
   int32_t write;    /* queues for the next (push) level must be written */
-  int32_t next_act; /* decided after a pull level by decide_kernel: direction * 3 + variant */
+  int32_t next_act; /* decided after a pull level by count_dyn_kernel: direction * 3 + variant */
   int32_t lb;       /* work-mapping variant of the level about to run */
   int64_t f_vertices, f_edges; /* frontier entering the level */
   unsigned long long visited_deg, reached;
@@ -291,40 +292,46 @@ __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const Bf
   else pull_level(a, ctl->lb);
 }
 
-/* After the traversal: a pull level's statistics are reduced from its slots and
- * the next direction decided now, because the count pass depends on it (queues
- * are only built when the next level pushes); a push level is counted first. */
-__global__ void decide_kernel(BfsCtl* ctl, const StatSlot* slots, DecideParams p, int32_t level) {
+/* Count pass of the level.  After a push it counts visited & ~prev.  After a
+ * pull the next action must be known first (queues are only built when the
+ * next level pushes): wave 0 of every workgroup reduces the pull's 64 stat
+ * slots and decides -- the same inputs, so the same answer everywhere, with no
+ * extra launch -- and workgroup 0 records the decision for finish_kernel. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, BfsCtl* ctl, const StatSlot* slots,
+                                                           DecideParams p, int32_t level) {
+  __shared__ int32_t s_mode;
   if (ctl->done) return;
   if (ctl->dir == 0) {
-    if (threadIdx.x == 0) ctl->mode = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->mode = 1;
+    compact_count_body<OffT, kFromDiff>(a);
     return;
   }
-  const StatSlot& x = slots[threadIdx.x]; /* 64 threads, one slot each */
-  const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
-  const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
-  const unsigned long long misses = wave_sum_u64(x.misses), max_deg = wave_max_u64(x.max_deg);
-  const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
-  const int32_t act = decide_action(p, 1, disc, disc_deg, unexplored, level + 1, max_deg);
-  const int32_t next = act / 3;
-  if (threadIdx.x == 0) {
-    ctl->disc = disc;
-    ctl->disc_deg = disc_deg;
-    ctl->edges = edges;
-    ctl->scanned = scanned;
-    ctl->misses = misses;
-    ctl->max_deg = max_deg;
-    ctl->next_act = act;
-    ctl->mode = (disc > 0 && next == 0) ? 2 : 0;
+  if (threadIdx.x < 64) {
+    const StatSlot& x = slots[threadIdx.x];
+    const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
+    const unsigned long long max_deg = wave_max_u64(x.max_deg);
+    const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
+    const int32_t act = decide_action(p, 1, disc, disc_deg, unexplored, level + 1, max_deg);
+    const int32_t mode = (disc > 0 && act / 3 == 0) ? 2 : 0;
+    const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
+    const unsigned long long misses = wave_sum_u64(x.misses);
+    if (threadIdx.x == 0) {
+      s_mode = mode;
+      if (blockIdx.x == 0) {
+        ctl->disc = disc;
+        ctl->disc_deg = disc_deg;
+        ctl->edges = edges;
+        ctl->scanned = scanned;
+        ctl->misses = misses;
+        ctl->max_deg = max_deg;
+        ctl->next_act = act;
+        ctl->mode = mode;
+      }
+    }
   }
-}
-
-template<typename OffT>
-__global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, const BfsCtl* ctl) {
-  if (ctl->done) return;
-  const int32_t mode = ctl->mode;
-  if (mode == 1) compact_count_body<OffT, kFromDiff>(a);
-  else if (mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
+  __syncthreads();
+  if (s_mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
 }
 
 /* Scan of the count pass, the push-level decision (wave 0, so the learned
@@ -486,8 +493,8 @@ void bfs_workspace_release(sg_graph_t g) {
 }
 } // namespace sg
 
-/* Host side of the level loop.  Each level is a fixed sequence of five
- * launches (traverse, decide, count, scan+finish, write) whose behaviour the
+/* Host side of the level loop.  Each level is a fixed sequence of four
+ * launches (traverse, decide+count, scan+finish, write) whose behaviour the
  * device-resident control block selects, so the host never has to decide
  * anything between levels: it keeps `lookahead` levels enqueued beyond the one
  * it waits for and only reads each level's record (pinned host memory) after
@@ -569,8 +576,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
-    decide_kernel<<<1, 64, 0, s>>>(ws->ctl, la.stats, P, L);
-    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P, L);
     finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, L, ws->log);
     write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     SG_HIP(hipGetLastError());

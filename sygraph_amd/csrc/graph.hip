@@ -244,13 +244,15 @@ __global__ void isolated_kernel(const OffT* offsets, int64_t n, int64_t w0, uint
   }
 }
 
-/* hint[v] = in-neighbour of v with the largest out-degree (lowest id on
- * ties), -1 without in-arcs.  A wave takes 64 consecutive vertices; lists
- * longer than 64 arcs are reduced by the whole wave.  Candidates are ranked
- * by the key (degree << 32) | ~id, so the maximum key is the answer. */
+/* hint[v] = {in-neighbour of v with the largest out-degree (lowest id on
+ * ties, -1 without in-arcs), out-degree of v}: one 8-byte load gives a pull
+ * its first probe and, on a hit, the degree it needs, without the offsets.
+ * A wave takes 64 consecutive vertices; lists longer than 64 arcs are reduced
+ * by the whole wave.  Candidates are ranked by the key
+ * (degree << 32) | ~id, so the maximum key is the answer. */
 template<typename OffT>
 __global__ void __launch_bounds__(256) pull_hint_kernel(const OffT* in_off, const int32_t* in_cols, const OffT* out_off,
-                                                        int64_t n, int32_t* hint) {
+                                                        int64_t n, int2* hint) {
   const int lane = lane_id();
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -285,7 +287,9 @@ __global__ void __launch_bounds__(256) pull_hint_kernel(const OffT* in_off, cons
       part = wave_max_u64(part);
       if (lane == k2) best = part;
     }
-    if (v < n) hint[v] = e > b ? (int32_t)(0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFull)) : -1;
+    if (v < n)
+      hint[v] = make_int2(e > b ? (int32_t)(0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFull)) : -1,
+                          (int32_t)(out_off[v + 1] - out_off[v]));
   }
 }
 
@@ -481,7 +485,7 @@ int graph_ensure_isolated(sg_graph_t g) {
 }
 int graph_ensure_pull_hint(sg_graph_t g) {
   if (g->pull_hint) return SG_OK;
-  SG_TRY(alloc_device(g->dev, (void**)&g->pull_hint, 4 * (g->n + 1)));
+  SG_TRY(alloc_device(g->dev, (void**)&g->pull_hint, sizeof(int2) * (g->n + 1)));
   const int grid = grid_for(g->dev, (g->n + 3) / 4, 256);
   const void* in_off = g->inv_offsets ? g->inv_offsets : g->offsets;
   const int32_t* in_cols = g->inv_cols ? g->inv_cols : g->cols;

@@ -106,7 +106,8 @@ struct sg_graph_s {
   bool owns_inverse = false;
   /* lazily built helpers */
   uint64_t* isolated = nullptr; /* bit set for vertices with in-degree 0 */
-  int32_t* pull_hint = nullptr;  /* per vertex: in-neighbour of largest out-degree (pull probes it first) */
+  int2* pull_hint = nullptr;     /* per vertex: {in-neighbour of largest out-degree (pull probes it first),
+                                    out-degree of the vertex} */
   double mean_weight = -1.0;     /* mean arc weight, computed on first use (automatic SSSP delta) */
   int64_t total_degree = 0;
   void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */

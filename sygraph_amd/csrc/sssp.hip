@@ -63,10 +63,14 @@ struct SsspArgs {
 /* Device-resident control block of the round loop (written only by
  * sssp_finish_kernel; kernel boundaries order the accesses). */
 struct SsspCtl {
-  uint32_t T_bits;  /* threshold: near = distance below T */
-  int32_t done;     /* no dirty vertex left: later iterations are no-ops */
-  int32_t relax;    /* the current iteration relaxes (its near set is not empty) */
-  int32_t rounds;   /* iterations that relaxed so far */
+  uint32_t T_bits;    /* threshold: near = distance below T */
+  int32_t done;       /* no dirty vertex left: later iterations are no-ops */
+  int32_t relax;      /* the current iteration relaxes (its near set is not empty) */
+  int32_t rounds;     /* iterations that relaxed so far */
+  int32_t heavy;      /* the current iteration relaxes the heavy arcs of the bucket's settled set */
+  int32_t heavy_next; /* the next iteration is the heavy one */
+  int32_t r_pending;  /* vertices were relaxed in this bucket since its heavy iteration */
+  int32_t pad;
   unsigned long long relaxed; /* arcs relaxed so far */
 };
 
@@ -78,12 +82,23 @@ struct alignas(64) RoundLog {
 
 constexpr int kRing = 8;
 
+/* Light iterations relax the arcs lighter than delta, the heavy iteration of
+ * a bucket the others (delta-stepping's light / heavy split: a heavy arc
+ * from a settled bucket cannot land inside it, so it is relaxed once).  The
+ * weight is read first; the random distance access only for the arcs of the
+ * phase. */
 template<typename OffT, typename W>
-__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl) {
+__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl, W dlt,
+                                                       unsigned long long* relax_slots) {
   if (ctl->done || !ctl->relax) return;
+  const bool heavy = ctl->heavy != 0;
+  unsigned long long cnt = 0;
   for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
+    const W wt = a.w[e];
+    if ((wt >= dlt) != heavy) return;
+    ++cnt;
     const W du = DistOps<W>::value(atomic_load(&a.dist[u]));
-    const W nd = du + a.w[e];
+    const W nd = du + wt;
     const uint32_t nb = DistOps<W>::bits(nd);
     if (nb < a.dist[v]) {
       const uint32_t old = atomic_min(&a.dist[v], nb);
@@ -93,6 +108,16 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, cons
       }
     }
   });
+  /* arcs relaxed in this phase: one atomic per workgroup into 64 slots */
+  __shared__ unsigned long long red[kBlock / 64];
+  cnt = wave_sum_u64(cnt);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+    if (t) atomic_add(&relax_slots[(blockIdx.x % kSlots) * 16], t);
+  }
 }
 
 /* per-block counters of the split, one 128-byte line per slot */
@@ -107,9 +132,18 @@ struct alignas(128) SplitSlot {
  * the round's frontier bitmap); count and smallest distance of the rest */
 
 template<typename W>
-__global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, const uint32_t* dist,
-                                                       const SsspCtl* ctl, int64_t w0, SplitSlot* slots) {
+__global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, uint64_t* settled,
+                                                       const uint32_t* dist, const SsspCtl* ctl, int64_t w0,
+                                                       SplitSlot* slots) {
   if (ctl->done) return;
+  if (ctl->heavy_next) { /* the heavy iteration: its frontier is the bucket's settled set */
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
+      const uint64_t r = settled[i];
+      near_bm[i] = r;
+      if (r) settled[i] = 0;
+    }
+    return;
+  }
   const W T = DistOps<W>::value(ctl->T_bits);
   unsigned long long far = 0;
   unsigned int fmin = 0xFFFFFFFFu;
@@ -123,7 +157,10 @@ __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t
       else fmin = db < fmin ? db : fmin;
     }
     near_bm[i] = nb;
-    if (nb) dirty[i] = f & ~nb;
+    if (nb) {
+      dirty[i] = f & ~nb;
+      settled[i] |= nb;
+    }
     far += (unsigned long long)__popcll(f & ~nb);
   }
   far = wave_sum_u64(far);
@@ -188,20 +225,37 @@ __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* bl
   e.iter = iter;
   e.near_v = (int64_t)tot.nvert;
   e.near_e = (int64_t)tot.deg;
-  if (tot.nvert > 0) {
+  if (ctl->heavy_next) { /* this iteration relaxes the heavy arcs of the settled set */
+    ctl->heavy_next = 0;
+    ctl->r_pending = 0;
+    ctl->heavy = 1;
+    ctl->relax = tot.nvert > 0 ? 1 : 0;
+    if (tot.nvert > 0) {
+      ctl->rounds += 1;
+      ctl->relaxed += tot.deg;
+    }
+  } else if (tot.nvert > 0) {
+    ctl->heavy = 0;
     ctl->relax = 1;
+    ctl->r_pending = 1;
     ctl->rounds += 1;
     ctl->relaxed += tot.deg;
   } else {
+    ctl->heavy = 0;
     ctl->relax = 0;
-    unsigned long long far = 0;
-    uint32_t fmin = 0xFFFFFFFFu;
-    for (int k = 0; k < kSlots; ++k) {
-      far += slots[k].far_count;
-      fmin = slots[k].far_min < fmin ? slots[k].far_min : fmin;
+    if (ctl->r_pending && delta > 0.0) {
+      ctl->heavy_next = 1; /* the bucket's light phase is over */
+    } else {
+      unsigned long long far = 0;
+      uint32_t fmin = 0xFFFFFFFFu;
+      for (int k = 0; k < kSlots; ++k) {
+        far += slots[k].far_count;
+        fmin = slots[k].far_min < fmin ? slots[k].far_min : fmin;
+      }
+      if (far == 0) ctl->done = 1;
+      else ctl->T_bits = next_threshold<W>(fmin, delta, w_max);
+      ctl->r_pending = 0;
     }
-    if (far == 0) ctl->done = 1;
-    else ctl->T_bits = next_threshold<W>(fmin, delta, w_max);
   }
   e.done = ctl->done;
   e.relax = ctl->relax;
@@ -243,19 +297,32 @@ __global__ void reached_kernel(const OffT* off, const uint32_t* dist, uint32_t u
       e += (uint64_t)(off[v + 1] - off[v]);
       ++c;
     }
+  /* one atomic pair per workgroup (not per wave) */
+  __shared__ unsigned long long red[2][4];
   e = wave_sum_u64(e);
   c = wave_sum_u64(c);
   if (lane_id() == 0) {
-    atomicAdd(&out[0], (unsigned long long)e);
-    atomicAdd(&out[1], (unsigned long long)c);
+    red[0][threadIdx.x >> 6] = e;
+    red[1][threadIdx.x >> 6] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long te = 0, tc = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      te += red[0][w];
+      tc += red[1][w];
+    }
+    atomicAdd(&out[0], te);
+    atomicAdd(&out[1], tc);
   }
 }
 
 struct SsspWorkspace {
-  uint64_t* bm[2] = {nullptr, nullptr}; /* dirty vertices, near set of the round */
+  uint64_t* bm[3] = {nullptr, nullptr, nullptr}; /* dirty vertices, near set of the round, bucket's settled set */
   unsigned long long* acc = nullptr;
   SplitSlot* slots = nullptr;
   SsspCtl* ctl = nullptr;
+  unsigned long long* relax_slots = nullptr; /* kSlots x 16 (one cache line each) */
   RoundLog* log = nullptr; /* kRing entries, pinned coherent host memory */
   hipEvent_t ev[kRing] = {};
   QueueSet qs;
@@ -266,9 +333,11 @@ void sssp_ws_free(SsspWorkspace* w) {
   if (!w) return;
   (void)hipFree(w->bm[0]);
   (void)hipFree(w->bm[1]);
+  (void)hipFree(w->bm[2]);
   (void)hipFree(w->acc);
   (void)hipFree(w->slots);
   (void)hipFree(w->ctl);
+  (void)hipFree(w->relax_slots);
   if (w->log) (void)hipHostFree(w->log);
   for (hipEvent_t e : w->ev)
     if (e) (void)hipEventDestroy(e);
@@ -276,7 +345,7 @@ void sssp_ws_free(SsspWorkspace* w) {
   delete w;
 }
 
-constexpr double kAutoDeltaFactor = 0.25; /* automatic delta = 1/4 of the mean arc weight */
+constexpr double kAutoDeltaFactor = 0.5; /* automatic delta = half the mean arc weight */
 
 int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
   auto* ws = static_cast<SsspWorkspace*>(g->sssp_ws);
@@ -288,9 +357,11 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
     int st = SG_OK;
     if ((st = sg::alloc_device(dev, (void**)&ws->bm[0], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->bm[1], 8 * (ws->w0 + 1))) ||
+        (st = sg::alloc_device(dev, (void**)&ws->bm[2], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->acc, 16)) ||
         (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
         (st = sg::alloc_device(dev, (void**)&ws->ctl, sizeof(SsspCtl))) ||
+        (st = sg::alloc_device(dev, (void**)&ws->relax_slots, 8 * 16 * kSlots)) ||
         (st = ws->qs.init(dev, n, g->m, default_tuning()))) {
       sssp_ws_free(ws);
       return st;
@@ -330,7 +401,10 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
   }
   uint64_t* dirty = ws->bm[0];
   uint64_t* near_bm = ws->bm[1];
+  uint64_t* settled = ws->bm[2];
   SG_TRY(sg::zero_async(dev, dirty, 8 * ws->w0));
+  SG_TRY(sg::zero_async(dev, settled, 8 * ws->w0));
+  SG_TRY(sg::zero_async(dev, ws->relax_slots, 8 * 16 * kSlots));
   sssp_init_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(off, dist, unreached, n, src, Q.q[0][0],
                                                                        Q.q[0][1], Q.q[0][2], Q.hdr, tune);
   set_bit_kernel<<<1, 1, 0, s>>>(dirty, src);
@@ -343,6 +417,8 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
     SG_HIP(hipMemcpyAsync(ws->ctl, &c, sizeof(c), hipMemcpyHostToDevice, s));
   }
   const int grid_w = sg::grid_for(dev, ws->w0, kBlock, 4);
+  /* arcs of weight >= dlt are heavy (none under the reference's schedule) */
+  const W dlt = bellman ? (W)w_max : (W)std::min(std::max(delta, DistOps<W>::is_float ? 0.0 : 1.0), w_max);
   const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, 0, off, nullptr, nullptr, near_bm, tune);
   SsspArgs<OffT, W> a{};
   a.cols = g->cols;
@@ -356,12 +432,12 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
    * stop), relax; everything the host needs is in the iteration's record */
   auto enqueue = [&](int32_t it) -> int {
     split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl);
-    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, dist, ws->ctl, ws->w0, ws->slots);
+    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, settled, dist, ws->ctl, ws->w0, ws->slots);
     near_count_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     sssp_finish_kernel<W><<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr, ws->ctl, ws->slots,
                                                      bellman ? 0.0 : delta, w_max, it, ws->log);
     near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
-    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl);
+    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, dlt, ws->relax_slots);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
     return SG_OK;
@@ -403,7 +479,15 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
     (void)hipEventDestroy(e1);
     stats->kernel_ms = ms;
     stats->rounds = round;
-    stats->relaxations = relaxed;
+    {
+      unsigned long long rs[16 * kSlots];
+      SG_HIP(hipMemcpyAsync(rs, ws->relax_slots, sizeof(rs), hipMemcpyDeviceToHost, s));
+      SG_HIP(hipStreamSynchronize(s));
+      unsigned long long t = 0;
+      for (int k = 0; k < kSlots; ++k) t += rs[k * 16];
+      stats->relaxations = (int64_t)t; /* arcs relaxed in their phase (examined: `relaxed`) */
+      (void)relaxed;
+    }
     SG_TRY(sg::zero_async(dev, ws->acc, 16));
     reached_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(off, dist, unreached, n, ws->acc);
     unsigned long long h[2];
