@@ -719,9 +719,11 @@ class Policy:
                                        C.c_void_p(self._grad.ptr), C.byref(loss)))
         return float(loss.value), self._grad.download(np.float32)
 
-    def train_step(self, xb: DeviceBuffer, yb: DeviceBuffer, batch: int, lr: float) -> float:
+    def train_step(self, xb: DeviceBuffer, yb: DeviceBuffer, batch: int, lr: float, first: int = 0) -> float:
+        """One Adam step on samples [first, first + batch) of the device arrays."""
         loss = C.c_float()
-        check(lib().sg_policy_train_step(self.h, C.c_void_p(xb.ptr), C.c_void_p(yb.ptr), batch, lr, C.byref(loss)))
+        check(lib().sg_policy_train_step(self.h, C.c_void_p(xb.ptr + 4 * POLICY_FEATURES * first),
+                                         C.c_void_p(yb.ptr + 4 * first), batch, lr, C.byref(loss)))
         return float(loss.value)
 
     def destroy(self):

@@ -7,11 +7,11 @@
  * remains the default and the fallback.
  *
  * Training runs on the device: a forward kernel (one thread per sample) keeps
- * the activations and the per-sample backward signals, a gradient kernel (one
- * thread per parameter, summing over the batch in a fixed order, so results
- * are deterministic) and an Adam kernel.  Inference inside the BFS level
- * decisions (bfs.hip) is sg::policy::infer_wave() on one wavefront
- * (policy.hpp).
+ * the activations and the per-sample backward signals; a gradient kernel sums
+ * each parameter over chunks of 2,048 samples in sample order and a second
+ * pass adds the chunks in order (no atomics, so gradients are deterministic);
+ * then an Adam kernel.  Inference inside the BFS level decisions (bfs.hip) is
+ * sg::policy::infer_wave() on one wavefront (policy.hpp).
  *
  * Parameter layout (floats): W1[32][8] b1[32] W2[32][32] b2[32] W3[6][32] b3[6].
  */
@@ -95,13 +95,18 @@ __global__ void __launch_bounds__(kB) mlp_forward_kernel(const float* __restrict
   }
 }
 
-/* One thread per parameter: dL/dp summed over the batch in sample order. */
+/* dL/dp as fixed-order partial sums: thread (p, c) sums parameter p over
+ * samples [c * kChunk, (c + 1) * kChunk) in sample order; mlp_reduce_kernel
+ * adds the chunks in order, so the gradient is deterministic. */
+constexpr int kChunk = 2048;
+
 __global__ void __launch_bounds__(kB) mlp_grad_kernel(const float* __restrict__ X, int32_t batch,
                                                       const float* __restrict__ H1, const float* __restrict__ H2,
                                                       const float* __restrict__ D1, const float* __restrict__ D2,
-                                                      const float* __restrict__ DS, float* __restrict__ G) {
+                                                      const float* __restrict__ DS, float* __restrict__ part) {
   using namespace sg::policy;
   const int p = blockIdx.x * kB + threadIdx.x;
+  const int c = blockIdx.y;
   if (p >= kParams) return;
   const float* a = nullptr; /* upstream signal (per sample, row of width wa) */
   const float* in = nullptr;  /* input activation (per sample, row of width wi), nullptr for a bias */
@@ -119,11 +124,20 @@ __global__ void __launch_bounds__(kB) mlp_grad_kernel(const float* __restrict__ 
   } else {
     ia = p - kOffB3, a = DS, wa = kOut;
   }
+  const int b0 = c * kChunk, b1 = b0 + kChunk < batch ? b0 + kChunk : batch;
   float acc = 0.f;
   if (in)
-    for (int b = 0; b < batch; ++b) acc += a[(int64_t)b * wa + ia] * in[(int64_t)b * wi + ii];
+    for (int b = b0; b < b1; ++b) acc += a[(int64_t)b * wa + ia] * in[(int64_t)b * wi + ii];
   else
-    for (int b = 0; b < batch; ++b) acc += a[(int64_t)b * wa + ia];
+    for (int b = b0; b < b1; ++b) acc += a[(int64_t)b * wa + ia];
+  part[(int64_t)c * kParams + p] = acc;
+}
+
+__global__ void mlp_reduce_kernel(const float* __restrict__ part, int32_t chunks, float* __restrict__ G) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= sg::policy::kParams) return;
+  float acc = 0.f;
+  for (int c = 0; c < chunks; ++c) acc += part[(int64_t)c * sg::policy::kParams + p];
   G[p] = acc;
 }
 
@@ -164,10 +178,10 @@ struct sg_policy_s {
   /* training scratch, grown on demand */
   int32_t cap = 0;
   float *h1 = nullptr, *h2 = nullptr, *s = nullptr, *d1 = nullptr, *d2 = nullptr, *ds = nullptr, *loss = nullptr,
-        *loss_mean = nullptr;
+        *loss_mean = nullptr, *part = nullptr;
   void free_scratch() {
-    for (float* p : {h1, h2, s, d1, d2, ds, loss, loss_mean}) (void)hipFree(p);
-    h1 = h2 = s = d1 = d2 = ds = loss = loss_mean = nullptr;
+    for (float* p : {h1, h2, s, d1, d2, ds, loss, loss_mean, part}) (void)hipFree(p);
+    h1 = h2 = s = d1 = d2 = ds = loss = loss_mean = part = nullptr;
     cap = 0;
   }
 };
@@ -187,6 +201,7 @@ int ensure_scratch(sg_policy_t p, int32_t batch) {
   SG_TRY(sg::alloc_device(p->dev, (void**)&p->ds, 4 * B * kOut));
   SG_TRY(sg::alloc_device(p->dev, (void**)&p->loss, 4 * B));
   SG_TRY(sg::alloc_device(p->dev, (void**)&p->loss_mean, 16));
+  SG_TRY(sg::alloc_device(p->dev, (void**)&p->part, 4 * (size_t)kParams * ((B + kChunk - 1) / kChunk)));
   p->cap = batch;
   return SG_OK;
 }
@@ -202,8 +217,10 @@ int run_forward(sg_policy_t p, const float* X, const int32_t* Y, int32_t batch) 
 
 int run_grad(sg_policy_t p, const float* X, int32_t batch) {
   hipStream_t s = p->dev->stream;
-  mlp_grad_kernel<<<(sg::policy::kParams + kB - 1) / kB, kB, 0, s>>>(X, batch, p->h1, p->h2, p->d1, p->d2, p->ds,
-                                                                     p->grad);
+  const int chunks = (batch + kChunk - 1) / kChunk;
+  mlp_grad_kernel<<<dim3((sg::policy::kParams + kB - 1) / kB, chunks), kB, 0, s>>>(X, batch, p->h1, p->h2, p->d1,
+                                                                                   p->d2, p->ds, p->part);
+  mlp_reduce_kernel<<<(sg::policy::kParams + kB - 1) / kB, kB, 0, s>>>(p->part, chunks, p->grad);
   SG_HIP(hipGetLastError());
   return SG_OK;
 }

@@ -14,7 +14,7 @@
    cost(L, a) = traversal + conversion time of level L in the run forced to a.
    Each level L >= 1 becomes one sample: the features the device computes
    when it decides level L (policy.hpp), labelled with argmin_a cost(L, a).
-3. Training: full-batch Adam on the device (sg_policy_train_step) on
+3. Training: minibatch Adam on the device (sg_policy_train_step) on
    samples resampled in proportion to the level's stake (slowest minus fastest
    action time), so the levels that dominate traversal time dominate the loss.
 4. Held-out evaluation: the summed device time of the BFS under the plain
@@ -123,21 +123,25 @@ def collect(dev, seeds, quick):
     return np.concatenate(X), np.concatenate(Y), np.concatenate(C)
 
 
-def train(dev, X, Y, C, epochs: int, lr: float, seed: int = 0):
+def train(dev, X, Y, C, steps: int, lr: float, batch: int = 8192, seed: int = 0):
     """Cost-sensitive training: samples are drawn in proportion to how much
     the choice matters at that level (slowest minus fastest action), so the
-    loss is dominated by the levels that dominate the traversal time."""
+    loss is dominated by the levels that dominate the traversal time.  Adam on
+    minibatches of a shuffled pool that lives on the device."""
     rng = np.random.default_rng(seed)
     stake = C.max(axis=1) - C.min(axis=1)
     prob = (stake + 1e-4) / (stake + 1e-4).sum()
-    idx = rng.choice(len(Y), size=4 * len(Y), replace=True, p=prob)
-    Xb, Yb = dev.to_device(X[idx]), dev.to_device(Y[idx])
+    pool = rng.choice(len(Y), size=max(2 * len(Y), batch), replace=True, p=prob)
+    Xb, Yb = dev.to_device(X[pool]), dev.to_device(Y[pool])
     scale = np.concatenate([np.full(256, np.sqrt(2 / 8)), np.zeros(32), np.full(1024, np.sqrt(2 / 32)), np.zeros(32),
                             np.full(192, np.sqrt(1 / 32)), np.zeros(6)])
     pol = sg.Policy(dev, (rng.normal(0, 1, sg.POLICY_PARAMS) * scale).astype(np.float32))
     losses = []
-    for ep in range(epochs):
-        losses.append(pol.train_step(Xb, Yb, len(idx), lr))
+    nb = len(pool) // batch
+    for step in range(steps):
+        losses.append(pol.train_step(Xb, Yb, batch, lr, first=(step % nb) * batch))
+        if step % 500 == 0 or step == steps - 1:
+            log(f"  step {step}: loss {losses[-1]:.4f}")
     return pol, losses
 
 
@@ -159,6 +163,9 @@ def evaluate(dev, pol, seeds, quick):
             cost, _ = level_costs(G, src)
             t["per_level_best"] = float(cost.min(axis=1).sum())
             rows.append({"seed": seed, "graph": name, "source": src, **t})
+        last = rows[-1]
+        log(f"  held-out seed {seed}: {name}: plain {last['plain']:.2f} heuristic {last['heuristic']:.2f} "
+            f"learned {last['learned']:.2f} best {last['per_level_best']:.2f} ms")
         G.destroy()
     return rows
 
@@ -167,7 +174,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "data"))
     ap.add_argument("--quick", action="store_true", help="smaller graphs and sets (smoke run)")
-    ap.add_argument("--epochs", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=3e-3)
     a = ap.parse_args()
     dev = sg.Device(0)
@@ -177,7 +184,7 @@ def main():
     log("collecting training levels ...")
     X, Y, C = collect(dev, train_seeds, a.quick)
     log(f"{len(Y)} samples, class counts {np.bincount(Y, minlength=6).tolist()} ({time.time() - t0:.1f}s)")
-    pol, losses = train(dev, X, Y, C, a.epochs, a.lr)
+    pol, losses = train(dev, X, Y, C, a.steps, a.lr)
     pred = np.argmax(pol.forward(X), axis=1)
     acc = float((pred == Y).mean())
     regret = float((C[np.arange(len(Y)), pred] - C.min(axis=1)).sum() / C.min(axis=1).sum())
@@ -195,7 +202,7 @@ def main():
     pol.get_params().astype(np.float32).tofile(os.path.join(a.out, "bfs_policy.f32"))
     meta = {"train_seeds": [train_seeds[0], train_seeds[-1]], "held_out_seeds": [test_seeds[0], test_seeds[-1]],
             "quick": a.quick, "samples": int(len(Y)), "class_counts": np.bincount(Y, minlength=6).tolist(),
-            "epochs": a.epochs, "lr": a.lr, "loss_first": losses[0], "loss_last": losses[-1],
+            "steps": a.steps, "lr": a.lr, "loss_first": losses[0], "loss_last": losses[-1],
             "train_accuracy": acc, "train_time_regret": regret, "held_out": summary, "held_out_runs": rows}
     with open(os.path.join(a.out, "bfs_policy.json"), "w") as f:
         json.dump(meta, f, indent=1)
