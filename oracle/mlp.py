@@ -86,7 +86,7 @@ def features(frontier_vertices, frontier_edges, unexplored, level, max_degree, n
     fe = np.asarray(frontier_edges, dtype=np.float64)
     f = np.stack([
         np.log2(1 + fv) / 24, np.log2(1 + fe) / 32, np.log2(1 + np.asarray(unexplored, dtype=np.float64)) / 32,
-        np.asarray(level, dtype=np.float64) / 32, np.log2(1 + fe / np.maximum(fv, 1)) / 16,
+        np.log2(1 + np.asarray(level, dtype=np.float64)) / 12, np.log2(1 + fe / np.maximum(fv, 1)) / 16,
         np.log2(1 + np.asarray(max_degree, dtype=np.float64)) / 24,
         np.full_like(fv, np.log2(1 + m / max(n, 1)) / 8), np.full_like(fv, np.log2(max(n, 1)) / 32)], axis=-1)
     return f.astype(np.float32)

@@ -247,17 +247,22 @@ __device__ __forceinline__ int32_t next_direction(const DecideParams& p, int32_t
 }
 
 /* Action (direction * 3 + variant) of the next level; every lane of one
- * wavefront calls with the same arguments. */
-__device__ __forceinline__ int32_t decide_action(const DecideParams& p, int32_t dir, unsigned long long disc,
-                                                 unsigned long long disc_deg, unsigned long long unexplored,
-                                                 int32_t next_level, unsigned long long max_deg) {
-  if (p.policy == SG_POLICY_LEARNED) {
-    float f[sg::policy::kIn];
-    sg::policy::features(f, (double)disc, (double)disc_deg, (double)unexplored, next_level, (double)max_deg, p.n_d,
-                         p.m_d);
-    return sg::policy::infer_wave(p.mlp, f);
-  }
+ * wavefront calls with the same arguments.  Under the learned policy `pol`
+ * holds the parameters, staged by the workgroup (stage_policy). */
+__device__ __forceinline__ int32_t decide_action(const DecideParams& p, const sg::policy::Staged* pol, int32_t dir,
+                                                 unsigned long long disc, unsigned long long disc_deg,
+                                                 unsigned long long unexplored, int32_t next_level,
+                                                 unsigned long long max_deg) {
+  if (p.policy == SG_POLICY_LEARNED)
+    return sg::policy::infer_staged(pol, (double)disc, (double)disc_deg, (double)unexplored, next_level,
+                                    (double)max_deg, p.n_d, p.m_d);
   return next_direction(p, dir, disc, disc_deg, unexplored) * 3 + p.lb_fixed;
+}
+
+/* All threads of the workgroup copy the learned parameters into LDS; the
+ * caller's next workgroup barrier publishes them. */
+__device__ __forceinline__ void stage_policy(const DecideParams& p, sg::policy::Staged* pol) {
+  if (p.policy == SG_POLICY_LEARNED) sg::policy::stage(pol, p.mlp, (int)threadIdx.x, (int)blockDim.x);
 }
 
 /* Level-0 setup: depth[src] = 0, visited = prev = isolated | {src},
@@ -301,18 +306,21 @@ template<typename OffT>
 __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, BfsCtl* ctl, const StatSlot* slots,
                                                            DecideParams p, int32_t level) {
   __shared__ int32_t s_mode;
+  __shared__ sg::policy::Staged s_pol;
   if (ctl->done) return;
   if (ctl->dir == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->mode = 1;
     compact_count_body<OffT, kFromDiff>(a);
     return;
   }
+  stage_policy(p, &s_pol);
+  __syncthreads();
   if (threadIdx.x < 64) {
     const StatSlot& x = slots[threadIdx.x];
     const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
     const unsigned long long max_deg = wave_max_u64(x.max_deg);
     const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
-    const int32_t act = decide_action(p, 1, disc, disc_deg, unexplored, level + 1, max_deg);
+    const int32_t act = decide_action(p, &s_pol, 1, disc, disc_deg, unexplored, level + 1, max_deg);
     const int32_t mode = (disc > 0 && act / 3 == 0) ? 2 : 0;
     const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
     const unsigned long long misses = wave_sum_u64(x.misses);
@@ -341,8 +349,10 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
                                                               BfsCtl* ctl, DecideParams p, int32_t level,
                                                               LevelLog* log) {
   __shared__ QueueHdr tot;
+  __shared__ sg::policy::Staged s_pol;
   if (ctl->done) return;
   const int32_t mode = ctl->mode;
+  if (ctl->dir == 0) stage_policy(p, &s_pol);
   if (mode != 0) compact_scan_body(tiles, ntiles, hdr_out, &tot);
   __syncthreads();
   if (threadIdx.x >= 64) return;
@@ -357,7 +367,8 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     edges = (unsigned long long)ctl->f_edges;
     scanned = (unsigned long long)ctl->f_vertices;
     misses = 0;
-    act = decide_action(p, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1, max_deg);
+    act = decide_action(p, &s_pol, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1,
+                      max_deg);
   } else {
     disc = ctl->disc;
     disc_deg = ctl->disc_deg;

@@ -11,7 +11,7 @@
  * each parameter over chunks of 2,048 samples in sample order and a second
  * pass adds the chunks in order (no atomics, so gradients are deterministic);
  * then an Adam kernel.  Inference inside the BFS level decisions (bfs.hip) is
- * sg::policy::infer_wave() on one wavefront (policy.hpp).
+ * sg::policy::infer_staged() on one wavefront (policy.hpp).
  *
  * Parameter layout (floats): W1[32][8] b1[32] W2[32][32] b2[32] W3[6][32] b3[6].
  */

@@ -101,7 +101,7 @@ def level_features(stats0, n: int, m: int):
     for L in range(1, len(stats0)):
         fv, fe, _, _ = stats0[L]
         _, _, unexplored, maxdeg = stats0[L - 1]
-        out.append([np.log2(1 + fv) / 24, np.log2(1 + fe) / 32, np.log2(1 + unexplored) / 32, L / 32,
+        out.append([np.log2(1 + fv) / 24, np.log2(1 + fe) / 32, np.log2(1 + unexplored) / 32, np.log2(1 + L) / 12,
                     np.log2(1 + fe / max(fv, 1)) / 16, np.log2(1 + maxdeg) / 24, np.log2(1 + m / max(n, 1)) / 8,
                     np.log2(max(n, 1)) / 32])
     return np.array(out, dtype=np.float32).reshape(-1, 8)

@@ -50,5 +50,19 @@ def test_adam_first_step_is_signed_lr():
 
 
 def test_features_hand_values():
-    f = mlp.features(1023, 1023 * 3, 2 ** 20 - 1, 4, 255, 2 ** 16, 2 ** 16 * 7)
-    assert np.allclose(f, [10 / 24, np.log2(3070) / 32, 20 / 32, 4 / 32, 2 / 16, 8 / 24, 3 / 8, 16 / 32], rtol=1e-6)
+    f = mlp.features(1023, 1023 * 3, 2 ** 20 - 1, 7, 255, 2 ** 16, 2 ** 16 * 7)
+    assert np.allclose(f, [10 / 24, np.log2(3070) / 32, 20 / 32, 3 / 12, 2 / 16, 8 / 24, 3 / 8, 16 / 32], rtol=1e-6)
+
+
+def test_training_features_match_the_oracle():
+    """train_policy.level_features (the training set) must compute what the
+    device computes at a decision; the oracle restates the device formula."""
+    from sygraph_amd.train_policy import level_features
+    rng = np.random.default_rng(3)
+    stats = [(int(rng.integers(1, 10 ** 6)), int(rng.integers(1, 10 ** 8)), int(rng.integers(0, 10 ** 9)),
+              int(rng.integers(0, 10 ** 5))) for _ in range(40)]
+    n, m = 1 << 22, 1 << 28
+    got = level_features(stats, n, m)
+    want = np.stack([mlp.features(stats[L][0], stats[L][1], stats[L - 1][2], L, stats[L - 1][3], n, m)
+                     for L in range(1, len(stats))])
+    assert np.array_equal(got, want)

@@ -108,3 +108,33 @@ def test_learned_without_policy_is_rejected(sg, device, oracle_mod):
     b.init(0)
     with pytest.raises(sg.SygraphError):
         b.run(sg.POLICY_LEARNED)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "grid"])
+def test_in_situ_decisions_match_the_oracle_forward(sg, device, oracle_mod, kind):
+    """Every level's action under a random policy equals the FP64 oracle's
+    argmax on the features of the logged level statistics (decisions whose
+    two best scores lie within FP32 rounding of each other are skipped)."""
+    g = oracle_mod.rmat(14, 16, seed=35) if kind == "rmat" else oracle_mod.grid(96, 96, 0.1, seed=36)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols)
+    rng = np.random.default_rng(37)
+    checked = set()
+    for trial in range(4):
+        p = rng.normal(0, 1.0, mlp.N_PARAMS).astype(np.float32)
+        pol = sg.Policy(device, p)
+        src = int(rng.choice(np.nonzero(g.degrees() > 0)[0]))
+        b = sg.BFS(G)
+        b.init(src)
+        st = b.run(sg.POLICY_LEARNED, learned=pol)
+        assert np.array_equal(b.get_distances(), oracle_mod.bfs(g, src)[0])
+        lv = [st.level[i] for i in range(st.levels)]
+        for L in range(1, st.levels):
+            f = mlp.features(lv[L].frontier_vertices, lv[L].frontier_edges, lv[L - 1].unexplored_edges, L,
+                             lv[L - 1].max_degree, g.n, g.m)
+            s, _ = mlp.forward(p.astype(np.float64), f[None, :].astype(np.float64))
+            top = np.sort(s[0])
+            if top[-1] - top[-2] <= 1e-4 * (1 + abs(top[-1])):
+                continue
+            assert lv[L].action == int(np.argmax(s[0])), (trial, L, s[0], lv[L].action)
+            checked.add(lv[L].action)
+    assert len(checked) >= 2, checked  # the random policies exercised several actions
