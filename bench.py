@@ -152,8 +152,9 @@ def batched_vs_plain(sg, G, sources16):
 def secondary(sg, dev):
     """CC, TC and BC on the other configurations of the path (best of 3,
     graphs generated on the device): CC on the 4096 x 4096 grid with 10 % of
-    the edges deleted and on RMAT scale 20 edge factor 32; TC on that RMAT;
-    BC from 64 sources on RMAT scale 20 edge factor 48, a = 0.55."""
+    the edges deleted and on RMAT scale 20 edge factor 32; BFS on that grid
+    with and without bursts of small levels; TC on that RMAT; BC from 64
+    sources on RMAT scale 20 edge factor 48, a = 0.55."""
     out = {}
 
     def best(fn, reps=3):
@@ -172,6 +173,18 @@ def secondary(sg, dev):
     t = best(cc.run)
     out["cc_grid4096_ms"] = round(t * 1e3, 3)
     out["cc_grid4096_gteps"] = round(G.m / t / 1e9, 3)
+    # BFS on the grid (diameter ~8,000 levels) from the first non-isolated
+    # vertex (a corner), with small levels run in single-workgroup bursts
+    # (default) and without
+    off, _, _ = G.download()
+    src = int(np.nonzero(np.diff(off) > 0)[0][0])
+    b = sg.BFS(G)
+    for key, burst in (("bfs_grid4096_ms", 0), ("bfs_grid4096_no_burst_ms", -1)):
+        def one():
+            b.init(src)
+            b.run(sg.POLICY_HYBRID, 15.0, 24.0, tuning={"burst_arcs": burst})
+        out[key] = round(best(one, reps=2) * 1e3, 3)
+    out["bfs_grid4096_levels"] = int(b.stats.levels)
     G.destroy()
     G = sg.Graph.rmat(dev, 20, 32, seed=5)
     cc = sg.CC(G)

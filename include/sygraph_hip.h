@@ -247,6 +247,9 @@ typedef struct sg_bfs_options {
   int32_t t_large;         /* degree from which its arcs are split into chunks (default 1024) */
   int32_t chunk;           /* arcs per chunk of a large vertex (default 1024, at least 64) */
   int32_t blocks_per_cu;   /* workgroups per compute unit of the traversal kernel (default 8) */
+  int32_t burst_arcs;      /* push levels whose frontier has at most this many arcs run back to back
+                              inside one single-workgroup launch (default 4096, at most 65536;
+                              negative: never) */
 } sg_bfs_options;
 
 typedef struct sg_level_stat {
@@ -265,6 +268,8 @@ typedef struct sg_level_stat {
                                 short pull head, 2 wave-split / long pull head) of this level */
   float kernel_ms;           /* device time of the level's advance kernel (record_levels) */
   float convert_ms;          /* device time of a queue<->bitmap conversion after it */
+  int32_t burst;             /* 1: the level ran inside a single-workgroup burst of small push levels;
+                                the burst's device time is split evenly over its levels (kernel_ms) */
 } sg_level_stat;
 
 typedef struct sg_bfs_stats {

@@ -68,7 +68,8 @@ class FrontierView(C.Structure):
 class BfsOptions(C.Structure):
     _fields_ = [("policy", C.c_int32), ("load_balancer", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
                 ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p),
-                ("t_medium", C.c_int32), ("t_large", C.c_int32), ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32)]
+                ("t_medium", C.c_int32), ("t_large", C.c_int32), ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("burst_arcs", C.c_int32)]
 
 
 class LevelStat(C.Structure):
@@ -76,7 +77,7 @@ class LevelStat(C.Structure):
                 ("frontier_edges", C.c_int64), ("discovered", C.c_int64), ("edges_inspected", C.c_int64),
                 ("vertices_scanned", C.c_int64), ("hint_misses", C.c_int64), ("unexplored_edges", C.c_int64),
                 ("max_degree", C.c_int64), ("action", C.c_int32),
-                ("kernel_ms", C.c_float), ("convert_ms", C.c_float)]
+                ("kernel_ms", C.c_float), ("convert_ms", C.c_float), ("burst", C.c_int32)]
 
 
 class SsspStats(C.Structure):
@@ -535,7 +536,7 @@ class BFS:
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
             load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
             learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsStats:
-        """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu."""
+        """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu, burst_arcs."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
         o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,

@@ -1,7 +1,7 @@
 """Autotuner for the traversal knobs of one graph on one device.
 
     from sygraph_amd import autotune
-    best = autotune.tune_bfs(G, sources)          # {"t_medium":..., "t_large":..., "chunk":..., "blocks_per_cu":...}
+    best = autotune.tune_bfs(G, sources)          # {"t_medium":..., "t_large":..., "chunk":..., "blocks_per_cu":..., "burst_arcs":...}
     bfs.run(sg.POLICY_HYBRID, 15, 24, tuning=best)
     delta = autotune.tune_sssp(G, sources)        # bucket width for SSSP.run(delta=...)
 
@@ -31,6 +31,7 @@ BFS_GRID = {
     "t_large": [512, 1024, 4096],
     "chunk": [512, 1024, 2048],
     "blocks_per_cu": [4, 8, 16],
+    "burst_arcs": [1024, 4096, 16384],
 }
 SSSP_DELTAS = [-1.0, 2.0, 4.0, 8.0, 16.0, 32.0, 64.0, 128.0]
 

@@ -59,28 +59,27 @@ struct BfsArgs {
   const QueueHdr* qin_hdr;
   StatSlot* stats;
   StatSlot* stats_zero;
-  int32_t level;
 };
 
 template<typename OffT>
-__device__ __forceinline__ void push_visit(const BfsArgs<OffT>& a, uint32_t y) {
-  if (claim_bit(a.visited, y)) a.depth[y] = a.level + 1;
+__device__ __forceinline__ void push_visit(const BfsArgs<OffT>& a, uint32_t y, int32_t level) {
+  if (claim_bit(a.visited, y)) a.depth[y] = level + 1;
 }
 
 /* Push level: every arc of the class queues; discovery is the test-and-set of
  * the visited bit. */
 template<typename OffT>
-__device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant) {
+__device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
   zero_slots(a.stats_zero);
   for_each_queued_arc<OffT>(
-      a.qin, a.qin_hdr, a.cols, [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y); }, variant);
+      a.qin, a.qin_hdr, a.cols, [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y, level); }, variant);
 }
 
 /* Pull level.  A wave loads 64 visited words at once and compacts their
  * unvisited vertices in registers (popcount scan + bit select), so all 64 lanes
  * pull for real vertices; new frontier bits are gathered per word in LDS. */
 template<typename OffT>
-__device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) {
+__device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
   zero_slots(a.stats_zero);
   /* arcs a lane scans alone before the wave helps: 16 / 4 / 64 */
   const int head_steps = variant == 1 ? 1 : (variant == 2 ? kPullHead : kPullHead / 4);
@@ -180,7 +179,7 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) 
         st_edges += 1 + kk; /* hint probe + list probes after a miss */
         st_miss += miss ? 1 : 0;
         if (found) {
-          a.depth[v] = a.level + 1;
+          a.depth[v] = level + 1;
           atomicOr(&wnew[owner], 1ull << bit);
           ++st_disc;
           st_deg += odeg;
@@ -199,29 +198,42 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant) 
   flush_block_stats(a.stats, st_disc, st_deg, st_edges, st_scanned, st_max, st_miss);
 }
 
-/* Device-resident control block of the level loop.  Every kernel of a level
- * reads it (kernel boundaries order the accesses); only count_dyn_kernel and
- * finish_kernel write it. */
+/* Device-resident control block of the level loop.  Every kernel of a step
+ * reads it (kernel boundaries order the accesses); count_dyn_kernel,
+ * finish_kernel and burst_kernel write it. */
 struct BfsCtl {
-  int32_t dir;      /* direction of the level about to run: 0 push, 1 pull */
-  int32_t done;     /* a level discovered nothing: later levels are no-ops */
+  int32_t level;    /* the level about to run */
+  int32_t dir;      /* its direction: 0 push, 1 pull */
+  int32_t done;     /* a level discovered nothing: later steps are no-ops */
   int32_t mode;     /* count pass of this level: 0 none, 1 visited & ~prev, 2 frontier bitmap + prev sync */
   int32_t write;    /* queues for the next (push) level must be written */
   int32_t next_act; /* decided after a pull level by count_dyn_kernel: direction * 3 + variant */
   int32_t lb;       /* work-mapping variant of the level about to run */
+  int32_t burst;    /* the next level is small enough for burst_kernel */
+  int32_t n_push, n_pull;
   int64_t f_vertices, f_edges; /* frontier entering the level */
   unsigned long long visited_deg, reached;
   unsigned long long disc, disc_deg, edges, scanned, misses, max_deg; /* staged pull-level statistics */
 };
 
-/* One level's record, written by finish_kernel into a ring in pinned host
- * memory; the host reads it after the level's completion event. */
+/* One level's record, written into pinned host memory (levels below
+ * SG_MAX_LEVELS_RECORDED) by finish_kernel or, inside a burst, by
+ * burst_kernel. */
 struct alignas(64) LevelLog {
-  int32_t level, dir, done, next_dir, action;
+  int32_t level, dir, done, next_dir, action, burst;
   int64_t f_vertices, f_edges, disc, disc_deg, edges, scanned, misses, max_deg, unexplored;
 };
 
-constexpr int kRing = 8; /* levels in the log / event rings (> lookahead + 1) */
+/* One step's record (a step = the launches the host enqueues together: one
+ * level, plus the levels of a burst), written into a ring in pinned host
+ * memory by the step's last working kernel; the host reads it after the
+ * step's completion event. */
+struct alignas(64) StepLog {
+  int32_t step, last_level, done, n_push, n_pull;
+  int64_t reached_v, reached_e;
+};
+
+constexpr int kRing = 8; /* steps in the log / event rings (> lookahead + 1) */
 
 struct DecideParams {
   int32_t policy;
@@ -231,6 +243,7 @@ struct DecideParams {
   int32_t lb_fixed;     /* variant under the push / pull / hybrid policies */
   const float* mlp;     /* learned policy parameters (SG_POLICY_LEARNED) */
   double n_d, m_d;      /* vertex and arc counts for the features */
+  int64_t burst_arcs;   /* frontier arcs up to which push levels run in burst_kernel (0: never) */
 };
 
 /* evaluateHeuristic (algorithms/bfs.hpp:341-350): push -> pull when the new
@@ -293,8 +306,9 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
   if (ctl->done) return;
-  if (ctl->dir == 0) push_level(a, ctl->lb);
-  else pull_level(a, ctl->lb);
+  const int32_t level = ctl->level;
+  if (ctl->dir == 0) push_level(a, ctl->lb, level);
+  else pull_level(a, ctl->lb, level);
 }
 
 /* Count pass of the level.  After a push it counts visited & ~prev.  After a
@@ -304,7 +318,7 @@ __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const Bf
  * extra launch -- and workgroup 0 records the decision for finish_kernel. */
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, BfsCtl* ctl, const StatSlot* slots,
-                                                           DecideParams p, int32_t level) {
+                                                           DecideParams p) {
   __shared__ int32_t s_mode;
   __shared__ sg::policy::Staged s_pol;
   if (ctl->done) return;
@@ -315,6 +329,7 @@ __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, 
   }
   stage_policy(p, &s_pol);
   __syncthreads();
+  const int32_t level = ctl->level;
   if (threadIdx.x < 64) {
     const StatSlot& x = slots[threadIdx.x];
     const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
@@ -344,10 +359,11 @@ __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, 
 
 /* Scan of the count pass, the push-level decision (wave 0, so the learned
  * policy can run on it), the level record, and the control block for the
- * next level. */
+ * next level.  Unless burst_kernel takes over the next levels, this is the
+ * step's last working kernel and writes the step record. */
 __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, int64_t ntiles, QueueHdr* hdr_out,
-                                                              BfsCtl* ctl, DecideParams p, int32_t level,
-                                                              LevelLog* log) {
+                                                              BfsCtl* ctl, DecideParams p, int32_t step,
+                                                              LevelLog* levlog, StepLog* steplog) {
   __shared__ QueueHdr tot;
   __shared__ sg::policy::Staged s_pol;
   if (ctl->done) return;
@@ -356,7 +372,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   if (mode != 0) compact_scan_body(tiles, ntiles, hdr_out, &tot);
   __syncthreads();
   if (threadIdx.x >= 64) return;
-  const int32_t dir = ctl->dir;
+  const int32_t dir = ctl->dir, level = ctl->level;
   unsigned long long disc, disc_deg, edges, scanned, misses, max_deg;
   int32_t act;
   const unsigned long long vdeg_before = ctl->visited_deg;
@@ -368,7 +384,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     scanned = (unsigned long long)ctl->f_vertices;
     misses = 0;
     act = decide_action(p, &s_pol, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1,
-                      max_deg);
+                        max_deg);
   } else {
     disc = ctl->disc;
     disc_deg = ctl->disc_deg;
@@ -381,37 +397,322 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   if (threadIdx.x != 0) return;
   const int32_t next = act / 3;
   const unsigned long long vdeg = vdeg_before + disc_deg;
-  LevelLog e;
-  e.level = level;
-  e.dir = dir;
-  e.done = disc == 0 ? 1 : 0;
-  e.next_dir = next;
-  e.action = dir * 3 + ctl->lb;
-  e.f_vertices = ctl->f_vertices;
-  e.f_edges = ctl->f_edges;
-  e.disc = (int64_t)disc;
-  e.disc_deg = (int64_t)disc_deg;
-  e.edges = (int64_t)edges;
-  e.scanned = (int64_t)scanned;
-  e.misses = (int64_t)misses;
-  e.max_deg = (int64_t)max_deg;
-  e.unexplored = (int64_t)(p.total_degree - vdeg);
-  log[level % kRing] = e;
-  __threadfence_system();
+  const int32_t done = disc == 0 ? 1 : 0;
+  if (level < SG_MAX_LEVELS_RECORDED) {
+    LevelLog e;
+    e.level = level;
+    e.dir = dir;
+    e.done = done;
+    e.next_dir = next;
+    e.action = dir * 3 + ctl->lb;
+    e.burst = 0;
+    e.f_vertices = ctl->f_vertices;
+    e.f_edges = ctl->f_edges;
+    e.disc = (int64_t)disc;
+    e.disc_deg = (int64_t)disc_deg;
+    e.edges = (int64_t)edges;
+    e.scanned = (int64_t)scanned;
+    e.misses = (int64_t)misses;
+    e.max_deg = (int64_t)max_deg;
+    e.unexplored = (int64_t)(p.total_degree - vdeg);
+    levlog[level] = e;
+  }
+  const int32_t n_push = ctl->n_push + (dir == 0 ? 1 : 0), n_pull = ctl->n_pull + (dir == 0 ? 0 : 1);
+  const unsigned long long reached = ctl->reached + disc;
+  const int32_t burst = (!done && next == 0 && p.burst_arcs > 0 && (int64_t)disc_deg <= p.burst_arcs) ? 1 : 0;
   ctl->visited_deg = vdeg;
-  ctl->reached += disc;
+  ctl->reached = reached;
   ctl->f_vertices = (int64_t)disc;
   ctl->f_edges = (int64_t)disc_deg;
-  ctl->write = (disc > 0 && next == 0) ? 1 : 0;
-  ctl->done = disc == 0 ? 1 : 0;
+  ctl->write = (!done && next == 0) ? 1 : 0;
+  ctl->done = done;
   ctl->dir = next;
   ctl->lb = act % 3;
+  ctl->level = level + 1;
+  ctl->n_push = n_push;
+  ctl->n_pull = n_pull;
+  ctl->burst = burst;
+  if (!burst) {
+    StepLog sl;
+    sl.step = step;
+    sl.last_level = level;
+    sl.done = done;
+    sl.n_push = n_push;
+    sl.n_pull = n_pull;
+    sl.reached_v = (int64_t)reached;
+    sl.reached_e = (int64_t)vdeg;
+    steplog[step % kRing] = sl;
+  }
+  __threadfence_system();
 }
 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) write_dyn_kernel(CompactArgs<OffT> a, const BfsCtl* ctl) {
   if (!ctl->write) return;
   compact_write_body<OffT>(a);
+}
+
+/* ------------------------------------------------------------ bursts --- */
+/* Small push levels are bound by launch and dispatch cost, not bandwidth: a
+ * level with a few thousand frontier arcs takes four grid-wide launches
+ * (about 40 us) for microseconds of memory work.  burst_kernel, one
+ * 512-thread workgroup, runs such levels back to back in one launch with
+ * workgroup barriers between them, for as long as the policy keeps pushing and
+ * the frontier keeps at most burst_arcs arcs.  The frontier lives in two
+ * lists of (first arc, vertex, degree) entries; each wave flattens 64 entries
+ * at a time into arcs (merge path) and claims discoveries four arcs per lane
+ * per round, so four claims are in flight per lane.
+ *
+ * State handed back to the level pipeline when the burst stops:
+ *  - visited, depth: every discovery, as in a push level;
+ *  - prev == visited (each discovery also sets its prev bit), so the next
+ *    count pass sees only that level's discoveries;
+ *  - the queues Q[nxt] and header hold the last discovered set, for a push;
+ *  - the frontier bitmap bm[nxt] gains the last discovered set.  It still
+ *    holds the frontier the burst started from: a pull tests only
+ *    in-neighbours of unvisited vertices, and every out-neighbour of an
+ *    earlier frontier vertex is visited, so those bits never match. */
+constexpr int kBurstThreads = 512;
+constexpr int kBurstWaves = kBurstThreads / 64;
+constexpr int kBurstDefaultArcs = 4096;
+constexpr int kBurstMaxArcs = 65536; /* capacity of each burst list (entries) */
+
+template<typename OffT>
+struct BurstArgs {
+  const OffT* off;
+  const int32_t* cols;
+  uint64_t* visited;
+  uint64_t* prev;
+  uint64_t* fbm;   /* frontier bitmap read by a pull at the next step (bm[nxt]) */
+  int32_t* depth;
+  QEntry* q[3];    /* queues of the next step (Q[nxt]) */
+  QueueHdr* hdr;
+  QEntry* list[2]; /* burst frontier lists, kBurstMaxArcs entries each */
+  Tuning tune;
+};
+
+struct BurstAcc {
+  unsigned long long disc, deg, mx;
+};
+
+/* Visit every arc of `cnt` entries; discoveries are claimed, get depth
+ * level + 1 and their prev bit, and are appended to `out`.  Per round a lane
+ * takes four arcs: four id loads, then four claims (atomic OR on the visited
+ * word) issued together with the eight offset loads a discovery needs, then
+ * one LDS counter update per wave for all four, then the stores -- three
+ * dependent memory round trips per round. */
+template<typename OffT>
+__device__ __forceinline__ void burst_visit(const BurstArgs<OffT>& a, const QEntry* ents, uint32_t cnt, int32_t level,
+                                            QEntry* out, uint32_t* s_count, BurstAcc& acc) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (uint32_t base = (uint32_t)wv * 64; base < cnt; base += kBurstThreads) {
+    QEntry qe;
+    qe.start = 0;
+    qe.v = 0;
+    qe.len = 0;
+    if (base + lane < cnt) qe = ents[base + lane];
+    const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - (uint32_t)qe.len;
+    for (uint32_t t0 = 0; t0 < total; t0 += 4 * 64) {
+      uint32_t y[4];
+      bool ok[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t t = t0 + 64 * k + lane;
+        const int owner = scan_owner(incl, t);
+        const uint32_t before = __shfl(excl, owner, 64);
+        const int64_t st = shfl_i64(qe.start, owner);
+        ok[k] = t < total;
+        y[k] = ok[k] ? (uint32_t)a.cols[st + (t - before)] : 0u;
+      }
+      uint64_t old[4];
+      int64_t vs[4], ve[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        old[k] = ~0ull;
+        vs[k] = ve[k] = 0;
+        if (ok[k]) {
+          old[k] = atomic_or(&a.visited[y[k] >> 6], 1ull << (y[k] & 63));
+          vs[k] = (int64_t)a.off[y[k]];
+          ve[k] = (int64_t)a.off[y[k] + 1];
+        }
+      }
+      bool nw[4];
+      uint64_t m[4];
+      uint32_t cnt4 = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        nw[k] = ((old[k] >> (y[k] & 63)) & 1ull) == 0ull;
+        m[k] = ballot(nw[k]);
+        cnt4 += (uint32_t)__popcll(m[k]);
+      }
+      if (cnt4 == 0) continue;
+      uint32_t pos = 0;
+      if (lane == 0) pos = atomicAdd(s_count, cnt4);
+      pos = __shfl(pos, 0, 64);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (nw[k]) {
+          const uint32_t v = y[k];
+          a.depth[v] = level + 1;
+          atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
+          QEntry e;
+          e.start = vs[k];
+          e.v = (int32_t)v;
+          e.len = (int32_t)(ve[k] - vs[k]);
+          out[pos + lanes_below(m[k])] = e;
+          ++acc.disc;
+          acc.deg += (unsigned long long)e.len;
+          acc.mx = (unsigned long long)e.len > acc.mx ? (unsigned long long)e.len : acc.mx;
+        }
+        pos += (uint32_t)__popcll(m[k]);
+      }
+    }
+  }
+}
+
+/* One barrier per level: the counter and the partial sums are double
+ * buffered by level parity, so a wave may start the next level while others
+ * still read this level's sums; every wave evaluates the (deterministic)
+ * decision itself. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a, BfsCtl* ctl, DecideParams p,
+                                                              int32_t step, LevelLog* levlog, StepLog* steplog) {
+  __shared__ sg::policy::Staged s_pol;
+  __shared__ uint32_t s_count[2], s_cls[3];
+  __shared__ unsigned long long s_red[2][kBurstWaves][3];
+  if (ctl->done || !ctl->burst) return;
+  stage_policy(p, &s_pol);
+  if (threadIdx.x == 0) s_count[0] = 0;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  int32_t level = ctl->level, lb = ctl->lb;
+  int64_t f_vertices = ctl->f_vertices, f_edges = ctl->f_edges;
+  unsigned long long vdeg = ctl->visited_deg, reached = ctl->reached;
+  int32_t n_push = ctl->n_push, done = 0, act = 0;
+  const QueueHdr h0 = *a.hdr;
+  int cur = 0;
+  unsigned long long disc = 0, disc_deg = 0, max_deg = 0;
+  __syncthreads();
+  for (int it = 0;; ++it) {
+    const int par = it & 1;
+    BurstAcc acc{0, 0, 0};
+    QEntry* out = a.list[cur ^ 1];
+    if (it == 0) {
+      for (int c = 0; c < 3; ++c) burst_visit(a, a.q[c], h0.count[c], level, out, &s_count[par], acc);
+    } else {
+      burst_visit(a, a.list[cur], (uint32_t)f_vertices, level, out, &s_count[par], acc);
+    }
+    acc.disc = wave_sum_u64(acc.disc);
+    acc.deg = wave_sum_u64(acc.deg);
+    acc.mx = wave_max_u64(acc.mx);
+    if (lane == 0) {
+      s_red[par][wv][0] = acc.disc;
+      s_red[par][wv][1] = acc.deg;
+      s_red[par][wv][2] = acc.mx;
+    }
+    if (threadIdx.x == 0) s_count[par ^ 1] = 0; /* last used two levels ago */
+    __syncthreads(); /* discoveries, the list `out` and the partial sums are complete */
+    disc = 0;
+    disc_deg = 0;
+    max_deg = 0;
+    for (int w = 0; w < kBurstWaves; ++w) {
+      disc += s_red[par][w][0];
+      disc_deg += s_red[par][w][1];
+      max_deg = s_red[par][w][2] > max_deg ? s_red[par][w][2] : max_deg;
+    }
+    const unsigned long long unexplored = p.total_degree - (vdeg + disc_deg);
+    act = decide_action(p, &s_pol, 0, disc, disc_deg, unexplored, level + 1, max_deg);
+    done = disc == 0 ? 1 : 0;
+    if (threadIdx.x == 0 && level < SG_MAX_LEVELS_RECORDED) {
+      LevelLog e;
+      e.level = level;
+      e.dir = 0;
+      e.done = done;
+      e.next_dir = act / 3;
+      e.action = lb;
+      e.burst = 1;
+      e.f_vertices = f_vertices;
+      e.f_edges = f_edges;
+      e.disc = (int64_t)disc;
+      e.disc_deg = (int64_t)disc_deg;
+      e.edges = f_edges;
+      e.scanned = f_vertices;
+      e.misses = 0;
+      e.max_deg = (int64_t)max_deg;
+      e.unexplored = (int64_t)unexplored;
+      levlog[level] = e;
+    }
+    vdeg += disc_deg;
+    reached += disc;
+    ++n_push;
+    f_vertices = (int64_t)disc;
+    f_edges = (int64_t)disc_deg;
+    lb = act % 3;
+    ++level;
+    cur ^= 1;
+    const bool more = !done && act / 3 == 0 && (int64_t)disc_deg <= p.burst_arcs;
+    if (!more) break;
+  }
+  /* hand the last discovered set (list[cur]) back to the level pipeline */
+  const QEntry* last = a.list[cur];
+  if (threadIdx.x < 3) s_cls[threadIdx.x] = 0;
+  __syncthreads();
+  if (!done) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)f_vertices; i += kBurstThreads) {
+      const QEntry e = last[i];
+      atomic_or(&a.fbm[(uint32_t)e.v >> 6], 1ull << ((uint32_t)e.v & 63));
+      if (e.len <= 0) continue;
+      const int cls = degree_class(a.tune, e.len);
+      if (cls < 2) {
+        const uint32_t pos = atomicAdd(&s_cls[cls], 1u);
+        a.q[cls][pos] = e;
+      } else {
+        const uint32_t nch = (uint32_t)((e.len + a.tune.chunk - 1) / a.tune.chunk);
+        const uint32_t pos = atomicAdd(&s_cls[2], nch);
+        for (uint32_t c = 0; c < nch; ++c) {
+          QEntry ce;
+          ce.start = e.start + (int64_t)c * a.tune.chunk;
+          ce.v = e.v;
+          const int64_t rem = (int64_t)e.len - (int64_t)c * a.tune.chunk;
+          ce.len = (int32_t)(rem < a.tune.chunk ? rem : a.tune.chunk);
+          a.q[2][pos + c] = ce;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    QueueHdr h{};
+    h.count[0] = s_cls[0];
+    h.count[1] = s_cls[1];
+    h.count[2] = s_cls[2];
+    h.nvert = disc;
+    h.deg = disc_deg;
+    h.maxdeg = max_deg;
+    *a.hdr = h;
+    ctl->level = level;
+    ctl->dir = act / 3;
+    ctl->lb = lb;
+    ctl->done = done;
+    ctl->write = 0;
+    ctl->burst = 0;
+    ctl->n_push = n_push;
+    ctl->visited_deg = vdeg;
+    ctl->reached = reached;
+    ctl->f_vertices = f_vertices;
+    ctl->f_edges = f_edges;
+    StepLog sl;
+    sl.step = step;
+    sl.last_level = level - 1;
+    sl.done = done;
+    sl.n_push = n_push;
+    sl.n_pull = ctl->n_pull;
+    sl.reached_v = (int64_t)reached;
+    sl.reached_e = (int64_t)vdeg;
+    steplog[step % kRing] = sl;
+    __threadfence_system();
+  }
 }
 
 __global__ void copy_words_kernel(uint64_t* dst, uint64_t* dst2, const uint64_t* src, int64_t nw) {
@@ -435,8 +736,10 @@ struct BfsWorkspace {
   QueueSet qs;
   StatSlot* stats = nullptr; /* 2 areas of kSlots */
   BfsCtl* ctl = nullptr;
-  LevelLog* log = nullptr; /* kRing entries, pinned coherent host memory */
-  hipEvent_t ev_start[kRing] = {}, ev_mid[kRing] = {}, ev_done[kRing] = {};
+  QEntry* burst_list[2] = {nullptr, nullptr}; /* kBurstMaxArcs entries each */
+  LevelLog* levlog = nullptr; /* SG_MAX_LEVELS_RECORDED entries, pinned coherent host memory */
+  StepLog* steplog = nullptr; /* kRing entries, pinned coherent host memory */
+  hipEvent_t ev_start[kRing] = {}, ev_mid[kRing] = {}, ev_burst[kRing] = {}, ev_done[kRing] = {};
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
@@ -449,9 +752,12 @@ void ws_free(BfsWorkspace* w) {
   w->qs.release();
   (void)hipFree(w->stats);
   (void)hipFree(w->ctl);
-  if (w->log) (void)hipHostFree(w->log);
+  (void)hipFree(w->burst_list[0]);
+  (void)hipFree(w->burst_list[1]);
+  if (w->levlog) (void)hipHostFree(w->levlog);
+  if (w->steplog) (void)hipHostFree(w->steplog);
   for (int r = 0; r < kRing; ++r)
-    for (hipEvent_t e : {w->ev_start[r], w->ev_mid[r], w->ev_done[r]})
+    for (hipEvent_t e : {w->ev_start[r], w->ev_mid[r], w->ev_burst[r], w->ev_done[r]})
       if (e) (void)hipEventDestroy(e);
   if (w->ev_begin) (void)hipEventDestroy(w->ev_begin);
   if (w->ev_end) (void)hipEventDestroy(w->ev_end);
@@ -473,14 +779,18 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
       (st = sg::alloc_device(dev, (void**)&w->bm[1], 8 * (w->w0 + 1))) ||
       (st = sg::alloc_device(dev, (void**)&w->stats, sizeof(StatSlot) * kSlots * 2)) ||
       (st = sg::alloc_device(dev, (void**)&w->ctl, sizeof(BfsCtl))) ||
+      (st = sg::alloc_device(dev, (void**)&w->burst_list[0], sizeof(QEntry) * kBurstMaxArcs)) ||
+      (st = sg::alloc_device(dev, (void**)&w->burst_list[1], sizeof(QEntry) * kBurstMaxArcs)) ||
       (st = w->qs.init(dev, g->n, g->m, default_tuning()))) {
     ws_free(w);
     return st;
   }
-  bool ok = hipHostMalloc((void**)&w->log, sizeof(LevelLog) * kRing, hipHostMallocCoherent) == hipSuccess;
+  bool ok = hipHostMalloc((void**)&w->levlog, sizeof(LevelLog) * SG_MAX_LEVELS_RECORDED, hipHostMallocCoherent) ==
+                hipSuccess &&
+            hipHostMalloc((void**)&w->steplog, sizeof(StepLog) * kRing, hipHostMallocCoherent) == hipSuccess;
   for (int r = 0; r < kRing && ok; ++r)
     ok = hipEventCreate(&w->ev_start[r]) == hipSuccess && hipEventCreate(&w->ev_mid[r]) == hipSuccess &&
-         hipEventCreate(&w->ev_done[r]) == hipSuccess;
+         hipEventCreate(&w->ev_burst[r]) == hipSuccess && hipEventCreate(&w->ev_done[r]) == hipSuccess;
   ok = ok && hipEventCreate(&w->ev_begin) == hipSuccess && hipEventCreate(&w->ev_end) == hipSuccess;
   if (!ok) {
     ws_free(w);
@@ -504,13 +814,15 @@ void bfs_workspace_release(sg_graph_t g) {
 }
 } // namespace sg
 
-/* Host side of the level loop.  Each level is a fixed sequence of four
- * launches (traverse, decide+count, scan+finish, write) whose behaviour the
- * device-resident control block selects, so the host never has to decide
- * anything between levels: it keeps `lookahead` levels enqueued beyond the one
- * it waits for and only reads each level's record (pinned host memory) after
- * that level's completion event, to stop after the first empty level.  Levels
- * enqueued beyond the last one find `done` set and return at once. */
+/* Host side of the level loop.  Each step is a fixed sequence of five
+ * launches (traverse, decide+count, scan+finish, write, burst) whose
+ * behaviour the device-resident control block selects, so the host never
+ * has to decide anything between levels: it keeps `lookahead` steps enqueued
+ * beyond the one it waits for and only reads each step's record (pinned host
+ * memory) after that step's completion event, to stop after the first empty
+ * level.  A step runs one level, plus the levels of a burst when the next
+ * frontiers are small.  Steps enqueued beyond the last level find `done` set
+ * and return at once. */
 template<typename OffT>
 int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats) {
   sg_device_t dev = g->dev;
@@ -571,12 +883,12 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   P.mlp = opt.policy == SG_POLICY_LEARNED ? sg::policy_device_params(opt.learned) : nullptr;
   P.n_d = (double)n;
   P.m_d = (double)g->m;
+  P.burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
   if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 
-  auto enqueue = [&](int32_t L) -> int {
+  auto enqueue = [&](int32_t step) -> int {
     BfsArgs<OffT> la = a;
-    const int cur = L & 1, nxt = (L + 1) & 1, r = L % kRing;
-    la.level = L;
+    const int cur = step & 1, nxt = (step + 1) & 1, r = step % kRing;
     la.stats = ws->stats + kSlots * cur;
     la.stats_zero = ws->stats + kSlots * nxt;
     for (int k = 0; k < 3; ++k) la.qin[k] = Q.q[cur][k];
@@ -584,12 +896,27 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     la.cur_bm = ws->bm[cur];
     la.next_bm = ws->bm[nxt];
     const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
+    BurstArgs<OffT> ba;
+    ba.off = off;
+    ba.cols = g->cols;
+    ba.visited = ws->visited;
+    ba.prev = ws->prev;
+    ba.fbm = ws->bm[nxt];
+    ba.depth = depth;
+    for (int k = 0; k < 3; ++k) ba.q[k] = Q.q[nxt][k];
+    ba.hdr = Q.hdr + nxt;
+    ba.list[0] = ws->burst_list[0];
+    ba.list[1] = ws->burst_list[1];
+    ba.tune = tune;
     if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
-    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P, L);
-    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, L, ws->log);
+    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P);
+    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, step, ws->levlog,
+                                             ws->steplog);
     write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    if (timed) SG_HIP(hipEventRecord(ws->ev_burst[r], s));
+    burst_kernel<OffT><<<1, kBurstThreads, 0, s>>>(ba, ws->ctl, P, step, ws->levlog, ws->steplog);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev_done[r], s));
     return SG_OK;
@@ -597,52 +924,59 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
 
   SG_TRY(enqueue(0));
   for (int k = 1; k <= lookahead; ++k) SG_TRY(enqueue(k));
-  int32_t levels = 0, n_push = 0, n_pull = 0;
-  int64_t reached_v = 1, reached_e = 0;
-  for (int32_t L = 0;; ++L) {
-    const int r = L % kRing;
+  StepLog last{};
+  int32_t first_level = 0;
+  for (int32_t step = 0;; ++step) {
+    const int r = step % kRing;
     SG_HIP(hipEventSynchronize(ws->ev_done[r]));
-    LevelLog e;
-    std::memcpy(&e, (const void*)&ws->log[r], sizeof(e));
-    if (e.level != L) {
+    std::memcpy(&last, (const void*)&ws->steplog[r], sizeof(last));
+    if (last.step != step || last.last_level < first_level) {
       SG_HIP(hipStreamSynchronize(s));
-      sg::set_error("sg_bfs: level record %d out of order (found %d)", L, e.level);
+      sg::set_error("sg_bfs: step record %d out of order (found step %d, level %d)", step, last.step,
+                    last.last_level);
       return SG_ERR_STATE;
     }
-    if (L == 0) reached_e = e.f_edges;
-    reached_v += e.disc;
-    reached_e += e.disc_deg;
-    if (e.dir == 0) ++n_push;
-    else ++n_pull;
-    if (stats && L < SG_MAX_LEVELS_RECORDED) {
-      sg_level_stat& ls = stats->level[L];
-      std::memset(&ls, 0, sizeof(ls));
-      ls.direction = e.dir;
-      ls.load_balancer = opt.load_balancer;
-      ls.frontier_vertices = e.f_vertices;
-      ls.frontier_edges = e.f_edges;
-      ls.discovered = e.disc;
-      ls.edges_inspected = e.edges;
-      ls.vertices_scanned = e.scanned;
-      ls.hint_misses = e.misses;
-      ls.max_degree = e.max_deg;
-      ls.unexplored_edges = e.unexplored;
-      ls.action = e.action;
+    if (stats) {
+      float k_ms = 0.f, c_ms = 0.f, b_ms = 0.f;
       if (timed) {
-        SG_HIP(hipEventElapsedTime(&ls.kernel_ms, ws->ev_start[r], ws->ev_mid[r]));
-        SG_HIP(hipEventElapsedTime(&ls.convert_ms, ws->ev_mid[r], ws->ev_done[r]));
+        SG_HIP(hipEventElapsedTime(&k_ms, ws->ev_start[r], ws->ev_mid[r]));
+        SG_HIP(hipEventElapsedTime(&c_ms, ws->ev_mid[r], ws->ev_burst[r]));
+        SG_HIP(hipEventElapsedTime(&b_ms, ws->ev_burst[r], ws->ev_done[r]));
+      }
+      const int32_t nb = last.last_level - first_level; /* levels of the burst */
+      for (int32_t L = first_level; L <= last.last_level && L < SG_MAX_LEVELS_RECORDED; ++L) {
+        LevelLog e;
+        std::memcpy(&e, (const void*)&ws->levlog[L], sizeof(e));
+        sg_level_stat& ls = stats->level[L];
+        std::memset(&ls, 0, sizeof(ls));
+        ls.direction = e.dir;
+        ls.load_balancer = opt.load_balancer;
+        ls.frontier_vertices = e.f_vertices;
+        ls.frontier_edges = e.f_edges;
+        ls.discovered = e.disc;
+        ls.edges_inspected = e.edges;
+        ls.vertices_scanned = e.scanned;
+        ls.hint_misses = e.misses;
+        ls.max_degree = e.max_deg;
+        ls.unexplored_edges = e.unexplored;
+        ls.action = e.action;
+        ls.burst = e.burst;
+        if (L == first_level) {
+          ls.kernel_ms = k_ms;
+          ls.convert_ms = c_ms;
+        } else {
+          ls.kernel_ms = b_ms / (float)nb;
+        }
       }
     }
-    if (e.done) {
-      levels = L + 1;
-      break;
-    }
-    if (L + 1 >= n + 1) { /* more levels than vertices: the control block is corrupt */
+    if (last.done) break;
+    if ((int64_t)last.last_level + 1 >= n + 1) { /* more levels than vertices: the control block is corrupt */
       SG_HIP(hipStreamSynchronize(s));
-      sg::set_error("sg_bfs: no termination after %lld levels", (long long)(L + 1));
+      sg::set_error("sg_bfs: no termination after %lld levels", (long long)(last.last_level + 1));
       return SG_ERR_STATE;
     }
-    SG_TRY(enqueue(L + 1 + lookahead));
+    first_level = last.last_level + 1;
+    SG_TRY(enqueue(step + 1 + lookahead));
   }
   if (want_total) SG_HIP(hipEventRecord(ws->ev_end, s));
   SG_HIP(hipStreamSynchronize(s));
@@ -650,11 +984,11 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     float ms = 0.f;
     SG_HIP(hipEventElapsedTime(&ms, ws->ev_begin, ws->ev_end));
     stats->kernel_ms = ms;
-    stats->levels = levels;
-    stats->n_push = n_push;
-    stats->n_pull = n_pull;
-    stats->reached_vertices = reached_v;
-    stats->reached_edges = reached_e;
+    stats->levels = last.last_level + 1;
+    stats->n_push = last.n_push;
+    stats->n_pull = last.n_pull;
+    stats->reached_vertices = last.reached_v;
+    stats->reached_edges = last.reached_e;
   }
   return SG_OK;
 }
@@ -691,6 +1025,8 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
                opt.t_medium <= (1 << 30) && opt.t_large <= (1 << 30) && opt.chunk <= (1 << 30),
            SG_ERR_INVALID, "sg_bfs: tuning out of range (t_medium %d, t_large %d, chunk %d, blocks_per_cu %d)",
            opt.t_medium, opt.t_large, opt.chunk, opt.blocks_per_cu);
+  SG_CHECK(opt.burst_arcs <= kBurstMaxArcs, SG_ERR_INVALID, "sg_bfs: burst_arcs %d above %d", opt.burst_arcs,
+           kBurstMaxArcs);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
   if (stats) {

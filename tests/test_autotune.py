@@ -19,7 +19,7 @@ def test_bfs_grid_covers_every_combination():
     from sygraph_amd import autotune
 
     pts = autotune.bfs_points()
-    assert len(pts) == 81 and len({tuple(sorted(p.items())) for p in pts}) == 81
+    assert len(pts) == 243 and len({tuple(sorted(p.items())) for p in pts}) == 243
     assert autotune.bfs_points({"chunk": [64, 128]}) == [{"chunk": 64}, {"chunk": 128}]
 
 

@@ -225,3 +225,50 @@ def test_lookahead_matches_level_by_level(sg, device, oracle_mod, policy):
         runs.append((b.get_distances(), st.levels, st.reached_edges, lv))
     assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1:] == runs[1][1:]
     assert np.array_equal(runs[0][0], oracle_mod.bfs(g, src)[0])
+
+
+def _directed_rmat(oracle_mod, scale, ef, seed):
+    """RMAT arcs kept one-way (no symmetrization): sorted, deduplicated, no loops."""
+    n = 1 << scale
+    u, v = oracle_mod.rmat_pairs(scale, ef, 0.57, 0.19, 0.19, seed)
+    key = np.unique(u.astype(np.int64)[u != v] * n + v.astype(np.int64)[u != v])
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(off, key // n + 1, 1)
+    return oracle_mod.CSRGraph(n, np.cumsum(off), (key % n).astype(np.int32))
+
+
+@pytest.mark.parametrize("graph", ["grid", "path", "rmat", "rmat_directed"])
+@pytest.mark.parametrize("policy", ["push", "hybrid"])
+def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy):
+    """Small push levels run back to back in one single-workgroup launch; every
+    threshold (off, tiny, default, maximal) gives the oracle's depths and the
+    same per-level records as running every level on its own."""
+    g = {"grid": lambda: oracle_mod.grid(120, 90, 0.15, seed=8),
+         "path": lambda: oracle_mod.grid(1, 3000),
+         "rmat": lambda: oracle_mod.rmat(13, 8, seed=19),
+         "rmat_directed": lambda: _directed_rmat(oracle_mod, 12, 8, 21)}[graph]()
+    directed = graph == "rmat_directed"
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, directed=directed)
+    pol = sg.POLICY_PUSH if policy == "push" else sg.POLICY_HYBRID
+    srcs = [int(np.argmax(g.degrees())), int(np.nonzero(g.degrees() > 0)[0][-1])]
+    for src in srcs:
+        ref, levels = oracle_mod.bfs(g, src)
+        runs = {}
+        for burst in (-1, 64, 0, 65536):
+            b = sg.BFS(G)
+            b.init(src)
+            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst})
+            assert np.array_equal(b.get_distances(), ref), (src, burst)
+            assert st.levels == levels
+            lv = [(st.level[i].direction, st.level[i].frontier_vertices, st.level[i].frontier_edges,
+                   st.level[i].discovered, st.level[i].unexplored_edges, st.level[i].max_degree,
+                   st.level[i].action) for i in range(st.levels)]
+            runs[burst] = ((st.n_push, st.n_pull, st.reached_vertices, st.reached_edges), lv,
+                           sum(st.level[i].burst for i in range(st.levels)))
+        assert runs[-1][2] == 0
+        for burst in (64, 0, 65536):
+            assert runs[burst][:2] == runs[-1][:2], (src, burst)
+        if graph in ("grid", "path"):
+            assert runs[0][2] >= levels // 2  # most levels of a high-diameter graph run inside bursts
+    with pytest.raises(sg.SygraphError):
+        b.run(pol, 15.0, 24.0, tuning={"burst_arcs": 65537})
