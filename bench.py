@@ -154,7 +154,8 @@ def secondary(sg, dev):
     graphs generated on the device): CC on the 4096 x 4096 grid with 10 % of
     the edges deleted and on RMAT scale 20 edge factor 32; BFS on that grid
     with and without bursts of small levels; TC on that RMAT; BC from 64
-    sources on RMAT scale 20 edge factor 48, a = 0.55."""
+    sources on RMAT scale 20 edge factor 48, a = 0.55, one source at a time
+    and in a 64-lane batch, and 256 sources in four batches."""
     out = {}
 
     def best(fn, reps=3):
@@ -204,6 +205,13 @@ def secondary(sg, dev):
     bc.init(srcs)
     t = best(bc.run, reps=1)
     out["bc_rmat20_64src_ms"] = round(t * 1e3, 3)
+    bc.init(srcs)
+    t = best(lambda: bc.run(batched=True), reps=2)
+    out["bc_rmat20_64src_batched_ms"] = round(t * 1e3, 3)
+    srcs256 = [int(x) for x in rng.choice(np.nonzero(np.diff(off) > 0)[0], 256, replace=False)]
+    bc.init(srcs256)
+    t = best(lambda: bc.run(batched=True), reps=1)
+    out["bc_rmat20_256src_batched_ms"] = round(t * 1e3, 3)
     G.destroy()
     return out
 

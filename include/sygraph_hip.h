@@ -346,6 +346,13 @@ int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex);
  * each level rescaled by a power of two) and of delta (FP64); the FP64 total
  * is added to bc once.  Sources are processed one after another. */
 int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
+/* Batched BC (no reference counterpart): the sums of sg_bc computed 64 sources
+ * at a time, one bit-lane per source -- one multi-source BFS per batch, then
+ * sigma and delta passes per level with FP64 rows of 64 lanes per vertex.
+ * Equal to sg_bc up to FP64 rounding; added into bc (device float[n]).
+ * Needs one n-word mask per BFS level (SG_ERR_UNSUPPORTED beyond 16 GiB of
+ * masks: long-diameter graphs use sg_bc). */
+int sg_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
 /* Multi-source BFS: depth is device int32[nsrc][n] (row i = sources_host[i]),
  * unreached = n+1.  Sources run 64 per batch, one bit-lane each: every vertex
  * holds 64-bit visited / frontier / next masks, so one pass over the arcs

@@ -163,6 +163,7 @@ _SIGS = {
     "sg_cc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
     "sg_tc": (_I, [_P, C.POINTER(C.c_uint64), _P]),
     "sg_bc": (_I, [_P, C.c_int32, _P, _P]),
+    "sg_bc_batched": (_I, [_P, C.c_int32, _P, _P]),
     "sg_bfs_multi": (_I, [_P, C.c_int32, _P, _P]),
 }
 
@@ -660,11 +661,14 @@ class BC:
         self.values = DeviceBuffer(self.g.dev, 4 * max(self.g.n, 1), dtype=np.float32, shape=(self.g.n,))
         self.values.fill(0, 4, max(self.g.n, 1))
 
-    def run(self):
+    def run(self, batched: bool = False):
+        """batched: 64 sources per bit-lane batch (sg_bc_batched); otherwise
+        one source after another (sg_bc).  Equal up to FP64 rounding."""
         if self.values is None:
             raise SygraphError(SG_ERR_STATE, "BC instance not initialized")
         s = np.ascontiguousarray(np.asarray(self.sources, dtype=np.int32))
-        check(lib().sg_bc(self.g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(self.values.ptr)))
+        fn = lib().sg_bc_batched if batched else lib().sg_bc
+        check(fn(self.g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(self.values.ptr)))
 
     def get_values(self) -> np.ndarray:
         return self.values.download(np.float32, self.g.n)

@@ -28,10 +28,10 @@ def _fixture(oracle_mod, name):
     return oracle_mod.from_dense(oracle_mod.parse_matrix_text(_cases()["matrices"][name]))
 
 
-def _bc(sg, G, sources):
+def _bc(sg, G, sources, batched=False):
     b = sg.BC(G)
     b.init(sources)
-    b.run()
+    b.run(batched=batched)
     return b.get_values()
 
 
@@ -40,19 +40,21 @@ def _close(got, ref):
     return np.all(np.abs(got - ref) <= RTOL * np.abs(ref) + 1e-6)
 
 
-def test_bc_line5_known_answer(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_line5_known_answer(sg, device, oracle_mod, batched):
     g = _fixture(oracle_mod, "line_5")
     G = sg.Graph.from_csr(device, g.offsets, g.cols)
-    assert _bc(sg, G, [0]).tolist() == _cases()["expect"]["bc_line_5_source0"]
+    assert _bc(sg, G, [0], batched).tolist() == _cases()["expect"]["bc_line_5_source0"]
 
 
-def test_bc_star_closed_forms(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_star_closed_forms(sg, device, oracle_mod, batched):
     n = 9
     adj = np.zeros((n, n), dtype=np.int64)
     adj[0, 1:] = adj[1:, 0] = 1
     G = sg.Graph.from_csr(device, *_csr(oracle_mod.from_dense(adj)))
-    assert np.all(_bc(sg, G, [0]) == 0.0)       # from the centre nothing passes through
-    b = _bc(sg, G, [1])                         # from a leaf the centre carries n-2 targets
+    assert np.all(_bc(sg, G, [0], batched) == 0.0)       # from the centre nothing passes through
+    b = _bc(sg, G, [1], batched)                         # from a leaf the centre carries n-2 targets
     assert b[0] == n - 2 and np.all(b[1:] == 0.0)
 
 
@@ -60,31 +62,35 @@ def _csr(g):
     return g.offsets, g.cols
 
 
-def test_bc_rmat14_golden(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_rmat14_golden(sg, device, oracle_mod, batched):
     z = np.load(os.path.join(GOLDEN, "rmat14_seed1.npz"))
     G = sg.Graph.rmat(device, 14, 16, seed=1)
-    assert _close(_bc(sg, G, z["bc_sources"]), z["bc"])
+    assert _close(_bc(sg, G, z["bc_sources"], batched), z["bc"])
 
 
-def test_bc_matches_oracle_shapes(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_matches_oracle_shapes(sg, device, oracle_mod, batched):
     rng = np.random.default_rng(11)
     for g in [oracle_mod.rmat(12, 16, seed=9), oracle_mod.grid(60, 45, 0.15, seed=8),
               oracle_mod.erdos_renyi(3000, 6000, seed=7)]:
         srcs = rng.integers(0, g.n, size=5).tolist()
         G = sg.Graph.from_csr(device, g.offsets, g.cols)
-        assert _close(_bc(sg, G, srcs), oracle_mod.bc(g, srcs))
+        assert _close(_bc(sg, G, srcs, batched), oracle_mod.bc(g, srcs))
 
 
-def test_bc_long_diameter_grid_stays_finite(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_long_diameter_grid_stays_finite(sg, device, oracle_mod, batched):
     # ~1e116 shortest paths to the far corner: sigma needs the per-level scaling
     g = oracle_mod.grid(200, 200)
     G = sg.Graph.from_csr(device, g.offsets, g.cols)
-    b = _bc(sg, G, [0])
+    b = _bc(sg, G, [0], batched)
     assert np.all(np.isfinite(b))
     assert _close(b, oracle_mod.bc(g, [0]))
 
 
-def test_bc_directed(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_directed(sg, device, oracle_mod, batched):
     rng = np.random.default_rng(5)
     n, m = 2000, 9000
     u, v = rng.integers(0, n, m), rng.integers(0, n, m)
@@ -95,27 +101,49 @@ def test_bc_directed(sg, device, oracle_mod):
     g = oracle_mod.CSRGraph(n, off, np.ascontiguousarray(pairs[:, 1], dtype=np.int32))
     G = sg.Graph.from_csr(device, g.offsets, g.cols, directed=True)
     srcs = [0, 3, 99, 1234]
-    assert _close(_bc(sg, G, srcs), oracle_mod.bc(g, srcs, inverse=oracle_mod.transpose(g)))
+    assert _close(_bc(sg, G, srcs, batched), oracle_mod.bc(g, srcs, inverse=oracle_mod.transpose(g)))
 
 
-def test_bc_accumulates_and_isolated_source(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_accumulates_and_isolated_source(sg, device, oracle_mod, batched):
     g = oracle_mod.grid(30, 30, 0.3, seed=2)
     G = sg.Graph.from_csr(device, g.offsets, g.cols)
     iso = int(np.flatnonzero(np.diff(g.offsets) == 0)[0])
     b = sg.BC(G)
     b.init([5, iso])
-    b.run()
-    b.run()  # a second run adds the same dependencies again
+    b.run(batched=batched)
+    b.run(batched=batched)  # a second run adds the same dependencies again
     assert _close(b.get_values(), 2.0 * oracle_mod.bc(g, [5, iso]))
 
 
-def test_bc_invalid_source(sg, device, oracle_mod):
+@pytest.mark.parametrize("batched", [False, True])
+def test_bc_invalid_source(sg, device, oracle_mod, batched):
     g = oracle_mod.rmat(8, 4, seed=1)
     G = sg.Graph.from_csr(device, g.offsets, g.cols)
     b = sg.BC(G)
     b.init([g.n])
     with pytest.raises(sg.SygraphError):
-        b.run()
+        b.run(batched=batched)
+
+
+def test_bc_batched_many_sources_and_hubs(sg, device, oracle_mod):
+    """Two lane batches (100 sources, repeats included), vertices whose lists
+    span several work chunks (degrees above 256) and a star centre of degree
+    599: the batched sums equal the per-source ones up to FP64 rounding."""
+    rng = np.random.default_rng(23)
+    g = oracle_mod.rmat(12, 16, seed=31)
+    assert g.degrees().max() > 3 * 256
+    srcs = rng.integers(0, g.n, size=98).tolist() + [int(np.argmax(g.degrees()))] * 2
+    G = sg.Graph.from_csr(device, g.offsets, g.cols)
+    ref = oracle_mod.bc(g, srcs)
+    batched, single = _bc(sg, G, srcs, True), _bc(sg, G, srcs, False)
+    assert _close(batched, ref) and _close(single, ref)
+    n = 600
+    adj = np.zeros((n, n), dtype=np.int64)
+    adj[0, 1:] = adj[1:, 0] = 1
+    G = sg.Graph.from_csr(device, *_csr(oracle_mod.from_dense(adj)))
+    b = _bc(sg, G, list(range(1, 71)), True)
+    assert b[0] == 70 * (n - 2) and np.all(b[1:] == 0.0)
 
 
 def test_bfs_multi_matches_oracle(sg, device, oracle_mod):
