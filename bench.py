@@ -45,7 +45,7 @@ def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
 def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, learned=None):
     b = sg.BFS(G)
     edges_total = 0
-    per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0]}  # dir -> [bytes, ms, launches]
+    per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0], "burst": [0.0, 0.0, 0]}  # -> [bytes, ms, launches]
     w0_bytes = ((G.n + 63) // 64) * 8
     kernel_ms_total = 0.0
     for _ in range(warmup):
@@ -63,7 +63,8 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
             if collect:
                 for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
                     ls = st.level[L]
-                    k = per_kernel[ls.direction]
+                    # levels run inside a burst (burst_kernel) are not level_kernel launches
+                    k = per_kernel["burst" if ls.burst else ls.direction]
                     k[0] += level_bytes(ls, w0_bytes)
                     k[1] += ls.kernel_ms
                     k[2] += 1
@@ -309,8 +310,9 @@ def main():
     gteps_kernel = edges / (kms / 1e3) / 1e9 if kms > 0 else None
     log(f"{a.policy}: {gteps:.2f} GTEPS wall, {gteps_kernel:.2f} GTEPS device, {wall / a.steps * 1e3:.2f} ms/step")
 
-    # the traversal of every level is one kernel (level_kernel, push or pull
-    # body chosen on the device): the roofline covers all its launches
+    # the traversal of every level outside a burst is one kernel
+    # (level_kernel, push or pull body chosen on the device): the roofline
+    # covers all its launches
     byts = per_kernel[0][0] + per_kernel[1][0]
     ms = per_kernel[0][1] + per_kernel[1][1]
     launches = per_kernel[0][2] + per_kernel[1][2]
@@ -320,10 +322,11 @@ def main():
             "kernel": "level_kernel",
             "launches": launches, "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
             "alg_bytes_per_launch": round(byts / max(launches, 1))}
-    for k, name in {0: "push levels", 1: "pull levels"}.items():
+    for k, name in {0: "level_kernel, push levels", 1: "level_kernel, pull levels",
+                    "burst": "burst_kernel, levels inside bursts"}.items():
         if per_kernel[k][2]:
             b_, m_, n_ = per_kernel[k]
-            log(f"  level_kernel, {name}: {n_} launches, avg {m_ / n_ * 1e3:.1f} us, "
+            log(f"  {name}: {n_} levels, avg {m_ / n_ * 1e3:.1f} us, "
                 f"{b_ / (m_ / 1e3) / 1e9:.0f} GB/s algorithmic")
 
     extra = {}
