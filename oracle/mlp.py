@@ -81,7 +81,8 @@ def adam_step(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
 
 
 def features(frontier_vertices, frontier_edges, unexplored, level, max_degree, n, m) -> np.ndarray:
-    """The eight level features of sg::policy::features (policy.hpp)."""
+    """The eight level features of sg::policy::feature (policy.hpp), in FP64 (the device
+    evaluates the same formulas in FP32)."""
     fv = np.asarray(frontier_vertices, dtype=np.float64)
     fe = np.asarray(frontier_edges, dtype=np.float64)
     f = np.stack([

@@ -242,7 +242,7 @@ struct DecideParams {
   unsigned long long total_degree;
   int32_t lb_fixed;     /* variant under the push / pull / hybrid policies */
   const float* mlp;     /* learned policy parameters (SG_POLICY_LEARNED) */
-  double n_d, m_d;      /* vertex and arc counts for the features */
+  float f6, f7;         /* the graph's constant features (average degree, vertex count) */
   int64_t burst_arcs;   /* frontier arcs up to which push levels run in burst_kernel (0: never) */
 };
 
@@ -268,7 +268,7 @@ __device__ __forceinline__ int32_t decide_action(const DecideParams& p, const sg
                                                  unsigned long long max_deg) {
   if (p.policy == SG_POLICY_LEARNED)
     return sg::policy::infer_staged(pol, (double)disc, (double)disc_deg, (double)unexplored, next_level,
-                                    (double)max_deg, p.n_d, p.m_d);
+                                    (double)max_deg, p.f6, p.f7);
   return next_direction(p, dir, disc, disc_deg, unexplored) * 3 + p.lb_fixed;
 }
 
@@ -881,8 +881,8 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   P.total_degree = (unsigned long long)g->total_degree;
   P.lb_fixed = lb_fixed;
   P.mlp = opt.policy == SG_POLICY_LEARNED ? sg::policy_device_params(opt.learned) : nullptr;
-  P.n_d = (double)n;
-  P.m_d = (double)g->m;
+  P.f6 = sg::policy::feature(6, 0, 0, 0, 0, 0, (double)n, (double)g->m);
+  P.f7 = sg::policy::feature(7, 0, 0, 0, 0, 0, (double)n, (double)g->m);
   P.burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
   if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 

@@ -23,18 +23,21 @@ static_assert(kParams == 1542, "8-32-32-6 MLP");
 
 /* Feature k (0..7) of the frontier entering a level, scaled to about [0, 1]:
  * log2 of frontier vertices, frontier arcs, unexplored arcs and the level,
- * mean and largest frontier degree, average degree and vertex count. */
+ * mean and largest frontier degree, average degree and vertex count.  FP32
+ * (one hardware log2 per feature: a decision sits on the level's critical
+ * path); the training features are the same formulas in FP64. */
 __host__ __device__ inline float feature(int k, double frontier_vertices, double frontier_edges, double unexplored,
                                          int level, double max_degree, double n, double m) {
+  const float fv = (float)frontier_vertices, fe = (float)frontier_edges;
   switch (k) {
-  case 0: return (float)(log2(1.0 + frontier_vertices) / 24.0);
-  case 1: return (float)(log2(1.0 + frontier_edges) / 32.0);
-  case 2: return (float)(log2(1.0 + unexplored) / 32.0);
-  case 3: return (float)(log2(1.0 + (double)level) / 12.0);
-  case 4: return (float)(log2(1.0 + frontier_edges / (frontier_vertices > 1.0 ? frontier_vertices : 1.0)) / 16.0);
-  case 5: return (float)(log2(1.0 + max_degree) / 24.0);
-  case 6: return (float)(log2(1.0 + m / (n > 1.0 ? n : 1.0)) / 8.0);
-  default: return (float)(log2(n > 1.0 ? n : 1.0) / 32.0);
+  case 0: return log2f(1.0f + fv) / 24.f;
+  case 1: return log2f(1.0f + fe) / 32.f;
+  case 2: return log2f(1.0f + (float)unexplored) / 32.f;
+  case 3: return log2f(1.0f + (float)level) / 12.f;
+  case 4: return log2f(1.0f + fe / (fv > 1.f ? fv : 1.f)) / 16.f;
+  case 5: return log2f(1.0f + (float)max_degree) / 24.f;
+  case 6: return log2f(1.0f + (float)(m / (n > 1.0 ? n : 1.0))) / 8.f;
+  default: return log2f((float)(n > 1.0 ? n : 1.0)) / 32.f;
   }
 }
 
@@ -70,19 +73,36 @@ __device__ inline void stage(Staged* s, const float* __restrict__ P, int tid, in
   }
 }
 
+/* Value of lane k (a constant) in every lane: v_readlane, a few cycles,
+ * against tens for an LDS permute. */
+__device__ __forceinline__ float lane_value(float x, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
+/* Feature of lane k & 7 without divergence: one log2 per lane.  f6 and f7
+ * (average degree and vertex count) are constant for a graph and come from
+ * feature() on the host. */
+__device__ __forceinline__ float feature_of_lane(int k, float fv, float fe, float un, float lvl, float md, float f6,
+                                                 float f7) {
+  const float x = k == 0 ? fv : (k == 1 ? fe : (k == 2 ? un : (k == 3 ? lvl : (k == 4 ? fe / (fv > 1.f ? fv : 1.f) : md))));
+  const float sc = (k == 0 || k == 5) ? 24.f : (k == 3 ? 12.f : (k == 4 ? 16.f : 32.f));
+  const float r = log2f(1.0f + x) / sc;
+  return k == 6 ? f6 : (k == 7 ? f7 : r);
+}
+
 /* Action = direction * 3 + load-balancer variant from staged parameters; all
  * 64 lanes of one wavefront call with the same level statistics.  Lane k < 8
  * evaluates feature k, lane j < 32 hidden unit j, lane o < 6 output o; every
  * unit accumulates its bias first and then its inputs in order.  Ties go to
  * the lowest action. */
 __device__ inline int infer_staged(const Staged* __restrict__ s, double frontier_vertices, double frontier_edges,
-                                   double unexplored, int level, double max_degree, double n, double m) {
+                                   double unexplored, int level, double max_degree, float f6, float f7) {
   const int lane = (int)__lane_id();
-  const float mine =
-      feature(lane & 7, frontier_vertices, frontier_edges, unexplored, level, max_degree, n, m);
+  const float mine = feature_of_lane(lane & 7, (float)frontier_vertices, (float)frontier_edges, (float)unexplored,
+                                     (float)level, (float)max_degree, f6, f7);
   float f[kIn];
 #pragma unroll
-  for (int k = 0; k < kIn; ++k) f[k] = __shfl(mine, k, 64);
+  for (int k = 0; k < kIn; ++k) f[k] = lane_value(mine, k);
   const int j = lane & (kH - 1);
   float a = s->b1[j];
 #pragma unroll
@@ -90,17 +110,17 @@ __device__ inline int infer_staged(const Staged* __restrict__ s, double frontier
   const float h1 = a > 0.f ? a : 0.f;
   float a2 = s->b2[j];
 #pragma unroll
-  for (int k = 0; k < kH; ++k) a2 += s->w2t[k * kH + j] * __shfl(h1, k, 64);
+  for (int k = 0; k < kH; ++k) a2 += s->w2t[k * kH + j] * lane_value(h1, k);
   const float h2 = a2 > 0.f ? a2 : 0.f;
   const int o = lane & 7;
   float sc = s->b3[o];
 #pragma unroll
-  for (int k = 0; k < kH; ++k) sc += s->w3t[k * 8 + o] * __shfl(h2, k, 64);
+  for (int k = 0; k < kH; ++k) sc += s->w3t[k * 8 + o] * lane_value(h2, k);
   int best = 0;
-  float bs = __shfl(sc, 0, 64);
+  float bs = lane_value(sc, 0);
 #pragma unroll
   for (int i = 1; i < kOut; ++i) {
-    const float si = __shfl(sc, i, 64);
+    const float si = lane_value(sc, i);
     if (si > bs) {
       bs = si;
       best = i;
