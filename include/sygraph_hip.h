@@ -317,7 +317,8 @@ typedef struct sg_sssp_stats {
                                 bucket's heavy round) */
   float kernel_ms;           /* device time of the whole run */
   int64_t round_vertices[SG_MAX_LEVELS_RECORDED]; /* |F| per round */
-  int64_t round_edges[SG_MAX_LEVELS_RECORDED];    /* sum deg over F per round */
+  int64_t round_edges[SG_MAX_LEVELS_RECORDED];    /* arcs relaxed per round: sum deg over F (Bellman-Ford),
+                                                     or over F of the light / heavy arcs (buckets) */
 } sg_sssp_stats;
 /* SSSP::run (algorithms/sssp.hpp:152-210): frontier Bellman-Ford with atomic
  * min; dist (device, 4-byte, same type as the weights) unreached = n+1.

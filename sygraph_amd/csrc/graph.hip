@@ -340,6 +340,13 @@ void free_graph_arrays(sg_graph_t g) {
   (void)hipFree(g->isolated);
   (void)hipFree(g->pull_hint);
   g->pull_hint = nullptr;
+  (void)hipFree(g->sssp_cols);
+  (void)hipFree(g->sssp_w);
+  (void)hipFree(g->sssp_light);
+  g->sssp_cols = nullptr;
+  g->sssp_w = nullptr;
+  g->sssp_light = nullptr;
+  g->sssp_split_valid = false;
   g->offsets = g->cols = nullptr;
   g->weights = g->inv_offsets = g->inv_weights = g->isolated = nullptr;
   g->inv_cols = nullptr;

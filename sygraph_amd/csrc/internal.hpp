@@ -109,6 +109,13 @@ struct sg_graph_s {
   int2* pull_hint = nullptr;     /* per vertex: {in-neighbour of largest out-degree (pull probes it first),
                                     out-degree of the vertex} */
   double mean_weight = -1.0;     /* mean arc weight, computed on first use (automatic SSSP delta) */
+  /* SSSP's light-first copy of the adjacency for one bucket width: per vertex
+   * the arcs lighter than the width first, then the others */
+  int32_t* sssp_cols = nullptr;
+  void* sssp_w = nullptr;
+  int32_t* sssp_light = nullptr; /* light arcs per vertex */
+  uint32_t sssp_split_bits = 0;  /* the width's bit pattern */
+  bool sssp_split_valid = false;
   int64_t total_degree = 0;
   void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */
   void* sssp_ws = nullptr;

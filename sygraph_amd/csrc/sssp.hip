@@ -12,10 +12,13 @@
  * set) and defers the rest, raising T to (smallest deferred distance) + delta
  * when the near set runs dry -- the bucket order of delta-stepping, which
  * relaxes far fewer arcs on weighted graphs.  delta = infinity is exactly the
- * reference's schedule.  Each round reads its frontier through the
- * degree-class queues (thread / wave / workgroup bins), relaxes with a
- * plain-load pre-check before the atomic, and marks improved vertices dirty
- * with a test-then-OR.  Shortest distances are unique, so the result does not
+ * reference's schedule.  Within a bucket, light iterations relax only the
+ * arcs lighter than delta and one heavy iteration the others; the adjacency is
+ * read from a light-first copy (per vertex the light arcs, then the heavy
+ * ones) so each phase reads only its own arcs.  Each round reads its frontier
+ * through the degree-class queues (thread / wave / workgroup bins), relaxes
+ * with a plain-load pre-check before the atomic, and marks improved vertices
+ * dirty with a test-then-OR.  Shortest distances are unique, so the result does not
  * depend on delta or on the relaxation order.
  *
  * Distances have the weights' 4-byte type: uint32 weights use integer min,
@@ -85,19 +88,18 @@ constexpr int kRing = 8;
 /* Light iterations relax the arcs lighter than delta, the heavy iteration of
  * a bucket the others (delta-stepping's light / heavy split: a heavy arc
  * from a settled bucket cannot land inside it, so it is relaxed once).  The
- * weight is read first; the random distance access only for the arcs of the
- * phase. */
+ * adjacency is the light-first copy, and the queue entries already cover only
+ * the phase's part of each list, so every visited arc is relaxed. */
 template<typename OffT, typename W>
-__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl, W dlt,
+__global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl,
                                                        unsigned long long* relax_slots) {
   if (ctl->done || !ctl->relax) return;
-  const bool heavy = ctl->heavy != 0;
   unsigned long long cnt = 0;
   for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
     const W wt = a.w[e];
-    if ((wt >= dlt) != heavy) return;
     ++cnt;
-    const W du = DistOps<W>::value(atomic_load(&a.dist[u]));
+    /* a stale dist[u] is an upper bound: u is dirty again if it improved */
+    const W du = DistOps<W>::value(a.dist[u]);
     const W nd = du + wt;
     const uint32_t nb = DistOps<W>::bits(nd);
     if (nb < a.dist[v]) {
@@ -117,6 +119,47 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, cons
     unsigned long long t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += red[w];
     if (t) atomic_add(&relax_slots[(blockIdx.x % kSlots) * 16], t);
+  }
+}
+
+/* The light-first adjacency: a wave per vertex writes the arcs lighter than
+ * dlt from the front of the vertex's range (in order) and the others from the
+ * back, and records the light count. */
+template<typename OffT, typename W>
+__global__ void __launch_bounds__(kBlock) split_build_kernel(const OffT* off, const int32_t* cols, const W* w,
+                                                             int64_t n, W dlt, int32_t* scols, W* sw,
+                                                             int32_t* light) {
+  const int lane = lane_id();
+  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (int64_t v = gwave; v < n; v += nwaves) {
+    const int64_t b = (int64_t)off[v], e = (int64_t)off[v + 1];
+    int64_t front = b, back = e;
+    for (int64_t j0 = b; j0 < e; j0 += 64) {
+      const int64_t j = j0 + lane;
+      const bool ok = j < e;
+      int32_t c = 0;
+      W x = 0;
+      if (ok) {
+        c = cols[j];
+        x = w[j];
+      }
+      const bool lt = ok && x < dlt, hv = ok && !(x < dlt);
+      const uint64_t ml = ballot(lt), mh = ballot(hv);
+      if (lt) {
+        const int64_t p = front + lanes_below(ml);
+        scols[p] = c;
+        sw[p] = x;
+      }
+      if (hv) {
+        const int64_t p = back - 1 - lanes_below(mh);
+        scols[p] = c;
+        sw[p] = x;
+      }
+      front += __popcll(ml);
+      back -= __popcll(mh);
+    }
+    if (lane == 0) light[v] = (int32_t)(front - b);
   }
 }
 
@@ -182,16 +225,18 @@ __global__ void split_slots_reset_kernel(SplitSlot* slots, const SsspCtl* ctl) {
   }
 }
 
+/* entries over the light part (light iterations) or the heavy part (the
+ * heavy iteration, decided before the count and recorded by the finish) */
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) near_count_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
   if (ctl->done) return;
-  compact_count_body<OffT, kFromBitmap>(a);
+  compact_count_body<OffT, kFromBitmap>(a, ctl->heavy_next ? 1 : 0);
 }
 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) near_write_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
   if (ctl->done || !ctl->relax) return;
-  compact_write_body<OffT>(a);
+  compact_write_body<OffT>(a, ctl->heavy ? 1 : 0);
 }
 
 /* Threshold after a dry iteration: the smallest deferred distance plus
@@ -379,6 +424,27 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
   return SG_OK;
 }
 
+/* The light-first adjacency for bucket width dlt, built on first use and
+ * kept with the graph until a run asks for another width. */
+template<typename OffT, typename W>
+int ensure_split(sg_graph_t g, W dlt) {
+  const uint32_t bits = DistOps<W>::bits(dlt);
+  if (g->sssp_split_valid && g->sssp_split_bits == bits) return SG_OK;
+  sg_device_t dev = g->dev;
+  if (!g->sssp_cols) {
+    SG_TRY(sg::alloc_device(dev, (void**)&g->sssp_cols, 4 * (size_t)(g->m + 1)));
+    SG_TRY(sg::alloc_device(dev, (void**)&g->sssp_w, 4 * (size_t)(g->m + 1)));
+    SG_TRY(sg::alloc_device(dev, (void**)&g->sssp_light, 4 * (size_t)(g->n + 1)));
+  }
+  split_build_kernel<OffT, W><<<dev->cus * 8, kBlock, 0, dev->stream>>>(
+      (const OffT*)g->offsets, g->cols, (const W*)g->weights, g->n, dlt, g->sssp_cols, (W*)g->sssp_w,
+      g->sssp_light);
+  SG_HIP(hipGetLastError());
+  g->sssp_split_bits = bits;
+  g->sssp_split_valid = true;
+  return SG_OK;
+}
+
 template<typename OffT, typename W>
 int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats* stats) {
   sg_device_t dev = g->dev;
@@ -419,10 +485,14 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
   const int grid_w = sg::grid_for(dev, ws->w0, kBlock, 4);
   /* arcs of weight >= dlt are heavy (none under the reference's schedule) */
   const W dlt = bellman ? (W)w_max : (W)std::min(std::max(delta, DistOps<W>::is_float ? 0.0 : 1.0), w_max);
-  const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, 0, off, nullptr, nullptr, near_bm, tune);
+  /* bucketed runs read the light-first adjacency, the entries covering one
+   * phase's part of each list */
+  if (!bellman) SG_TRY((ensure_split<OffT, W>(g, dlt)));
+  CompactArgs<OffT> ca = make_compact_args<OffT>(Q, 0, off, nullptr, nullptr, near_bm, tune);
+  ca.split = bellman ? nullptr : g->sssp_light;
   SsspArgs<OffT, W> a{};
-  a.cols = g->cols;
-  a.w = (const W*)g->weights;
+  a.cols = bellman ? g->cols : g->sssp_cols;
+  a.w = bellman ? (const W*)g->weights : (const W*)g->sssp_w;
   a.dist = dist;
   a.next_bm = dirty;
   a.w0 = ws->w0;
@@ -437,7 +507,7 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
     sssp_finish_kernel<W><<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr, ws->ctl, ws->slots,
                                                      bellman ? 0.0 : delta, w_max, it, ws->log);
     near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
-    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, dlt, ws->relax_slots);
+    relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
     return SG_OK;

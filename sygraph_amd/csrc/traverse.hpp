@@ -175,12 +175,30 @@ struct CompactArgs {
   QEntry* qout[3];
   QueueHdr* hdr;
   Tuning tune;
+  const int32_t* split; /* optional: per vertex, the length of the first part of its list (SSSP's light
+                           arcs); the entries then cover the first part (phase 0) or the rest (phase 1) */
 };
+
+/* The arc range [s, s + d) of vertex v's queue entries. */
+template<typename OffT>
+__device__ __forceinline__ void entry_range(const CompactArgs<OffT>& a, int64_t v, int phase, int64_t& s, int64_t& d) {
+  s = (int64_t)a.off[v];
+  d = (int64_t)(a.off[v + 1] - a.off[v]);
+  if (a.split) {
+    const int64_t first = a.split[v];
+    if (phase) {
+      s += first;
+      d -= first;
+    } else {
+      d = first;
+    }
+  }
+}
 
 /* pass 1: one wave per 64-word tile; the workgroup also sums its four tiles
  * (all threads of the workgroup must call) */
 template<typename OffT, int kMode>
-__device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a) {
+__device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, int phase = 0) {
   __shared__ TileCount s_wave[kBlock / 64];
   const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; /* tiles >= ntiles see only words >= w0 */
   const int lane = lane_id();
@@ -210,7 +228,8 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a) {
     const uint64_t word = shfl_u64(mynew, k);
     if ((word >> lane) & 1ull) {
       const int64_t v = (tile * 64 + k) * 64 + lane;
-      const int64_t d = (int64_t)(a.off[v + 1] - a.off[v]);
+      int64_t s0, d;
+      entry_range(a, v, phase, s0, d);
       ++nv;
       deg += d;
       mx = (uint64_t)d > mx ? (uint64_t)d : mx;
@@ -346,7 +365,7 @@ static __global__ void __launch_bounds__(kScanThreads) compact_scan_kernel(TileC
 
 /* pass 3: emit the class queues in vertex order at the scanned offsets */
 template<typename OffT>
-__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a) {
+__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, int phase = 0) {
   const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (tile >= a.ntiles) return;
   const int lane = lane_id();
@@ -370,10 +389,7 @@ __device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a) {
     const int64_t v = (tile * 64 + k) * 64 + lane;
     const bool has = (word >> lane) & 1ull;
     int64_t s = 0, d = 0;
-    if (has) {
-      s = (int64_t)a.off[v];
-      d = (int64_t)(a.off[v + 1] - a.off[v]);
-    }
+    if (has) entry_range(a, v, phase, s, d);
     const int cls = (has && d > 0) ? degree_class(a.tune, d) : -1;
     const uint64_t m0 = ballot(cls == 0), m1 = ballot(cls == 1);
     QEntry qe;

@@ -79,6 +79,20 @@ def test_sssp_bucket_width_does_not_change_distances(sg, device, oracle_mod, del
         assert st.relaxations >= st.reached_edges
 
 
+def test_sssp_width_changes_on_one_graph(sg, device, oracle_mod):
+    """The light-first adjacency is rebuilt whenever a run asks for another
+    bucket width; alternating widths on one graph keep every distance exact."""
+    g = oracle_mod.rmat(12, 8, seed=47)
+    oracle_mod.fill_weights(g, 8)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+    s = sg.SSSP(G)
+    ref = oracle_mod.sssp(g, 5)
+    for delta in (7.0, 40.0, 7.0, 0.0, -1.0, 255.0, 1.0, 7.0):
+        s.init(5)
+        s.run(delta=delta)
+        assert np.array_equal(s.get_distances(), ref), delta
+
+
 def test_sssp_buckets_relax_fewer_arcs(sg, device, oracle_mod):
     g = oracle_mod.rmat(13, 16, seed=45)
     oracle_mod.fill_weights(g, 6)
