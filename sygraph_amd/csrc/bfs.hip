@@ -61,18 +61,30 @@ struct BfsArgs {
   StatSlot* stats_zero;
 };
 
-template<typename OffT>
-__device__ __forceinline__ void push_visit(const BfsArgs<OffT>& a, uint32_t y, int32_t level) {
-  if (claim_bit(a.visited, y)) a.depth[y] = level + 1;
-}
-
 /* Push level: every arc of the class queues; discovery is the test-and-set of
  * the visited bit. */
 template<typename OffT>
 __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
   zero_slots(a.stats_zero);
-  for_each_queued_arc<OffT>(
-      a.qin, a.qin_hdr, a.cols, [&](uint32_t, uint32_t y, int64_t) { push_visit(a, y, level); }, variant);
+  /* four arcs per lane: four visited-word loads, then the atomic ORs of the
+   * unset ones, then the depth stores of the claimed ones */
+  for_each_queued_arc4<OffT>(
+      a.qin, a.qin_hdr, a.cols,
+      [&](const uint32_t (&y)[4], const int64_t (&)[4], const bool (&ok)[4]) {
+        uint64_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = ok[k] ? a.visited[y[k] >> 6] : ~0ull;
+        uint64_t old[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint64_t bit = 1ull << (y[k] & 63);
+          old[k] = (w[k] & bit) ? bit : atomic_or(&a.visited[y[k] >> 6], bit);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (!(old[k] & (1ull << (y[k] & 63)))) a.depth[y[k]] = level + 1;
+      },
+      variant);
 }
 
 /* Pull level.  A wave loads 64 visited words at once and compacts their
@@ -487,23 +499,88 @@ struct BurstArgs {
   int32_t* depth;
   QEntry* q[3];    /* queues of the next step (Q[nxt]) */
   QueueHdr* hdr;
-  QEntry* list[2]; /* burst frontier lists, kBurstMaxArcs entries each */
+  QEntry* list[2][2]; /* burst frontier lists [parity][small / big], kBurstMaxArcs entries each */
   Tuning tune;
 };
 
 struct BurstAcc {
-  unsigned long long disc, deg, mx;
+  unsigned long long disc, deg, mx, nsmall, nbig;
 };
 
-/* Visit every arc of `cnt` entries; discoveries are claimed, get depth
- * level + 1 and their prev bit, and are appended to `out`.  Per round a lane
- * takes four arcs: four id loads, then four claims (atomic OR on the visited
- * word) issued together with the eight offset loads a discovery needs, then
- * one LDS counter update per wave for all four, then the stores -- three
- * dependent memory round trips per round. */
+constexpr int kBurstBig = 64;     /* lists this long go to the big list, walked in pieces by every wave */
+constexpr int kBurstPiece = 256;  /* arcs per piece of a big list: four per lane */
+
+/* Claim the (up to) four arcs y[k] of each lane: atomic OR on the visited
+ * word issued together with the two offset loads a discovery needs, one LDS
+ * counter update per wave and list for all four, then the stores.  A
+ * discovery gets depth level + 1 and its prev bit and goes to the small or the
+ * big list of the next level by its degree. */
 template<typename OffT>
-__device__ __forceinline__ void burst_visit(const BurstArgs<OffT>& a, const QEntry* ents, uint32_t cnt, int32_t level,
-                                            QEntry* out, uint32_t* s_count, BurstAcc& acc) {
+__device__ __forceinline__ void burst_claim4(const BurstArgs<OffT>& a, const uint32_t (&y)[4], const bool (&ok)[4],
+                                             int32_t level, QEntry* out_small, QEntry* out_big, uint32_t* cnt2,
+                                             BurstAcc& acc) {
+  const int lane = lane_id();
+  uint64_t old[4];
+  int64_t vs[4], ve[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    old[k] = ~0ull;
+    vs[k] = ve[k] = 0;
+    if (ok[k]) {
+      old[k] = atomic_or(&a.visited[y[k] >> 6], 1ull << (y[k] & 63));
+      vs[k] = (int64_t)a.off[y[k]];
+      ve[k] = (int64_t)a.off[y[k] + 1];
+    }
+  }
+  bool nw[4], big[4];
+  uint64_t ms[4], mb[4];
+  uint32_t cs = 0, cb = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    nw[k] = ((old[k] >> (y[k] & 63)) & 1ull) == 0ull;
+    big[k] = ve[k] - vs[k] >= kBurstBig;
+    ms[k] = ballot(nw[k] && !big[k]);
+    mb[k] = ballot(nw[k] && big[k]);
+    cs += (uint32_t)__popcll(ms[k]);
+    cb += (uint32_t)__popcll(mb[k]);
+  }
+  if ((cs | cb) == 0) return;
+  uint32_t ps = 0, pb = 0;
+  if (lane == 0) {
+    if (cs) ps = atomicAdd(&cnt2[0], cs);
+    if (cb) pb = atomicAdd(&cnt2[1], cb);
+  }
+  ps = __shfl(ps, 0, 64);
+  pb = __shfl(pb, 0, 64);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (nw[k]) {
+      const uint32_t v = y[k];
+      a.depth[v] = level + 1;
+      atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
+      QEntry e;
+      e.start = vs[k];
+      e.v = (int32_t)v;
+      e.len = (int32_t)(ve[k] - vs[k]);
+      if (big[k]) out_big[pb + lanes_below(mb[k])] = e;
+      else out_small[ps + lanes_below(ms[k])] = e;
+      ++acc.disc;
+      acc.deg += (unsigned long long)e.len;
+      acc.mx = (unsigned long long)e.len > acc.mx ? (unsigned long long)e.len : acc.mx;
+    }
+    ps += (uint32_t)__popcll(ms[k]);
+    pb += (uint32_t)__popcll(mb[k]);
+  }
+  acc.nsmall += lane == 0 ? cs : 0;
+  acc.nbig += lane == 0 ? cb : 0;
+}
+
+/* Short lists: a wave lays 64 entries end to end (merge path) and takes four
+ * arcs per lane per round; waves take groups of 64 entries in turn. */
+template<typename OffT>
+__device__ __forceinline__ void burst_visit_small(const BurstArgs<OffT>& a, const QEntry* ents, uint32_t cnt,
+                                                  int32_t level, QEntry* out_small, QEntry* out_big, uint32_t* cnt2,
+                                                  BurstAcc& acc) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (uint32_t base = (uint32_t)wv * 64; base < cnt; base += kBurstThreads) {
     QEntry qe;
@@ -526,49 +603,38 @@ __device__ __forceinline__ void burst_visit(const BurstArgs<OffT>& a, const QEnt
         ok[k] = t < total;
         y[k] = ok[k] ? (uint32_t)a.cols[st + (t - before)] : 0u;
       }
-      uint64_t old[4];
-      int64_t vs[4], ve[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        old[k] = ~0ull;
-        vs[k] = ve[k] = 0;
-        if (ok[k]) {
-          old[k] = atomic_or(&a.visited[y[k] >> 6], 1ull << (y[k] & 63));
-          vs[k] = (int64_t)a.off[y[k]];
-          ve[k] = (int64_t)a.off[y[k] + 1];
-        }
-      }
-      bool nw[4];
-      uint64_t m[4];
-      uint32_t cnt4 = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        nw[k] = ((old[k] >> (y[k] & 63)) & 1ull) == 0ull;
-        m[k] = ballot(nw[k]);
-        cnt4 += (uint32_t)__popcll(m[k]);
-      }
-      if (cnt4 == 0) continue;
-      uint32_t pos = 0;
-      if (lane == 0) pos = atomicAdd(s_count, cnt4);
-      pos = __shfl(pos, 0, 64);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (nw[k]) {
-          const uint32_t v = y[k];
-          a.depth[v] = level + 1;
-          atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
-          QEntry e;
-          e.start = vs[k];
-          e.v = (int32_t)v;
-          e.len = (int32_t)(ve[k] - vs[k]);
-          out[pos + lanes_below(m[k])] = e;
-          ++acc.disc;
-          acc.deg += (unsigned long long)e.len;
-          acc.mx = (unsigned long long)e.len > acc.mx ? (unsigned long long)e.len : acc.mx;
-        }
-        pos += (uint32_t)__popcll(m[k]);
-      }
+      burst_claim4(a, y, ok, level, out_small, out_big, cnt2, acc);
     }
+  }
+}
+
+/* Long lists: every list is cut into pieces of kBurstPiece arcs, and the
+ * pieces of the level go to the waves in turn, so one hub keeps every wave
+ * busy. */
+template<typename OffT>
+__device__ __forceinline__ void burst_visit_big(const BurstArgs<OffT>& a, const QEntry* ents, uint32_t cnt,
+                                                int32_t level, QEntry* out_small, QEntry* out_big, uint32_t* cnt2,
+                                                BurstAcc& acc) {
+  const int lane = lane_id();
+  const uint32_t wv = threadIdx.x >> 6;
+  uint32_t pbase = 0; /* index of the entry's first piece among the level's pieces */
+  for (uint32_t i = 0; i < cnt; ++i) {
+    const QEntry qe = ents[i];
+    const uint32_t np = (uint32_t)((qe.len + kBurstPiece - 1) / kBurstPiece);
+    const uint32_t first = pbase + (wv + kBurstWaves - pbase % kBurstWaves) % kBurstWaves;
+    for (uint32_t pc = first; pc < pbase + np; pc += kBurstWaves) {
+      const int64_t o0 = (int64_t)(pc - pbase) * kBurstPiece;
+      uint32_t y[4];
+      bool ok[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t o = o0 + 64 * k + lane;
+        ok[k] = o < qe.len;
+        y[k] = ok[k] ? (uint32_t)a.cols[qe.start + o] : 0u;
+      }
+      burst_claim4(a, y, ok, level, out_small, out_big, cnt2, acc);
+    }
+    pbase += np;
   }
 }
 
@@ -580,11 +646,11 @@ template<typename OffT>
 __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a, BfsCtl* ctl, DecideParams p,
                                                               int32_t step, LevelLog* levlog, StepLog* steplog) {
   __shared__ sg::policy::Staged s_pol;
-  __shared__ uint32_t s_count[2], s_cls[3];
-  __shared__ unsigned long long s_red[2][kBurstWaves][3];
+  __shared__ uint32_t s_count[2][2], s_cls[3];
+  __shared__ unsigned long long s_red[2][kBurstWaves][5];
   if (ctl->done || !ctl->burst) return;
   stage_policy(p, &s_pol);
-  if (threadIdx.x == 0) s_count[0] = 0;
+  if (threadIdx.x == 0) s_count[0][0] = s_count[0][1] = 0;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   int32_t level = ctl->level, lb = ctl->lb;
   int64_t f_vertices = ctl->f_vertices, f_edges = ctl->f_edges;
@@ -592,34 +658,47 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
   int32_t n_push = ctl->n_push, done = 0, act = 0;
   const QueueHdr h0 = *a.hdr;
   int cur = 0;
+  uint32_t n_small = 0, n_big = 0; /* entries of the lists of the level about to run (after the first) */
   unsigned long long disc = 0, disc_deg = 0, max_deg = 0;
   __syncthreads();
   for (int it = 0;; ++it) {
     const int par = it & 1;
-    BurstAcc acc{0, 0, 0};
-    QEntry* out = a.list[cur ^ 1];
-    if (it == 0) {
-      for (int c = 0; c < 3; ++c) burst_visit(a, a.q[c], h0.count[c], level, out, &s_count[par], acc);
+    BurstAcc acc{0, 0, 0, 0, 0};
+    QEntry* out_small = a.list[cur ^ 1][0];
+    QEntry* out_big = a.list[cur ^ 1][1];
+    if (it == 0) { /* the queues: class 0 lists are short, classes 1 and 2 long */
+      burst_visit_small(a, a.q[0], h0.count[0], level, out_small, out_big, s_count[par], acc);
+      burst_visit_big(a, a.q[1], h0.count[1], level, out_small, out_big, s_count[par], acc);
+      burst_visit_big(a, a.q[2], h0.count[2], level, out_small, out_big, s_count[par], acc);
     } else {
-      burst_visit(a, a.list[cur], (uint32_t)f_vertices, level, out, &s_count[par], acc);
+      burst_visit_small(a, a.list[cur][0], n_small, level, out_small, out_big, s_count[par], acc);
+      burst_visit_big(a, a.list[cur][1], n_big, level, out_small, out_big, s_count[par], acc);
     }
     acc.disc = wave_sum_u64(acc.disc);
     acc.deg = wave_sum_u64(acc.deg);
     acc.mx = wave_max_u64(acc.mx);
+    acc.nsmall = wave_sum_u64(acc.nsmall);
+    acc.nbig = wave_sum_u64(acc.nbig);
     if (lane == 0) {
       s_red[par][wv][0] = acc.disc;
       s_red[par][wv][1] = acc.deg;
       s_red[par][wv][2] = acc.mx;
+      s_red[par][wv][3] = acc.nsmall;
+      s_red[par][wv][4] = acc.nbig;
     }
-    if (threadIdx.x == 0) s_count[par ^ 1] = 0; /* last used two levels ago */
-    __syncthreads(); /* discoveries, the list `out` and the partial sums are complete */
+    if (threadIdx.x == 0) s_count[par ^ 1][0] = s_count[par ^ 1][1] = 0; /* last used two levels ago */
+    __syncthreads(); /* discoveries, the lists and the partial sums are complete */
     disc = 0;
     disc_deg = 0;
     max_deg = 0;
+    n_small = 0;
+    n_big = 0;
     for (int w = 0; w < kBurstWaves; ++w) {
       disc += s_red[par][w][0];
       disc_deg += s_red[par][w][1];
       max_deg = s_red[par][w][2] > max_deg ? s_red[par][w][2] : max_deg;
+      n_small += (uint32_t)s_red[par][w][3];
+      n_big += (uint32_t)s_red[par][w][4];
     }
     const unsigned long long unexplored = p.total_degree - (vdeg + disc_deg);
     act = decide_action(p, &s_pol, 0, disc, disc_deg, unexplored, level + 1, max_deg);
@@ -654,13 +733,12 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
     const bool more = !done && act / 3 == 0 && (int64_t)disc_deg <= p.burst_arcs;
     if (!more) break;
   }
-  /* hand the last discovered set (list[cur]) back to the level pipeline */
-  const QEntry* last = a.list[cur];
+  /* hand the last discovered set (the lists list[cur]) back to the level pipeline */
   if (threadIdx.x < 3) s_cls[threadIdx.x] = 0;
   __syncthreads();
   if (!done) {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)f_vertices; i += kBurstThreads) {
-      const QEntry e = last[i];
+    for (uint32_t i = threadIdx.x; i < n_small + n_big; i += kBurstThreads) {
+      const QEntry e = i < n_small ? a.list[cur][0][i] : a.list[cur][1][i - n_small];
       atomic_or(&a.fbm[(uint32_t)e.v >> 6], 1ull << ((uint32_t)e.v & 63));
       if (e.len <= 0) continue;
       const int cls = degree_class(a.tune, e.len);
@@ -736,7 +814,7 @@ struct BfsWorkspace {
   QueueSet qs;
   StatSlot* stats = nullptr; /* 2 areas of kSlots */
   BfsCtl* ctl = nullptr;
-  QEntry* burst_list[2] = {nullptr, nullptr}; /* kBurstMaxArcs entries each */
+  QEntry* burst_list[4] = {nullptr, nullptr, nullptr, nullptr}; /* kBurstMaxArcs entries each */
   LevelLog* levlog = nullptr; /* SG_MAX_LEVELS_RECORDED entries, pinned coherent host memory */
   StepLog* steplog = nullptr; /* kRing entries, pinned coherent host memory */
   hipEvent_t ev_start[kRing] = {}, ev_mid[kRing] = {}, ev_burst[kRing] = {}, ev_done[kRing] = {};
@@ -752,8 +830,7 @@ void ws_free(BfsWorkspace* w) {
   w->qs.release();
   (void)hipFree(w->stats);
   (void)hipFree(w->ctl);
-  (void)hipFree(w->burst_list[0]);
-  (void)hipFree(w->burst_list[1]);
+  for (QEntry* l : w->burst_list) (void)hipFree(l);
   if (w->levlog) (void)hipHostFree(w->levlog);
   if (w->steplog) (void)hipHostFree(w->steplog);
   for (int r = 0; r < kRing; ++r)
@@ -781,6 +858,8 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
       (st = sg::alloc_device(dev, (void**)&w->ctl, sizeof(BfsCtl))) ||
       (st = sg::alloc_device(dev, (void**)&w->burst_list[0], sizeof(QEntry) * kBurstMaxArcs)) ||
       (st = sg::alloc_device(dev, (void**)&w->burst_list[1], sizeof(QEntry) * kBurstMaxArcs)) ||
+      (st = sg::alloc_device(dev, (void**)&w->burst_list[2], sizeof(QEntry) * kBurstMaxArcs)) ||
+      (st = sg::alloc_device(dev, (void**)&w->burst_list[3], sizeof(QEntry) * kBurstMaxArcs)) ||
       (st = w->qs.init(dev, g->n, g->m, default_tuning()))) {
     ws_free(w);
     return st;
@@ -905,8 +984,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     ba.depth = depth;
     for (int k = 0; k < 3; ++k) ba.q[k] = Q.q[nxt][k];
     ba.hdr = Q.hdr + nxt;
-    ba.list[0] = ws->burst_list[0];
-    ba.list[1] = ws->burst_list[1];
+    for (int k = 0; k < 4; ++k) ba.list[k >> 1][k & 1] = ws->burst_list[k];
     ba.tune = tune;
     if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);

@@ -465,6 +465,72 @@ __device__ __forceinline__ void enqueue_single(const OffT* off, const Tuning& t,
  *      64 per step) -- the bucketing balancer;
  *   1  classes 0 and 1 both through the merge path (edge-balanced);
  *   2  class 2 chunks per wave instead of per workgroup. */
+/* for_each_queued_arc with four arcs per lane per call: visit4(v, e, ok)
+ * receives four arc heads, their arc indices and validity, so the visitor can
+ * issue the four arcs' memory accesses together (memory-level parallelism
+ * for latency-bound levels).  Same work mappings as for_each_queued_arc. */
+template<typename OffT, typename Visit4>
+__device__ __forceinline__ void for_each_queued_arc4(QEntry* const q[3], const QueueHdr* hdr, const int32_t* cols,
+                                                     Visit4&& visit4, int variant = 0) {
+  const uint32_t n_small = hdr->count[0], n_med = hdr->count[1], n_large = hdr->count[2];
+  const int lane = lane_id();
+  const int wave_in_block = threadIdx.x >> 6;
+  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  uint32_t v[4];
+  int64_t e[4];
+  bool ok[4];
+  /* one list [start, start + len) walked by `width` threads from position `first` */
+  auto strided = [&](const QEntry& qe, int64_t first, int64_t width) {
+    for (int64_t j0 = first; j0 < qe.len; j0 += 4 * width) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t j = j0 + k * width;
+        ok[k] = j < qe.len;
+        e[k] = qe.start + j;
+        v[k] = ok[k] ? (uint32_t)cols[e[k]] : 0u;
+      }
+      visit4(v, e, ok);
+    }
+  };
+  if (variant == 2) {
+    for (int64_t i = gwave; i < n_large; i += nwaves) strided(q[2][i], lane, 64);
+  } else {
+    for (uint32_t i = blockIdx.x; i < n_large; i += gridDim.x) strided(q[2][i], wave_in_block * 64 + lane, kBlock);
+  }
+  if (variant != 1) {
+    for (int64_t i = gwave; i < n_med; i += nwaves) strided(q[1][i], lane, 64);
+  }
+  for (int cls = 0; cls < (variant == 1 ? 2 : 1); ++cls) {
+    const QEntry* qq = q[cls];
+    const uint32_t cnt = cls == 0 ? n_small : n_med;
+    for (int64_t base = gwave * 64; base < cnt; base += nwaves * 64) {
+      const int64_t i = base + lane;
+      QEntry qe;
+      qe.start = 0;
+      qe.v = 0;
+      qe.len = 0;
+      if (i < cnt) qe = qq[i];
+      const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
+      const uint32_t total = __shfl(incl, 63, 64);
+      const uint32_t excl = incl - (uint32_t)qe.len;
+      for (uint32_t t0 = 0; t0 < total; t0 += 4 * 64) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t t = t0 + 64 * k + lane;
+          const int owner = scan_owner(incl, t);
+          const uint32_t before = __shfl(excl, owner, 64);
+          const int64_t st = shfl_i64(qe.start, owner);
+          ok[k] = t < total;
+          e[k] = st + (t - before);
+          v[k] = ok[k] ? (uint32_t)cols[e[k]] : 0u;
+        }
+        visit4(v, e, ok);
+      }
+    }
+  }
+}
+
 template<typename OffT, typename Visit>
 __device__ __forceinline__ void for_each_queued_arc(QEntry* const q[3], const QueueHdr* hdr, const int32_t* cols,
                                                     Visit&& visit, int variant = 0) {
