@@ -10,10 +10,11 @@
  * out-degrees are O(sqrt(m)).
  *   1. orient: one wave per vertex keeps, in ascending id order, the
  *      neighbours that rank higher (ballot + prefix to preserve order);
- *   2. intersect: one wave per vertex u stages N+(u) in LDS, then for each
- *      v in N+(u) the lanes take 64 ids of N+(v) at a time and binary-search
- *      them in the staged list (segmented intersection, kernel (e)).  Lists
- *      longer than the LDS stage are searched in global memory.
+ *   2. intersect: one wave per vertex u puts N+(u) in an LDS hash table,
+ *      then walks the lists N+(v), v in N+(u), 64 lists laid end to end at a
+ *      time, a lane per arc (v, w) probing w in the table (segmented
+ *      intersection).  Lists longer than the LDS stage are binary-searched in
+ *      global memory, where they are sorted.
  * The count is uint64 (the reference's uint32 would overflow at scale 20).
  */
 #include <rocprim/device/device_scan.hpp>
@@ -26,7 +27,8 @@ using namespace sg::trav;
 
 namespace {
 
-constexpr int kStage = 2048; /* ids of N+(u) staged per wave */
+constexpr int kStage = 1024; /* longest N+(u) hashed in LDS (table of 2 * kStage slots per wave) */
+constexpr int kTcUnroll = 8;  /* arcs (v, w) per lane in flight */
 
 template<typename OffT>
 __device__ __forceinline__ bool ranks_higher(const OffT* off, uint32_t u, int64_t du, uint32_t v) {
@@ -74,12 +76,31 @@ __device__ __forceinline__ bool search(const int32_t* list, int32_t len, int32_t
   return false;
 }
 
+__device__ __forceinline__ uint32_t hash_slot(int32_t x, uint32_t mask) {
+  return ((uint32_t)x * 2654435761u >> 7) & mask; /* Knuth's multiplicative hash */
+}
+
+/* Open-addressing membership in the wave's LDS table (empty slots hold -1). */
+__device__ __forceinline__ bool hash_contains(const int32_t* table, uint32_t mask, int32_t x) {
+  for (uint32_t h = hash_slot(x, mask);; h = (h + 1) & mask) {
+    const int32_t y = table[h];
+    if (y == x) return true;
+    if (y < 0) return false;
+  }
+}
+
+/* One wave per vertex u.  N+(u) goes into an LDS hash table of at least
+ * twice its size (lists up to kStage ids; longer ones are binary-searched in
+ * global memory, where they are sorted).  The wave then walks the lists
+ * N+(v), v in N+(u), 64 of them laid end to end at a time (merge path), a lane
+ * per arc (v, w): w in N+(u) closes the triangle (u, v, w), counted once at
+ * its lowest-ranked corner. */
 __global__ void __launch_bounds__(kBlock) intersect_kernel(const int64_t* dag_off, const int32_t* dag_cols, int64_t n,
                                                            unsigned long long* per_vertex, unsigned long long* total) {
-  __shared__ int32_t stage[kBlock / 64][kStage];
+  __shared__ int32_t table_all[kBlock / 64][2 * kStage];
   const int lane = lane_id();
   const int wv = threadIdx.x >> 6;
-  int32_t* my = stage[wv];
+  int32_t* table = table_all[wv];
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
   uint64_t acc_total = 0;
@@ -90,23 +111,59 @@ __global__ void __launch_bounds__(kBlock) intersect_kernel(const int64_t* dag_of
       if (per_vertex && lane == 0) per_vertex[u] = 0;
       continue;
     }
-    const bool staged = du <= kStage;
-    if (staged)
-      for (int32_t i = lane; i < du; i += 64) my[i] = dag_cols[ub + i];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int32_t* list = staged ? my : dag_cols + ub;
+    const bool hashed = du <= kStage;
+    uint32_t size = 64;
+    while (size < 2u * (uint32_t)du) size <<= 1;
+    const uint32_t mask = size - 1;
+    if (hashed) {
+      for (uint32_t i = lane; i < size; i += 64) table[i] = -1;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      for (int32_t i = lane; i < du; i += 64) {
+        const int32_t x = dag_cols[ub + i];
+        for (uint32_t h = hash_slot(x, mask);; h = (h + 1) & mask)
+          if (atomicCAS(&table[h], -1, x) == -1) break;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    const int32_t* list = dag_cols + ub;
     uint64_t cnt = 0;
-    for (int32_t k = 0; k < du; ++k) {
-      const int32_t v = list[k];
-      const int64_t vb = dag_off[v], ve = dag_off[v + 1];
-      for (int64_t j = vb + lane; j < ve; j += 64) cnt += search(list, du, dag_cols[j]) ? 1u : 0u;
+    for (int32_t vbase = 0; vbase < du; vbase += 64) {
+      int64_t vb = 0, vlen = 0;
+      if (vbase + lane < du) {
+        const int32_t v = list[vbase + lane];
+        vb = dag_off[v];
+        vlen = dag_off[v + 1] - vb;
+      }
+      const uint32_t incl = wave_inclusive_scan((uint32_t)vlen);
+      const uint32_t tot = __shfl(incl, 63, 64);
+      const uint32_t excl = incl - (uint32_t)vlen;
+      for (uint32_t t0 = 0; t0 < tot; t0 += kTcUnroll * 64) {
+        int32_t w[kTcUnroll];
+        bool ok[kTcUnroll];
+#pragma unroll
+        for (int k = 0; k < kTcUnroll; ++k) {
+          const uint32_t t = t0 + 64 * k + lane;
+          const int owner = scan_owner(incl, t);
+          const uint32_t before = __shfl(excl, owner, 64);
+          const int64_t st = shfl_i64(vb, owner);
+          ok[k] = t < tot;
+          w[k] = ok[k] ? dag_cols[st + (t - before)] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kTcUnroll; ++k)
+          if (ok[k]) cnt += (hashed ? hash_contains(table, mask, w[k]) : search(list, du, w[k])) ? 1u : 0u;
+      }
     }
     cnt = wave_sum_u64(cnt);
     if (lane == 0 && per_vertex) per_vertex[u] = cnt;
     acc_total += cnt;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   if (lane == 0 && acc_total) atomicAdd(total, (unsigned long long)acc_total);
 }
