@@ -31,14 +31,15 @@ def log(*a):
 
 
 def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
-    """Algorithmic bytes of one BFS level.  Push: 2*w_o per frontier vertex,
-    8 B per arc (4 B id + 4 B status probe), 4 B per discovered vertex, three
-    bitmap passes.  Pull: per scanned vertex 2*w_o (its offsets) + 8 B (hint id
-    and its probe); per probe of the adjacency list after a hint miss 8 B
-    (edges_inspected counts the hint probe too); 4 B per discovered vertex;
-    three bitmap passes."""
+    """Algorithmic bytes of one BFS level's traversal kernel.  Push: 2*w_o per
+    frontier vertex, 8 B per arc (4 B id + 4 B status probe), three bitmap
+    passes (the discoveries' depths are written by the count pass).  Pull: per
+    scanned vertex 2*w_o (its offsets) + 8 B (hint id and its probe); per probe
+    of the adjacency list after a hint miss 8 B (edges_inspected counts the
+    hint probe too); 4 B per discovered vertex; three bitmap passes."""
     if ls.direction == 0:
-        return 2 * off_bytes * ls.frontier_vertices + 8 * ls.frontier_edges + 4 * ls.discovered + 3 * w0_bytes
+        # the depths of a push level's discoveries are written by the count pass, in vertex order
+        return 2 * off_bytes * ls.frontier_vertices + 8 * ls.frontier_edges + 3 * w0_bytes
     return (2 * off_bytes * ls.vertices_scanned + 8 * ls.edges_inspected + 4 * ls.discovered + 3 * w0_bytes)
 
 

@@ -67,7 +67,8 @@ template<typename OffT>
 __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
   zero_slots(a.stats_zero);
   /* four arcs per lane: four visited-word loads, then the atomic ORs of the
-   * unset ones, then the depth stores of the claimed ones */
+   * unset ones (no result needed: the count pass finds the new vertices as
+   * visited & ~prev and writes their depths in vertex order) */
   for_each_queued_arc4<OffT>(
       a.qin, a.qin_hdr, a.cols,
       [&](const uint32_t (&y)[4], const int64_t (&)[4], const bool (&ok)[4]) {
@@ -78,11 +79,8 @@ __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint64_t bit = 1ull << (y[k] & 63);
-          old[k] = (w[k] & bit) ? bit : atomic_or(&a.visited[y[k] >> 6], bit);
+          if (!(w[k] & bit)) atomic_or(&a.visited[y[k] >> 6], bit);
         }
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (!(old[k] & (1ull << (y[k] & 63)))) a.depth[y[k]] = level + 1;
       },
       variant);
 }
@@ -336,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, 
   if (ctl->done) return;
   if (ctl->dir == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->mode = 1;
-    compact_count_body<OffT, kFromDiff>(a);
+    compact_count_body<OffT, kFromDiff>(a, 0, ctl->level + 1); /* the push left the depths to this pass */
     return;
   }
   stage_policy(p, &s_pol);
@@ -974,7 +972,8 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     la.qin_hdr = Q.hdr + cur;
     la.cur_bm = ws->bm[cur];
     la.next_bm = ws->bm[nxt];
-    const CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
+    CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
+    ca.depth = depth; /* depths of the vertices a push level discovered */
     BurstArgs<OffT> ba;
     ba.off = off;
     ba.cols = g->cols;

@@ -177,6 +177,8 @@ struct CompactArgs {
   Tuning tune;
   const int32_t* split; /* optional: per vertex, the length of the first part of its list (SSSP's light
                            arcs); the entries then cover the first part (phase 0) or the rest (phase 1) */
+  int32_t* depth;       /* optional (kFromDiff): depth of the new frontier's vertices, written here in
+                           vertex order rather than scattered by the push that discovered them */
 };
 
 /* The arc range [s, s + d) of vertex v's queue entries. */
@@ -198,7 +200,7 @@ __device__ __forceinline__ void entry_range(const CompactArgs<OffT>& a, int64_t 
 /* pass 1: one wave per 64-word tile; the workgroup also sums its four tiles
  * (all threads of the workgroup must call) */
 template<typename OffT, int kMode>
-__device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, int phase = 0) {
+__device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, int phase = 0, int32_t depth_val = 0) {
   __shared__ TileCount s_wave[kBlock / 64];
   const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; /* tiles >= ntiles see only words >= w0 */
   const int lane = lane_id();
@@ -228,6 +230,7 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, i
     const uint64_t word = shfl_u64(mynew, k);
     if ((word >> lane) & 1ull) {
       const int64_t v = (tile * 64 + k) * 64 + lane;
+      if (kMode == kFromDiff && a.depth) a.depth[v] = depth_val;
       int64_t s0, d;
       entry_range(a, v, phase, s0, d);
       ++nv;
