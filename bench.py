@@ -271,7 +271,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the traversal core runs on a "
-              f"single device; see DESIGN.md for how it would shard).", file=sys.stderr)
+              f"single device; sharding over several GPUs is a design, not built).", file=sys.stderr)
         sys.exit(2)
 
     import sygraph_amd as sg
