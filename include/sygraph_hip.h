@@ -269,7 +269,9 @@ typedef struct sg_level_stat {
   float kernel_ms;           /* device time of the level's advance kernel (record_levels) */
   float convert_ms;          /* device time of a queue<->bitmap conversion after it */
   int32_t burst;             /* 1: the level ran inside a single-workgroup burst of small push levels;
-                                the burst's device time is split evenly over its levels (kernel_ms) */
+                                the burst's device time is split evenly over the levels it completed
+                                (kernel_ms); the level at which a burst stops is counted and decided
+                                by the next step, whose times it carries */
 } sg_level_stat;
 
 typedef struct sg_bfs_stats {

@@ -220,6 +220,7 @@ struct BfsCtl {
   int32_t next_act; /* decided after a pull level by count_dyn_kernel: direction * 3 + variant */
   int32_t lb;       /* work-mapping variant of the level about to run */
   int32_t burst;    /* the next level is small enough for burst_kernel */
+  int32_t skip;     /* burst_kernel ran this level: the next step only counts and decides it */
   int32_t n_push, n_pull;
   int64_t f_vertices, f_edges; /* frontier entering the level */
   unsigned long long visited_deg, reached;
@@ -316,6 +317,10 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
   if (ctl->done) return;
+  if (ctl->skip) { /* the level was traversed by the burst that ended the previous step */
+    zero_slots(a.stats_zero);
+    return;
+  }
   const int32_t level = ctl->level;
   if (ctl->dir == 0) push_level(a, ctl->lb, level);
   else pull_level(a, ctl->lb, level);
@@ -415,7 +420,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     e.done = done;
     e.next_dir = next;
     e.action = dir * 3 + ctl->lb;
-    e.burst = 0;
+    e.burst = ctl->skip; /* traversed by the burst that ended the previous step */
     e.f_vertices = ctl->f_vertices;
     e.f_edges = ctl->f_edges;
     e.disc = (int64_t)disc;
@@ -442,6 +447,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   ctl->n_push = n_push;
   ctl->n_pull = n_pull;
   ctl->burst = burst;
+  ctl->skip = 0;
   if (!burst) {
     StepLog sl;
     sl.step = step;
@@ -473,15 +479,15 @@ __global__ void __launch_bounds__(kBlock) write_dyn_kernel(CompactArgs<OffT> a, 
  * at a time into arcs (merge path) and claims discoveries four arcs per lane
  * per round, so four claims are in flight per lane.
  *
- * State handed back to the level pipeline when the burst stops:
- *  - visited, depth: every discovery, as in a push level;
- *  - prev == visited (each discovery also sets its prev bit), so the next
- *    count pass sees only that level's discoveries;
- *  - the queues Q[nxt] and header hold the last discovered set, for a push;
- *  - the frontier bitmap bm[nxt] gains the last discovered set.  It still
- *    holds the frontier the burst started from: a pull tests only
- *    in-neighbours of unvisited vertices, and every out-neighbour of an
- *    earlier frontier vertex is visited, so those bits never match. */
+ * Hand-back.  A burst that ends because a level discovered nothing ends the
+ * traversal.  Otherwise the level at which it stops (its frontier grew past
+ * burst_arcs, or the policy chose pull next) has been traversed but is
+ * treated as a push level that level_kernel ran: its discoveries are in
+ * visited and depth but not in prev, the control block describes it, and the
+ * next step skips the traversal (`skip`) and only counts (visited & ~prev:
+ * the exact next frontier and its queues), scans and decides it.  Earlier
+ * burst levels set the prev bits of their discoveries when the following
+ * level starts, so the count sees that level's discoveries only. */
 constexpr int kBurstThreads = 512;
 constexpr int kBurstWaves = kBurstThreads / 64;
 constexpr int kBurstDefaultArcs = 4096;
@@ -493,7 +499,6 @@ struct BurstArgs {
   const int32_t* cols;
   uint64_t* visited;
   uint64_t* prev;
-  uint64_t* fbm;   /* frontier bitmap read by a pull at the next step (bm[nxt]) */
   int32_t* depth;
   QEntry* q[3];    /* queues of the next step (Q[nxt]) */
   QueueHdr* hdr;
@@ -511,8 +516,8 @@ constexpr int kBurstPiece = 256;  /* arcs per piece of a big list: four per lane
 /* Claim the (up to) four arcs y[k] of each lane: atomic OR on the visited
  * word issued together with the two offset loads a discovery needs, one LDS
  * counter update per wave and list for all four, then the stores.  A
- * discovery gets depth level + 1 and its prev bit and goes to the small or the
- * big list of the next level by its degree. */
+ * discovery gets depth level + 1 and goes to the small or the big list of the
+ * next level by its degree. */
 template<typename OffT>
 __device__ __forceinline__ void burst_claim4(const BurstArgs<OffT>& a, const uint32_t (&y)[4], const bool (&ok)[4],
                                              int32_t level, QEntry* out_small, QEntry* out_big, uint32_t* cnt2,
@@ -555,7 +560,6 @@ __device__ __forceinline__ void burst_claim4(const BurstArgs<OffT>& a, const uin
     if (nw[k]) {
       const uint32_t v = y[k];
       a.depth[v] = level + 1;
-      atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
       QEntry e;
       e.start = vs[k];
       e.v = (int32_t)v;
@@ -669,6 +673,12 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
       burst_visit_big(a, a.q[1], h0.count[1], level, out_small, out_big, s_count[par], acc);
       burst_visit_big(a, a.q[2], h0.count[2], level, out_small, out_big, s_count[par], acc);
     } else {
+      /* the previous burst level's discoveries join prev (fire and forget:
+       * nothing in the burst reads prev) */
+      for (uint32_t i = threadIdx.x; i < n_small + n_big; i += kBurstThreads) {
+        const uint32_t v = (uint32_t)(i < n_small ? a.list[cur][0][i] : a.list[cur][1][i - n_small]).v;
+        atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
+      }
       burst_visit_small(a, a.list[cur][0], n_small, level, out_small, out_big, s_count[par], acc);
       burst_visit_big(a, a.list[cur][1], n_big, level, out_small, out_big, s_count[par], acc);
     }
@@ -701,6 +711,8 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
     const unsigned long long unexplored = p.total_degree - (vdeg + disc_deg);
     act = decide_action(p, &s_pol, 0, disc, disc_deg, unexplored, level + 1, max_deg);
     done = disc == 0 ? 1 : 0;
+    const bool more = !done && act / 3 == 0 && (int64_t)disc_deg <= p.burst_arcs;
+    if (!more && !done) break; /* the next step counts and decides this level (hand-back) */
     if (threadIdx.x == 0 && level < SG_MAX_LEVELS_RECORDED) {
       LevelLog e;
       e.level = level;
@@ -728,51 +740,19 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
     lb = act % 3;
     ++level;
     cur ^= 1;
-    const bool more = !done && act / 3 == 0 && (int64_t)disc_deg <= p.burst_arcs;
-    if (!more) break;
+    if (!more) break; /* done */
   }
-  /* hand the last discovered set (the lists list[cur]) back to the level pipeline */
-  if (threadIdx.x < 3) s_cls[threadIdx.x] = 0;
-  __syncthreads();
-  if (!done) {
-    for (uint32_t i = threadIdx.x; i < n_small + n_big; i += kBurstThreads) {
-      const QEntry e = i < n_small ? a.list[cur][0][i] : a.list[cur][1][i - n_small];
-      atomic_or(&a.fbm[(uint32_t)e.v >> 6], 1ull << ((uint32_t)e.v & 63));
-      if (e.len <= 0) continue;
-      const int cls = degree_class(a.tune, e.len);
-      if (cls < 2) {
-        const uint32_t pos = atomicAdd(&s_cls[cls], 1u);
-        a.q[cls][pos] = e;
-      } else {
-        const uint32_t nch = (uint32_t)((e.len + a.tune.chunk - 1) / a.tune.chunk);
-        const uint32_t pos = atomicAdd(&s_cls[2], nch);
-        for (uint32_t c = 0; c < nch; ++c) {
-          QEntry ce;
-          ce.start = e.start + (int64_t)c * a.tune.chunk;
-          ce.v = e.v;
-          const int64_t rem = (int64_t)e.len - (int64_t)c * a.tune.chunk;
-          ce.len = (int32_t)(rem < a.tune.chunk ? rem : a.tune.chunk);
-          a.q[2][pos + c] = ce;
-        }
-      }
-    }
-  }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    QueueHdr h{};
-    h.count[0] = s_cls[0];
-    h.count[1] = s_cls[1];
-    h.count[2] = s_cls[2];
-    h.nvert = disc;
-    h.deg = disc_deg;
-    h.maxdeg = max_deg;
-    *a.hdr = h;
+    /* done: the traversal ended inside the burst.  Otherwise `level` is the
+     * level the burst traversed last, described as a push level that the
+     * next step only counts and decides. */
     ctl->level = level;
-    ctl->dir = act / 3;
+    ctl->dir = 0;
     ctl->lb = lb;
     ctl->done = done;
     ctl->write = 0;
     ctl->burst = 0;
+    ctl->skip = done ? 0 : 1;
     ctl->n_push = n_push;
     ctl->visited_deg = vdeg;
     ctl->reached = reached;
@@ -979,7 +959,6 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     ba.cols = g->cols;
     ba.visited = ws->visited;
     ba.prev = ws->prev;
-    ba.fbm = ws->bm[nxt];
     ba.depth = depth;
     for (int k = 0; k < 3; ++k) ba.q[k] = Q.q[nxt][k];
     ba.hdr = Q.hdr + nxt;

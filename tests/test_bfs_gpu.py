@@ -272,3 +272,20 @@ def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph,
             assert runs[0][2] >= levels // 2  # most levels of a high-diameter graph run inside bursts
     with pytest.raises(sg.SygraphError):
         b.run(pol, 15.0, 24.0, tuning={"burst_arcs": 65537})
+
+
+def test_bursts_with_fine_degree_classes(sg, device, oracle_mod):
+    """Bursts hand their last discovered set back as queues: with small class
+    thresholds and chunks, short and long lists both become chunked (class 2)
+    entries, written per thread (short lists) and per wave (long lists)."""
+    g = oracle_mod.rmat(13, 16, seed=23)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols)
+    for src in [int(np.argmax(g.degrees())), 17, 4000]:
+        ref, levels = oracle_mod.bfs(g, src)
+        for tuning in ({"t_medium": 8, "t_large": 16, "chunk": 64, "burst_arcs": 65536},
+                       {"t_medium": 8, "t_large": 16, "chunk": 64, "burst_arcs": 512}):
+            b = sg.BFS(G)
+            b.init(src)
+            st = b.run(sg.POLICY_HYBRID, 15.0, 24.0, tuning=tuning)
+            assert np.array_equal(b.get_distances(), ref), (src, tuning)
+            assert st.levels == levels
