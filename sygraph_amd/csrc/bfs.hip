@@ -293,7 +293,8 @@ __device__ __forceinline__ void stage_policy(const DecideParams& p, sg::policy::
  * cur_bm = {src}, F0 queue = {src}, control block for level 0. */
 template<typename OffT>
 __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, QEntry* q1, QEntry* q2, QueueHdr* hdr,
-                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir, int32_t first_lb) {
+                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir, int32_t first_lb,
+                                   int64_t burst_arcs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint64_t bit = 1ull << (src & 63);
   a.depth[src] = 0;
@@ -310,6 +311,9 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
   c.f_edges = (int64_t)d;
   c.visited_deg = d; /* the source's arcs count as explored */
   c.reached = 1;
+  /* a pushing start with a small source list begins in a burst (launched
+   * ahead of the first step) */
+  c.burst = (first_dir == 0 && burst_arcs > 0 && (int64_t)d <= burst_arcs) ? 1 : 0;
   *ctl = c;
 }
 
@@ -922,9 +926,10 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   SG_TRY(sg::zero_async(dev, ws->stats, sizeof(StatSlot) * kSlots * 2));
   /* work-mapping variant under the fixed policies (see sg_bfs_options) */
   const int32_t lb_fixed = opt.load_balancer == SG_LB_WORKGROUP ? 1 : (opt.load_balancer == SG_LB_SUBGROUP ? 2 : 0);
+  const int64_t burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
   init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
                                             opt.policy == SG_POLICY_PULL ? 1 : 0,
-                                            opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed);
+                                            opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed, burst_arcs);
   SG_HIP(hipGetLastError());
 
   const bool timed = opt.record_levels && stats;
@@ -940,9 +945,23 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   P.mlp = opt.policy == SG_POLICY_LEARNED ? sg::policy_device_params(opt.learned) : nullptr;
   P.f6 = sg::policy::feature(6, 0, 0, 0, 0, 0, (double)n, (double)g->m);
   P.f7 = sg::policy::feature(7, 0, 0, 0, 0, 0, (double)n, (double)g->m);
-  P.burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
+  P.burst_arcs = burst_arcs;
   if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 
+  /* the burst of a step reads the queue set its count pass wrote */
+  auto burst_args = [&](int qset) {
+    BurstArgs<OffT> ba;
+    ba.off = off;
+    ba.cols = g->cols;
+    ba.visited = ws->visited;
+    ba.prev = ws->prev;
+    ba.depth = depth;
+    for (int k = 0; k < 3; ++k) ba.q[k] = Q.q[qset][k];
+    ba.hdr = Q.hdr + qset;
+    for (int k = 0; k < 4; ++k) ba.list[k >> 1][k & 1] = ws->burst_list[k];
+    ba.tune = tune;
+    return ba;
+  };
   auto enqueue = [&](int32_t step) -> int {
     BfsArgs<OffT> la = a;
     const int cur = step & 1, nxt = (step + 1) & 1, r = step % kRing;
@@ -954,16 +973,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     la.next_bm = ws->bm[nxt];
     CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
     ca.depth = depth; /* depths of the vertices a push level discovered */
-    BurstArgs<OffT> ba;
-    ba.off = off;
-    ba.cols = g->cols;
-    ba.visited = ws->visited;
-    ba.prev = ws->prev;
-    ba.depth = depth;
-    for (int k = 0; k < 3; ++k) ba.q[k] = Q.q[nxt][k];
-    ba.hdr = Q.hdr + nxt;
-    for (int k = 0; k < 4; ++k) ba.list[k >> 1][k & 1] = ws->burst_list[k];
-    ba.tune = tune;
+    const BurstArgs<OffT> ba = burst_args(nxt);
     if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
@@ -978,6 +988,11 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     return SG_OK;
   };
 
+  /* a start in a burst: the levels from the source while they stay small,
+   * handed back to step 0 (which then only counts and decides) or ending the
+   * traversal; the kernel returns at once unless init set ctl->burst */
+  if (burst_arcs > 0 && opt.policy != SG_POLICY_PULL)
+    burst_kernel<OffT><<<1, kBurstThreads, 0, s>>>(burst_args(0), ws->ctl, P, 0, ws->levlog, ws->steplog);
   SG_TRY(enqueue(0));
   for (int k = 1; k <= lookahead; ++k) SG_TRY(enqueue(k));
   StepLog last{};
