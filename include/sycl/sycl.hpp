@@ -317,6 +317,12 @@ public:
     e.finish();
     return e;
   }
+  event prefetch(const void* ptr, size_t bytes) {
+    event e = event::begin(handle(), stream(), origin());
+    detail::check(sg_prefetch(handle(), ptr, bytes));
+    e.finish();
+    return e;
+  }
   event memset(void* dst, int value, size_t bytes) {
     event e = event::begin(handle(), stream(), origin());
     detail::check(sg_fill(handle(), dst, (uint64_t)(uint8_t)value, 1, bytes));

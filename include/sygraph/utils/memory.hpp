@@ -21,7 +21,11 @@ inline T* memoryAlloc(size_t n, sycl::queue& q) {
   } else if constexpr (V == space::device) {
     return sycl::malloc_device<T>(n, q);
   } else {
-    return sycl::malloc_shared<T>(n, q);
+    T* ptr = sycl::malloc_shared<T>(n, q);
+#ifdef ENABLE_PREFETCH
+    q.prefetch(ptr, n * sizeof(T)).wait(); /* utils/memory.hpp:24-26 */
+#endif
+    return ptr;
   }
 }
 

@@ -100,6 +100,10 @@ int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes);
  * it is the copy kernel against which HBM bandwidth and the FETCH_SIZE /
  * WRITE_SIZE counters are calibrated. */
 int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes);
+/* queue.prefetch of shared (managed) memory to the device, used by
+ * memoryAlloc<T, space::shared> under ENABLE_PREFETCH (utils/memory.hpp:24-26).
+ * A hint: memory the runtime cannot migrate is left where it is (SG_OK). */
+int sg_prefetch(sg_device_t dev, const void* ptr, size_t bytes);
 /* queue.fill for 1/4/8-byte patterns (e.g. algorithms/bfs.hpp:62-66). */
 int sg_fill(sg_device_t dev, void* dst, uint64_t pattern, int pattern_bytes, size_t count);
 

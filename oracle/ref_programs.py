@@ -60,6 +60,11 @@ def _jobs(ref: str):
                    "-I", os.path.join(REPO, "include"), os.path.join(ref, "examples", ex, ex + ".cpp"), *_link(),
                    "-o", os.path.join(OUT, "examples", name)]
             jobs.append((name, cmd))
+    # the graph in managed memory with the prefetch of memoryAlloc<shared>
+    # (examples/CMakeLists.txt GRAPH_LOCATION / ENABLE_PREFETCH options)
+    cmd = [HIPCC, *FLAGS, "-DGRAPH_LOCATION=2", "-DENABLE_PREFETCH", "-I", os.path.join(REPO, "include"),
+           os.path.join(ref, "examples", "bfs", "bfs.cpp"), *_link(), "-o", os.path.join(OUT, "examples", "bfs_shared")]
+    jobs.append(("bfs_shared", cmd))
     for t in TESTS:
         name = t.replace("/", "_")
         # asserts stay enabled (no -DNDEBUG): the tests check with assert()

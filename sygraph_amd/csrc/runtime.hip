@@ -198,6 +198,19 @@ int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes) {
   return SG_OK;
 }
 
+int sg_prefetch(sg_device_t dev, const void* ptr, size_t bytes) {
+  SG_CHECK(dev, SG_ERR_INVALID, "sg_prefetch: null device");
+  if (bytes == 0) return SG_OK;
+  SG_CHECK(ptr, SG_ERR_INVALID, "sg_prefetch: null pointer");
+  const hipError_t e = hipMemPrefetchAsync(ptr, bytes, dev->ordinal, dev->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError(); /* not migratable (not managed, or no support): a hint only */
+    return SG_OK;
+  }
+  SG_HIP(hipStreamSynchronize(dev->stream));
+  return SG_OK;
+}
+
 int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes) {
   SG_CHECK(dev, SG_ERR_INVALID, "sg_copy_device: null device");
   if (bytes == 0) return SG_OK;

@@ -47,3 +47,19 @@ def test_copy_device_kernel(sg, device):
     assert np.array_equal(b.download(np.uint32), x)
     # sizes and pointers must be 16-byte multiples
     assert sg.lib().sg_copy_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), 20) == sg.SG_ERR_INVALID
+
+
+def test_prefetch_shared_memory(sg, device):
+    """sg_prefetch migrates managed memory to the device (a hint: memory it
+    cannot migrate is accepted too) and leaves the contents intact."""
+    n = 1 << 18
+    buf = device.alloc(4 * n, sg.SPACE_SHARED)
+    view = _host_view(buf, n)
+    view[:] = np.arange(n, dtype=np.uint32)
+    sg.check(sg.lib().sg_prefetch(device.h, C.c_void_p(buf.ptr), 4 * n))
+    dev_copy = device.alloc(4 * n, sg.SPACE_DEVICE)
+    sg.check(sg.lib().sg_memcpy(device.h, C.c_void_p(dev_copy.ptr), C.c_void_p(buf.ptr), 4 * n))
+    assert np.array_equal(dev_copy.download(np.uint32), np.arange(n, dtype=np.uint32))
+    pinned = device.alloc(4 * n, sg.SPACE_HOST)
+    sg.check(sg.lib().sg_prefetch(device.h, C.c_void_p(pinned.ptr), 4 * n))
+    assert sg.lib().sg_prefetch(device.h, None, 16) != 0  # null pointer rejected

@@ -95,6 +95,18 @@ def test_example_bfs(device, oracle_mod, rmat_file, mode):
     assert f"Iterations: {levels}" in r.stderr
 
 
+def test_example_bfs_shared_prefetch(device, oracle_mod, rmat_file):
+    """examples/bfs built with GRAPH_LOCATION=2 (graph in managed memory) and
+    ENABLE_PREFETCH: the reference's own validation and the oracle's depths."""
+    g, plain, _ = rmat_file
+    src = int(np.argmax(np.diff(g.offsets)))
+    r = _run([_exe("examples", "bfs_shared"), "-u", "-c", "-p", "-s", str(src), "--advance", "hybrid", plain])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Validation: [Success]" in r.stdout
+    ref, _ = oracle_mod.bfs(g, src)
+    assert _printed_table(r.stdout) == {i: int(d) for i, d in enumerate(ref) if d != g.n + 1}
+
+
 def test_example_sssp(device, oracle_mod, rmat_file):
     g, _, weighted = rmat_file
     oracle_mod.fill_weights(g, 3)
