@@ -66,3 +66,25 @@ def test_training_features_match_the_oracle():
     want = np.stack([mlp.features(stats[L][0], stats[L][1], stats[L - 1][2], L, stats[L - 1][3], n, m)
                      for L in range(1, len(stats))])
     assert np.array_equal(got, want)
+
+
+def test_shipped_policy_and_its_held_out_record():
+    """The shipped parameters (1,542 finite FP32 values) and the record of the
+    run that trained them: held-out graphs disjoint from the training graphs,
+    the learned policy faster than plain push in total and in the geometric
+    mean, and never above the per-level-best bound by more than the run's
+    noise."""
+    import json
+    import os
+    d = os.path.join(os.path.dirname(__file__), "..", "sygraph_amd", "data")
+    p = np.fromfile(os.path.join(d, "bfs_policy.f32"), dtype=np.float32)
+    assert p.shape == (mlp.N_PARAMS,) and np.all(np.isfinite(p))
+    with open(os.path.join(d, "bfs_policy.json")) as f:
+        meta = json.load(f)
+    tr, ho = meta["train_seeds"], meta["held_out_seeds"]
+    assert tr[1] < ho[0] or ho[1] < tr[0]
+    h = meta["held_out"]
+    assert h["learned_speedup_vs_plain_total"] > 1.0 and h["learned_speedup_vs_plain_geomean"] > 1.0
+    tot = h["held_out_total_ms"]
+    assert tot["per_level_best"] <= min(tot["plain"], tot["heuristic"], tot["learned"])
+    assert len(meta["held_out_runs"]) == 4 * (ho[1] - ho[0] + 1)
