@@ -167,6 +167,10 @@ int sg_graph_download(sg_graph_t g, int64_t* offsets, int32_t* cols, uint32_t* w
 /* Order-sensitive digests sum_i mix64(i*K ^ x_i) of offsets and cols, computed
  * on the device (used to pin generated graphs without moving them). */
 int sg_graph_digest(sg_graph_t g, uint64_t* offsets_digest, uint64_t* cols_digest);
+/* The same order-sensitive digest of any device array of count 4- or 8-byte
+ * elements (no reference counterpart: the autotuner compares outputs of
+ * tuned runs without copying them to the host). */
+int sg_digest(sg_device_t dev, const void* ptr, int64_t count, int elem_bytes, uint64_t* digest);
 /* Sum of degrees of vertices whose depth != n+1 (algorithms/bfs.hpp:88-99,
  * the TEPS numerator). depth is a device int32 array. */
 int sg_graph_reached_edges(sg_graph_t g, const int32_t* depth, int64_t* edges, int64_t* vertices);

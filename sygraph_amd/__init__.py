@@ -139,6 +139,7 @@ _SIGS = {
     "sg_graph_generate_er": (_I, [_P, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(_P)]),
     "sg_graph_download": (_I, [_P, _P, _P, _P]),
     "sg_graph_digest": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "sg_digest": (_I, [_P, _P, C.c_int64, C.c_int, C.POINTER(C.c_uint64)]),
     "sg_graph_reached_edges": (_I, [_P, _P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "sg_frontier_create": (_I, [_P, C.c_int64, C.POINTER(_P)]),
     "sg_frontier_destroy": (_I, [_P]),
@@ -293,6 +294,12 @@ class DeviceBuffer:
 
     def fill(self, value: int, pattern_bytes: int, count: int):
         check(lib().sg_fill(self.dev.h, C.c_void_p(self.ptr), value, pattern_bytes, count))
+
+    def digest(self, count: int, elem_bytes: int = 4) -> int:
+        """Order-sensitive 64-bit digest of the first `count` elements, on the device."""
+        h = C.c_uint64()
+        check(lib().sg_digest(self.dev.h, C.c_void_p(self.ptr), count, elem_bytes, C.byref(h)))
+        return h.value
 
 
 class Event:

@@ -714,6 +714,28 @@ int sg_graph_download(sg_graph_t g, int64_t* offsets, int32_t* cols, uint32_t* w
   return SG_OK;
 }
 
+int sg_digest(sg_device_t dev, const void* ptr, int64_t count, int elem_bytes, uint64_t* digest) {
+  SG_CHECK(dev && digest && (ptr || count == 0) && count >= 0, SG_ERR_INVALID, "sg_digest: bad argument");
+  SG_CHECK(elem_bytes == 4 || elem_bytes == 8, SG_ERR_INVALID, "sg_digest: element size %d not 4 or 8", elem_bytes);
+  unsigned long long* acc = nullptr;
+  SG_TRY(sg::alloc_device(dev, (void**)&acc, 8));
+  SG_TRY(sg::zero_async(dev, acc, 8));
+  if (count > 0) {
+    if (elem_bytes == 8)
+      digest_kernel<int64_t><<<sg::grid_for(dev, count, 256), 256, 0, dev->stream>>>((const int64_t*)ptr, count, acc);
+    else
+      digest_kernel<uint32_t><<<sg::grid_for(dev, count, 256), 256, 0, dev->stream>>>((const uint32_t*)ptr, count, acc);
+  }
+  unsigned long long h = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, acc, 8, hipMemcpyDeviceToHost, dev->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+  (void)hipFree(acc);
+  SG_HIP(e);
+  *digest = h;
+  return SG_OK;
+}
+
 int sg_graph_digest(sg_graph_t g, uint64_t* offsets_digest, uint64_t* cols_digest) {
   SG_CHECK(g && offsets_digest && cols_digest, SG_ERR_INVALID, "sg_graph_digest: null argument");
   sg_device_t dev = g->dev;

@@ -63,3 +63,16 @@ def test_prefetch_shared_memory(sg, device):
     pinned = device.alloc(4 * n, sg.SPACE_HOST)
     sg.check(sg.lib().sg_prefetch(device.h, C.c_void_p(pinned.ptr), 4 * n))
     assert sg.lib().sg_prefetch(device.h, None, 16) != 0  # null pointer rejected
+
+
+def test_device_digest_matches_oracle(sg, device, oracle_mod):
+    """sg_digest (the autotuner's output comparison) equals the oracle's
+    order-sensitive digest for 4- and 8-byte arrays."""
+    rng = np.random.default_rng(9)
+    x32 = rng.integers(0, 2**31, size=123_457, dtype=np.int32)
+    x64 = rng.integers(0, 2**62, size=54_321, dtype=np.int64)
+    assert device.to_device(x32).digest(len(x32), 4) == oracle_mod.digest_i32(x32)
+    assert device.to_device(x64).digest(len(x64), 8) == oracle_mod.digest_i64(x64)
+    y = x32.copy()
+    y[[0, 1]] = y[[1, 0]]  # order-sensitive
+    assert device.to_device(y).digest(len(y), 4) != oracle_mod.digest_i32(x32)
