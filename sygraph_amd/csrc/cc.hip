@@ -4,12 +4,18 @@
  * The reference propagates the larger label along arcs until nothing changes,
  * so on an undirected graph every vertex ends with the largest vertex id of
  * its component.  That fixpoint is reached here without per-level rounds:
- *  - undirected graphs: concurrent union-find.  Every arc (u,v), u < v, hooks
- *    the smaller of the two roots under the larger with a CAS, so parent[x] >= x
+ *  - undirected graphs: concurrent union-find.  An arc (u,v) hooks the
+ *    smaller of the two roots under the larger with a CAS, so parent[x] >= x
  *    always holds and the surviving root of a component is its largest id
- *    (the largest vertex can never be hooked under a larger root).  A final
- *    pass points every vertex at its root.  Work is O(m) regardless of the
- *    graph's diameter, which matters on the 4096x4096 grid.
+ *    (the largest vertex can never be hooked under a larger root).  Work is
+ *    O(m) regardless of the graph's diameter, and most arcs are skipped by
+ *    sampling (the Afforest scheme of Sutton, Ben-Nun and Barak, IPDPS 2018):
+ *    every vertex first links its first kSampleArcs arcs; after a flatten the
+ *    most frequent root among a fixed sample of vertices is taken as the
+ *    giant component; then only vertices outside it link their remaining
+ *    arcs (an arc into the giant component is seen from its other end, since
+ *    both directions are stored), and a last flatten points every vertex at
+ *    its root.
  *  - directed graphs: label propagation along out-arcs with atomic max
  *    (labels[v] = largest id that reaches v), frontier by frontier, which is
  *    exactly the reference's fixpoint on a directed graph.
@@ -18,6 +24,7 @@
  */
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "traverse.hpp"
 
@@ -51,9 +58,40 @@ template<typename OffT>
 __global__ void __launch_bounds__(kBlock) hook_kernel(QEntry* q0, QEntry* q1, QEntry* q2, const QueueHdr* hdr,
                                                       const int32_t* cols, uint32_t* parent) {
   QEntry* const q[3] = {q0, q1, q2};
-  for_each_queued_arc<OffT>(q, hdr, cols, [&](uint32_t u, uint32_t v, int64_t) {
-    if (u < v) hook(parent, u, v);
-  });
+  for_each_queued_arc<OffT>(q, hdr, cols, [&](uint32_t u, uint32_t v, int64_t) { hook(parent, u, v); });
+}
+
+constexpr int kSampleArcs = 2;
+constexpr int kRootSamples = 1024;
+
+/* every vertex links its first kSampleArcs arcs */
+template<typename OffT>
+__global__ void sample_link_kernel(const OffT* off, const int32_t* cols, int64_t n, uint32_t* parent) {
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < n; u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = (int64_t)off[u], e = (int64_t)off[u + 1];
+    for (int64_t j = b; j < e && j < b + kSampleArcs; ++j) hook(parent, (uint32_t)u, (uint32_t)cols[j]);
+  }
+}
+
+/* roots of the sampled vertices i * n / kRootSamples (flattened parents) */
+__global__ void sample_roots_kernel(const uint32_t* parent, int64_t n, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kRootSamples) out[i] = parent[(int64_t)i * n / kRootSamples];
+}
+
+/* vertices outside the giant component with arcs left after the sampled ones */
+template<typename OffT>
+__global__ void outside_kernel(const OffT* off, const uint32_t* parent, int64_t n, int64_t w0, uint32_t giant,
+                               uint64_t* bm) {
+  const int lane = lane_id();
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = gw; w < w0; w += nw) {
+    const int64_t v = w * 64 + lane;
+    const bool keep = v < n && parent[v] != giant && (int64_t)(off[v + 1] - off[v]) > kSampleArcs;
+    const uint64_t m = ballot(keep);
+    if (lane == 0) bm[w] = m;
+  }
 }
 
 __global__ void iota_kernel(uint32_t* p, int64_t n) {
@@ -122,10 +160,38 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
   iota_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(labels, n);
   fill_words_kernel<<<sg::grid_for(dev, w0, 256, 4), 256, 0, s>>>(bm[0], w0, n);
   if (!(g->flags & SG_GRAPH_DIRECTED)) {
-    st = compact_to_queues<OffT, kFromBitmap>(s, Q, 0, off, nullptr, nullptr, bm[0], tune);
+    const int grid_n = sg::grid_for(dev, n, 256, 4);
+    sample_link_kernel<OffT><<<grid_n, 256, 0, s>>>(off, g->cols, n, labels);
+    flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
+    /* the giant component: the most frequent root of a fixed vertex sample */
+    uint32_t* roots = nullptr;
+    if ((st = sg::alloc_device(dev, (void**)&roots, 4 * kRootSamples))) return cleanup(), st;
+    std::vector<uint32_t> h(kRootSamples);
+    if (n > 0) sample_roots_kernel<<<kRootSamples / 256, 256, 0, s>>>(labels, n, roots);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess && n > 0) he = hipMemcpyAsync(h.data(), roots, 4 * kRootSamples, hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    (void)hipFree(roots);
+    if (he != hipSuccess) {
+      cleanup();
+      SG_HIP(he);
+    }
+    std::sort(h.begin(), h.end());
+    uint32_t giant = h[0];
+    for (size_t i = 0, best = 0; i < h.size();) {
+      size_t j = i;
+      while (j < h.size() && h[j] == h[i]) ++j;
+      if (j - i > best) {
+        best = j - i;
+        giant = h[i];
+      }
+      i = j;
+    }
+    outside_kernel<OffT><<<sg::grid_for(dev, w0 * 64, 256, 4), 256, 0, s>>>(off, labels, n, w0, giant, bm[0]);
+    st = compact_to_queues<OffT, kFromBitmap>(s, Q, 0, off, nullptr, nullptr, bm[0], tune, false, kSampleArcs);
     if (st == SG_OK) {
       hook_kernel<OffT><<<grid, kBlock, 0, s>>>(Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, g->cols, labels);
-      flatten_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(labels, n);
+      flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
       rounds = 1;
     }
   } else {

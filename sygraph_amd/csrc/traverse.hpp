@@ -179,6 +179,7 @@ struct CompactArgs {
                            arcs); the entries then cover the first part (phase 0) or the rest (phase 1) */
   int32_t* depth;       /* optional (kFromDiff): depth of the new frontier's vertices, written here in
                            vertex order rather than scattered by the push that discovered them */
+  int32_t split_k;      /* without `split`: a first part of min(split_k, degree) arcs (0: none) */
 };
 
 /* The arc range [s, s + d) of vertex v's queue entries. */
@@ -186,8 +187,8 @@ template<typename OffT>
 __device__ __forceinline__ void entry_range(const CompactArgs<OffT>& a, int64_t v, int phase, int64_t& s, int64_t& d) {
   s = (int64_t)a.off[v];
   d = (int64_t)(a.off[v + 1] - a.off[v]);
-  if (a.split) {
-    const int64_t first = a.split[v];
+  if (a.split || a.split_k > 0) {
+    const int64_t first = a.split ? (int64_t)a.split[v] : (d < a.split_k ? d : (int64_t)a.split_k);
     if (phase) {
       s += first;
       d -= first;
@@ -275,8 +276,8 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, i
 }
 
 template<typename OffT, int kMode>
-__global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT> a) {
-  compact_count_body<OffT, kMode>(a);
+__global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT> a, int phase) {
+  compact_count_body<OffT, kMode>(a, phase);
 }
 
 
@@ -421,8 +422,8 @@ __device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, i
 }
 
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT> a) {
-  compact_write_body<OffT>(a);
+__global__ void __launch_bounds__(kBlock) compact_write_kernel(CompactArgs<OffT> a, int phase) {
+  compact_write_body<OffT>(a, phase);
 }
 
 
@@ -660,10 +661,12 @@ inline CompactArgs<OffT> make_compact_args(QueueSet& qs, int dst, const OffT* of
 }
 
 /* Launch the three compaction passes into queue set `dst`.  count_only skips
- * the write pass (totals still land in hdr[dst]). */
+ * the write pass (totals still land in hdr[dst]).  split_k > 0: the entries
+ * cover each list after its first split_k arcs. */
 template<typename OffT, int kMode>
 inline int compact_to_queues(hipStream_t s, QueueSet& qs, int dst, const OffT* off, const uint64_t* visited,
-                             uint64_t* prev, uint64_t* frontier, const Tuning& t, bool count_only = false) {
+                             uint64_t* prev, uint64_t* frontier, const Tuning& t, bool count_only = false,
+                             int32_t split_k = 0) {
   CompactArgs<OffT> a{};
   a.off = off;
   a.w0 = qs.w0;
@@ -676,9 +679,11 @@ inline int compact_to_queues(hipStream_t s, QueueSet& qs, int dst, const OffT* o
   for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
   a.hdr = qs.hdr + dst;
   a.tune = t;
-  compact_count_kernel<OffT, kMode><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  a.split_k = split_k;
+  const int phase = split_k > 0 ? 1 : 0;
+  compact_count_kernel<OffT, kMode><<<qs.grid_tiles(), kBlock, 0, s>>>(a, phase);
   compact_scan_kernel<<<1, kScanThreads, 0, s>>>(qs.blocks, qs.nblocks, qs.hdr + dst);
-  if (!count_only) compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  if (!count_only) compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a, phase);
   SG_HIP(hipGetLastError());
   return SG_OK;
 }
@@ -698,7 +703,7 @@ inline int compact_write_only(hipStream_t s, QueueSet& qs, int dst, const OffT* 
   for (int k = 0; k < 3; ++k) a.qout[k] = qs.q[dst][k];
   a.hdr = qs.hdr + dst;
   a.tune = t;
-  compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a);
+  compact_write_kernel<OffT><<<qs.grid_tiles(), kBlock, 0, s>>>(a, 0);
   SG_HIP(hipGetLastError());
   return SG_OK;
 }

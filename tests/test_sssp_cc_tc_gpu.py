@@ -232,3 +232,32 @@ def test_tc_rmat20_full_size(sg, device, oracle_mod):
     G = sg.Graph.rmat(device, 20, 32, seed=5)
     assert [str(x) for x in G.digest()] == ref["rmat20"]["digest"]
     assert str(sg.TC(G).run()) == ref["rmat20"]["triangles"]
+
+
+def test_cc_sampling_shapes(sg, device, oracle_mod):
+    """Undirected CC links a sample of arcs, takes the most frequent root as the
+    giant component and links only the other vertices' remaining arcs: graphs
+    without a giant component (a perfect matching, many small stars), with
+    two large components of equal size, and with hubs, all give the oracle's
+    labels (largest id per component)."""
+    rng = np.random.default_rng(48)
+    n = 6000
+    graphs = []
+    perm = rng.permutation(n)
+    graphs.append(oracle_mod.csr_from_pairs(n, perm[0::2], perm[1::2]))  # perfect matching
+    centres = rng.choice(n, 300, replace=False)
+    leaves = np.setdiff1d(np.arange(n), centres)
+    graphs.append(oracle_mod.csr_from_pairs(n, centres[rng.integers(0, 300, len(leaves))], leaves))  # stars
+    half = n // 2  # two random trees of equal size, interleaved ids
+    ids = rng.permutation(n)
+    a, b = ids[:half], ids[half:]
+    u = np.concatenate([a[1:], b[1:]])
+    v = np.concatenate([a[rng.integers(0, np.arange(1, half))], b[rng.integers(0, np.arange(1, half))]])
+    graphs.append(oracle_mod.csr_from_pairs(n, u, v))
+    graphs.append(oracle_mod.rmat(13, 4, seed=49))  # hubs, many isolated vertices
+    for g in graphs:
+        G = sg.Graph.from_csr(device, g.offsets, g.cols)
+        c = sg.CC(G)
+        c.init()
+        c.run()
+        assert np.array_equal(c.get_labels(), oracle_mod.cc(g))
