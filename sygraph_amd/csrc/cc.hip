@@ -143,17 +143,15 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
   hipStream_t s = dev->stream;
   const int64_t n = g->n, w0 = (n + 63) / 64;
   const Tuning tune = default_tuning();
-  QueueSet Q;
+  QueueSet* Qp = nullptr;
   uint64_t* bm[2] = {nullptr, nullptr};
-  int st = Q.init(dev, n, g->m, tune);
-  if (st == SG_OK) st = sg::alloc_device(dev, (void**)&bm[0], 8 * (w0 + 1));
-  if (st == SG_OK) st = sg::alloc_device(dev, (void**)&bm[1], 8 * (w0 + 1));
-  auto cleanup = [&]() {
-    Q.release();
-    (void)hipFree(bm[0]);
-    (void)hipFree(bm[1]);
-  };
-  if (st != SG_OK) return cleanup(), st;
+  uint32_t* roots = nullptr;
+  int st = graph_queue_set(g, &Qp);
+  if (st == SG_OK) st = sg::graph_buffer(g, "cc_bm0", 8 * (w0 + 1), &bm[0]);
+  if (st == SG_OK) st = sg::graph_buffer(g, "cc_bm1", 8 * (w0 + 1), &bm[1]);
+  if (st == SG_OK) st = sg::graph_buffer(g, "cc_roots", 4 * kRootSamples, &roots);
+  if (st != SG_OK) return st;
+  QueueSet& Q = *Qp;
   const OffT* off = (const OffT*)g->offsets;
   const int grid = dev->cus * 8;
   int32_t rounds = 0;
@@ -164,17 +162,12 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
     sample_link_kernel<OffT><<<grid_n, 256, 0, s>>>(off, g->cols, n, labels);
     flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
     /* the giant component: the most frequent root of a fixed vertex sample */
-    uint32_t* roots = nullptr;
-    if ((st = sg::alloc_device(dev, (void**)&roots, 4 * kRootSamples))) return cleanup(), st;
     std::vector<uint32_t> h(kRootSamples);
-    if (n > 0) sample_roots_kernel<<<kRootSamples / 256, 256, 0, s>>>(labels, n, roots);
-    hipError_t he = hipGetLastError();
-    if (he == hipSuccess && n > 0) he = hipMemcpyAsync(h.data(), roots, 4 * kRootSamples, hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess) he = hipStreamSynchronize(s);
-    (void)hipFree(roots);
-    if (he != hipSuccess) {
-      cleanup();
-      SG_HIP(he);
+    if (n > 0) {
+      sample_roots_kernel<<<kRootSamples / 256, 256, 0, s>>>(labels, n, roots);
+      SG_HIP(hipGetLastError());
+      SG_HIP(hipMemcpyAsync(h.data(), roots, 4 * kRootSamples, hipMemcpyDeviceToHost, s));
+      SG_HIP(hipStreamSynchronize(s));
     }
     std::sort(h.begin(), h.end());
     uint32_t giant = h[0];
@@ -213,14 +206,23 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
       if (hh->nvert == 0) break;
     }
   }
-  hipError_t e = hipStreamSynchronize(s);
-  cleanup();
-  SG_HIP(e);
+  SG_HIP(hipStreamSynchronize(s));
   if (rounds_out) *rounds_out = rounds;
   return st;
 }
 
 } // namespace
+
+namespace sg {
+void aux_queues_release(sg_graph_t g) {
+  if (g && g->aux_queues) {
+    auto* qs = static_cast<sg::trav::QueueSet*>(g->aux_queues);
+    qs->release();
+    delete qs;
+    g->aux_queues = nullptr;
+  }
+}
+} // namespace sg
 
 extern "C" int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds) {
   SG_CHECK(g && labels, SG_ERR_INVALID, "sg_cc: null argument");

@@ -507,6 +507,31 @@ int graph_ensure_pull_hint(sg_graph_t g) {
 }
 } // namespace sg
 
+namespace sg {
+int graph_buffer(sg_graph_t g, const char* name, size_t bytes, void** out) {
+  auto& e = g->buffers[name];
+  if (e.first && e.second >= bytes) {
+    *out = e.first;
+    return SG_OK;
+  }
+  if (e.first) {
+    SG_HIP(hipStreamSynchronize(g->dev->stream)); /* queued work may still use the old buffer */
+    (void)hipFree(e.first);
+    e.first = nullptr;
+    e.second = 0;
+  }
+  SG_TRY(alloc_device(g->dev, &e.first, bytes > 0 ? bytes : 1));
+  e.second = bytes;
+  *out = e.first;
+  return SG_OK;
+}
+
+void graph_buffers_release(sg_graph_t g) {
+  for (auto& kv : g->buffers) (void)hipFree(kv.second.first);
+  g->buffers.clear();
+}
+} // namespace sg
+
 extern "C" {
 
 int sg_graph_create(sg_device_t dev, int64_t n, int64_t m, const void* offsets, int offset_bytes, const int32_t* cols,
@@ -576,11 +601,14 @@ int sg_graph_create(sg_device_t dev, int64_t n, int64_t m, const void* offsets, 
   return SG_OK;
 }
 
+
 int sg_graph_destroy(sg_graph_t g) {
   if (!g) return SG_OK;
   (void)hipStreamSynchronize(g->dev->stream);
   sg::bfs_workspace_release(g);
   sg::sssp_workspace_release(g);
+  sg::graph_buffers_release(g);
+  sg::aux_queues_release(g);
   free_graph_arrays(g);
   delete g;
   return SG_OK;

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <atomic>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -119,6 +120,8 @@ struct sg_graph_s {
   int64_t total_degree = 0;
   void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */
   void* sssp_ws = nullptr;
+  void* aux_queues = nullptr;   /* degree-class queue set shared by CC, multi-source BFS and batched BC */
+  std::map<std::string, std::pair<void*, size_t>> buffers; /* named device buffers kept with the graph */
 };
 
 struct sg_frontier_s {
@@ -150,6 +153,19 @@ int graph_ensure_pull_hint(sg_graph_t g);
 const float* policy_device_params(sg_policy_t p); /* device parameters of a learned policy */
 void bfs_workspace_release(sg_graph_t g);
 void sssp_workspace_release(sg_graph_t g);
+/* A device buffer of at least `bytes` kept with the graph under `name`
+ * (grown, not preserved, when a call needs more); repeated calls on one graph
+ * then allocate nothing.  Freed with the graph. */
+int graph_buffer(sg_graph_t g, const char* name, size_t bytes, void** out);
+template<typename T>
+inline int graph_buffer(sg_graph_t g, const char* name, size_t bytes, T** out) {
+  void* p = nullptr;
+  const int st = graph_buffer(g, name, bytes, &p);
+  *out = static_cast<T*>(p);
+  return st;
+}
+void graph_buffers_release(sg_graph_t g);
+void aux_queues_release(sg_graph_t g);
 
 /* frontier compaction used by other modules */
 int frontier_compact_launch(sg_frontier_t f, int mode);

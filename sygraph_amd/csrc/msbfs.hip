@@ -173,16 +173,12 @@ __global__ void ms_fill_kernel(int32_t* p, int32_t val, int64_t cnt) {
     p[i] = val;
 }
 
+/* buffers kept with the graph (repeated calls allocate nothing) */
 struct MsBuffers {
   uint64_t *visited = nullptr, *frontier = nullptr, *next = nullptr, *active = nullptr;
   int32_t* src = nullptr;
   MsStats* stats = nullptr;
-  QueueSet qs;
-  ~MsBuffers() {
-    void* ps[] = {visited, frontier, next, active, src, stats};
-    for (void* p : ps) (void)hipFree(p);
-    qs.release();
-  }
+  QueueSet* qs = nullptr;
 };
 
 template<typename OffT>
@@ -192,13 +188,13 @@ int run_msbfs(sg_graph_t g, int32_t nsrc, const int32_t* sources, int32_t* depth
   const int64_t n = g->n, w0 = (n + 63) / 64;
   const Tuning tune = default_tuning();
   MsBuffers B;
-  SG_TRY(sg::alloc_device(dev, (void**)&B.visited, 8 * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.frontier, 8 * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.next, 8 * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.active, 8 * (w0 + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.src, 4 * 64));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.stats, sizeof(MsStats) * kSlots));
-  SG_TRY(B.qs.init(dev, n, g->m, tune));
+  SG_TRY(sg::graph_buffer(g, "ms_visited", 8 * (n + 1), &B.visited));
+  SG_TRY(sg::graph_buffer(g, "ms_frontier", 8 * (n + 1), &B.frontier));
+  SG_TRY(sg::graph_buffer(g, "ms_next", 8 * (n + 1), &B.next));
+  SG_TRY(sg::graph_buffer(g, "ms_active", 8 * (w0 + 1), &B.active));
+  SG_TRY(sg::graph_buffer(g, "ms_src", 4 * 64, &B.src));
+  SG_TRY(sg::graph_buffer(g, "ms_stats", sizeof(MsStats) * kSlots, &B.stats));
+  SG_TRY(graph_queue_set(g, &B.qs));
   const OffT* off = (const OffT*)g->offsets;
   const OffT* in_off = (const OffT*)(g->inv_offsets ? g->inv_offsets : g->offsets);
   const int32_t* in_cols = g->inv_cols ? g->inv_cols : g->cols;
@@ -228,8 +224,8 @@ int run_msbfs(sg_graph_t g, int32_t nsrc, const int32_t* sources, int32_t* depth
       if (pull) {
         ms_pull_kernel<OffT><<<grid_main, kBlock, 0, s>>>(in_off, in_cols, n, full, B.frontier, B.visited, B.next);
       } else {
-        SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, B.qs, 0, off, nullptr, nullptr, B.active, tune)));
-        ms_push_kernel<OffT><<<grid_main, kBlock, 0, s>>>(B.qs.q[0][0], B.qs.q[0][1], B.qs.q[0][2], B.qs.hdr, g->cols,
+        SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, *B.qs, 0, off, nullptr, nullptr, B.active, tune)));
+        ms_push_kernel<OffT><<<grid_main, kBlock, 0, s>>>(B.qs->q[0][0], B.qs->q[0][1], B.qs->q[0][2], B.qs->hdr, g->cols,
                                                           B.frontier, B.visited, B.next);
       }
       SG_TRY(sg::zero_async(dev, B.stats, sizeof(MsStats) * kSlots));
@@ -570,6 +566,7 @@ __global__ void bc_add_to_float_kernel(const double* acc, float* out, int64_t n)
     out[i] = (float)((double)out[i] + acc[i]);
 }
 
+/* buffers kept with the graph (repeated calls allocate nothing) */
 struct BcBuffers {
   uint64_t *visited = nullptr, *active = nullptr;
   std::vector<uint64_t*> masks;
@@ -580,13 +577,7 @@ struct BcBuffers {
   BcCounts *groups = nullptr, *total = nullptr;
   int32_t* src = nullptr;
   MsStats* stats = nullptr;
-  QueueSet qs;
-  ~BcBuffers() {
-    void* ps[] = {visited, active, T, part, bc64, gtab, slots, ents, verts, groups, total, src, stats};
-    for (void* p : ps) (void)hipFree(p);
-    for (uint64_t* m : masks) (void)hipFree(m);
-    qs.release();
-  }
+  QueueSet* qs = nullptr;
 };
 
 /* count / scan / write of the level's work lists, then the task and
@@ -614,19 +605,19 @@ int run_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources, float* ou
   const Tuning tune = default_tuning();
   BcBuffers B;
   const int64_t ent_cap = n + 2 * (m / kBcChunk) + 64, part_cap = 2 * (m / kBcChunk) + 64;
-  SG_TRY(sg::alloc_device(dev, (void**)&B.visited, 8 * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.active, 8 * (w0 + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.T, 8 * 64 * (size_t)(n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.part, 8 * 64 * (size_t)part_cap));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.bc64, 8 * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.slots, 8 * 64 * kBcSlotStride));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.ents, sizeof(BcEntry) * ent_cap));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.verts, sizeof(BcVert) * (n + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.groups, sizeof(BcCounts) * (w0 + 1)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.total, sizeof(BcCounts)));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.src, 4 * 64));
-  SG_TRY(sg::alloc_device(dev, (void**)&B.stats, sizeof(MsStats) * kSlots));
-  SG_TRY(B.qs.init(dev, n, m, tune));
+  SG_TRY(sg::graph_buffer(g, "ms_visited", 8 * (n + 1), &B.visited));
+  SG_TRY(sg::graph_buffer(g, "ms_active", 8 * (w0 + 1), &B.active));
+  SG_TRY(sg::graph_buffer(g, "bcb_rows", 8 * 64 * (size_t)(n + 1), &B.T));
+  SG_TRY(sg::graph_buffer(g, "bcb_part", 8 * 64 * (size_t)part_cap, &B.part));
+  SG_TRY(sg::graph_buffer(g, "bcb_bc64", 8 * (n + 1), &B.bc64));
+  SG_TRY(sg::graph_buffer(g, "bcb_slots", 8 * 64 * kBcSlotStride, &B.slots));
+  SG_TRY(sg::graph_buffer(g, "bcb_entries", sizeof(BcEntry) * ent_cap, &B.ents));
+  SG_TRY(sg::graph_buffer(g, "bcb_verts", sizeof(BcVert) * (n + 1), &B.verts));
+  SG_TRY(sg::graph_buffer(g, "bcb_groups", sizeof(BcCounts) * (w0 + 1), &B.groups));
+  SG_TRY(sg::graph_buffer(g, "bcb_total", sizeof(BcCounts), &B.total));
+  SG_TRY(sg::graph_buffer(g, "ms_src", 4 * 64, &B.src));
+  SG_TRY(sg::graph_buffer(g, "ms_stats", sizeof(MsStats) * kSlots, &B.stats));
+  SG_TRY(graph_queue_set(g, &B.qs));
   SG_TRY(sg::zero_async(dev, B.bc64, 8 * (n + 1)));
   SG_TRY(sg::zero_async(dev, B.slots, 8 * 64 * kBcSlotStride));
   const OffT* off = (const OffT*)g->offsets;
@@ -641,7 +632,8 @@ int run_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources, float* ou
       SG_CHECK((int64_t)B.masks.size() < max_levels, SG_ERR_UNSUPPORTED,
                "sg_bc_batched: more than %lld levels exceed the mask budget; use sg_bc", (long long)max_levels);
       uint64_t* p = nullptr;
-      SG_TRY(sg::alloc_device(dev, (void**)&p, 8 * (n + 1)));
+      const std::string name = "bcb_mask_" + std::to_string(B.masks.size());
+      SG_TRY(sg::graph_buffer(g, name.c_str(), 8 * (n + 1), &p));
       B.masks.push_back(p);
     }
     bc_fill_u64_kernel<<<grid_n, kBlock, 0, s>>>(B.masks[d], n + 1);
@@ -658,7 +650,7 @@ int run_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources, float* ou
     bc_fill_u64_kernel<<<sg::grid_for(dev, w0 + 1, kBlock, 4), kBlock, 0, s>>>(B.active, w0 + 1);
     uint64_t* m0 = nullptr;
     SG_TRY(mask(0, &m0));
-    if (!B.gtab) SG_TRY(sg::alloc_device(dev, (void**)&B.gtab, 8 * 64 * (size_t)(max_levels + 1)));
+    if (!B.gtab) SG_TRY(sg::graph_buffer(g, "bcb_gtab", 8 * 64 * (size_t)(max_levels + 1), &B.gtab));
     bc_sources_kernel<<<1, 64, 0, s>>>(B.src, k, B.visited, m0, B.active, B.T, gl_of(0));
     int64_t active = 0;
     {
@@ -674,8 +666,8 @@ int run_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources, float* ou
       if (active * 20 > n) {
         ms_pull_kernel<OffT><<<grid_main, kBlock, 0, s>>>(in_off, in_cols, n, full, fr, B.visited, nx);
       } else {
-        SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, B.qs, 0, off, nullptr, nullptr, B.active, tune)));
-        ms_push_kernel<OffT><<<grid_main, kBlock, 0, s>>>(B.qs.q[0][0], B.qs.q[0][1], B.qs.q[0][2], B.qs.hdr, g->cols,
+        SG_TRY((compact_to_queues<OffT, kFromBitmap>(s, *B.qs, 0, off, nullptr, nullptr, B.active, tune)));
+        ms_push_kernel<OffT><<<grid_main, kBlock, 0, s>>>(B.qs->q[0][0], B.qs->q[0][1], B.qs->q[0][2], B.qs->hdr, g->cols,
                                                           fr, B.visited, nx);
       }
       /* sigma of the vertices reached at depth d + 1 (in-arcs from depth d) */

@@ -176,23 +176,17 @@ extern "C" int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   const int64_t n = g->n;
+  /* the oriented graph and the scan scratch are kept with the graph, so
+   * repeated counts allocate nothing once the buffers have grown */
   uint32_t* outdeg = nullptr;
   int64_t* dag_off = nullptr;
   int32_t* dag_cols = nullptr;
   unsigned long long* total = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
-  auto cleanup = [&]() {
-    (void)hipFree(outdeg);
-    (void)hipFree(dag_off);
-    (void)hipFree(dag_cols);
-    (void)hipFree(total);
-    (void)hipFree(tmp);
-  };
-  int st = SG_OK;
-  if ((st = sg::alloc_device(dev, (void**)&outdeg, 4 * (n + 1))) ||
-      (st = sg::alloc_device(dev, (void**)&dag_off, 8 * (n + 1))) || (st = sg::alloc_device(dev, (void**)&total, 8)))
-    return cleanup(), st;
+  SG_TRY(sg::graph_buffer(g, "tc_outdeg", 4 * (n + 1), &outdeg));
+  SG_TRY(sg::graph_buffer(g, "tc_dag_off", 8 * (n + 1), &dag_off));
+  SG_TRY(sg::graph_buffer(g, "tc_total", 8, &total));
   const int grid = dev->cus * 8;
   if (g->offset_bytes == 8)
     orient_kernel<int64_t, false><<<grid, kBlock, 0, s>>>((const int64_t*)g->offsets, g->cols, n, outdeg, nullptr, nullptr);
@@ -204,15 +198,15 @@ extern "C" int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex) {
   SG_TRY(sg::zero_async(dev, outdeg + n, 4));
   if (rocprim::exclusive_scan(nullptr, tmp_bytes, outdeg, dag_off, (int64_t)0, (size_t)(n + 1), rocprim::plus<int64_t>(),
                               s) != hipSuccess)
-    return cleanup(), sg::set_error("sg_tc: scan sizing failed"), SG_ERR_HIP;
-  if ((st = sg::alloc_device(dev, &tmp, tmp_bytes))) return cleanup(), st;
+    return sg::set_error("sg_tc: scan sizing failed"), SG_ERR_HIP;
+  SG_TRY(sg::graph_buffer(g, "tc_scan_tmp", tmp_bytes, &tmp));
   if (rocprim::exclusive_scan(tmp, tmp_bytes, outdeg, dag_off, (int64_t)0, (size_t)(n + 1), rocprim::plus<int64_t>(),
                               s) != hipSuccess)
-    return cleanup(), sg::set_error("sg_tc: scan failed"), SG_ERR_HIP;
+    return sg::set_error("sg_tc: scan failed"), SG_ERR_HIP;
   int64_t mdag = 0;
   SG_HIP(hipMemcpyAsync(&mdag, dag_off + n, 8, hipMemcpyDeviceToHost, s));
   SG_HIP(hipStreamSynchronize(s));
-  if ((st = sg::alloc_device(dev, (void**)&dag_cols, 4 * (mdag + 1)))) return cleanup(), st;
+  SG_TRY(sg::graph_buffer(g, "tc_dag_cols", 4 * (mdag + 1), &dag_cols));
   if (g->offset_bytes == 8)
     orient_kernel<int64_t, true><<<grid, kBlock, 0, s>>>((const int64_t*)g->offsets, g->cols, n, nullptr, dag_off, dag_cols);
   else
@@ -223,9 +217,7 @@ extern "C" int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex) {
   SG_HIP(hipGetLastError());
   unsigned long long h = 0;
   SG_HIP(hipMemcpyAsync(&h, total, 8, hipMemcpyDeviceToHost, s));
-  hipError_t e = hipStreamSynchronize(s);
-  cleanup();
-  SG_HIP(e);
+  SG_HIP(hipStreamSynchronize(s));
   *count = h;
   return SG_OK;
 }

@@ -642,6 +642,23 @@ struct QueueSet {
   int grid_tiles() const { return (int)nblocks; }
 };
 
+/* The queue set kept with the graph for CC, multi-source BFS and batched BC
+ * (default tuning), created on first use and freed with the graph. */
+inline int graph_queue_set(sg_graph_t g, QueueSet** out) {
+  if (!g->aux_queues) {
+    auto* qs = new QueueSet();
+    const int st = qs->init(g->dev, g->n, g->m, default_tuning());
+    if (st != SG_OK) {
+      qs->release();
+      delete qs;
+      return st;
+    }
+    g->aux_queues = qs;
+  }
+  *out = static_cast<QueueSet*>(g->aux_queues);
+  return SG_OK;
+}
+
 template<typename OffT>
 inline CompactArgs<OffT> make_compact_args(QueueSet& qs, int dst, const OffT* off, const uint64_t* visited,
                                            uint64_t* prev, uint64_t* frontier, const Tuning& t) {
