@@ -46,9 +46,12 @@ def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
 def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, learned=None):
     b = sg.BFS(G)
     edges_total = 0
-    per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0], "burst": [0.0, 0.0, 0]}  # -> [bytes, ms, launches]
+    # -> [bytes, ms, launches]; "step rest": the count / scan / write launches
+    # of each step and the bursts that run inside them (timed together)
+    per_kernel = {0: [0.0, 0.0, 0], 1: [0.0, 0.0, 0], "burst": [0.0, 0.0, 0], "step rest": [0.0, 0.0, 0]}
     w0_bytes = ((G.n + 63) // 64) * 8
     kernel_ms_total = 0.0
+    per_trav = []  # device-time GTEPS of each timed traversal
     for _ in range(warmup):
         for s in sources:
             b.init(s)
@@ -61,17 +64,22 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
             st = b.run(policy, alpha, beta, learned=learned)
             edges_total += st.reached_edges
             kernel_ms_total += st.kernel_ms
+            if st.kernel_ms > 0:
+                per_trav.append(st.reached_edges / (st.kernel_ms / 1e3) / 1e9)
             if collect:
                 for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
                     ls = st.level[L]
-                    # levels run inside a burst (burst_kernel) are not level_kernel launches
+                    # levels run inside a burst are not level_kernel launches
                     k = per_kernel["burst" if ls.burst else ls.direction]
                     k[0] += level_bytes(ls, w0_bytes)
                     k[1] += ls.kernel_ms
                     k[2] += 1
+                    if not ls.burst:
+                        per_kernel["step rest"][1] += ls.convert_ms
+                        per_kernel["step rest"][2] += 1
     G.dev.synchronize()
     wall = time.perf_counter() - t0
-    return edges_total, wall, kernel_ms_total, per_kernel
+    return edges_total, wall, kernel_ms_total, per_kernel, per_trav
 
 
 def copy_ceiling(sg, dev, mib: int = 1024, reps: int = 5):
@@ -306,7 +314,7 @@ def main():
                           "unit": "GTEPS", "steps": a.steps, "delta": a.delta}), flush=True)
         return
 
-    edges, wall, kms, per_kernel = run_policy(sg, G, sources, pol, 15.0, 24.0, a.steps, a.warmup, True)
+    edges, wall, kms, per_kernel, per_trav = run_policy(sg, G, sources, pol, 15.0, 24.0, a.steps, a.warmup, True)
     gteps = edges / wall / 1e9
     gteps_kernel = edges / (kms / 1e3) / 1e9 if kms > 0 else None
     log(f"{a.policy}: {gteps:.2f} GTEPS wall, {gteps_kernel:.2f} GTEPS device, {wall / a.steps * 1e3:.2f} ms/step")
@@ -323,12 +331,15 @@ def main():
             "kernel": "level_kernel",
             "launches": launches, "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
             "alg_bytes_per_launch": round(byts / max(launches, 1))}
-    for k, name in {0: "level_kernel, push levels", 1: "level_kernel, pull levels",
-                    "burst": "burst_kernel, levels inside bursts"}.items():
+    for k, name in {0: "level_kernel, push levels", 1: "level_kernel, pull levels"}.items():
         if per_kernel[k][2]:
             b_, m_, n_ = per_kernel[k]
             log(f"  {name}: {n_} levels, avg {m_ / n_ * 1e3:.1f} us, "
                 f"{b_ / (m_ / 1e3) / 1e9:.0f} GB/s algorithmic")
+    rest, burst = per_kernel["step rest"], per_kernel["burst"]
+    if rest[2]:
+        log(f"  count / scan / write passes and bursts: {rest[2]} steps, avg {rest[1] / rest[2] * 1e3:.1f} us; "
+            f"{burst[2]} levels ran inside bursts")
 
     extra = {}
     cc = copy_ceiling(sg, dev)
@@ -351,13 +362,13 @@ def main():
         for name, p in [("push", sg.POLICY_PUSH), ("pull", sg.POLICY_PULL), ("hybrid", sg.POLICY_HYBRID)]:
             if name == a.policy:
                 continue
-            e2, w2, k2, _ = run_policy(sg, G, sources, p, 15.0, 24.0, 1, 0, False)
+            e2, w2, k2, _, _ = run_policy(sg, G, sources, p, 15.0, 24.0, 1, 0, False)
             extra[f"gteps_{name}"] = round(e2 / w2 / 1e9, 3)
             log(f"{name}: {e2 / w2 / 1e9:.2f} GTEPS wall")
         weights = os.path.join(ROOT, "sygraph_amd", "data", "bfs_policy.f32")
         if os.path.exists(weights):
             pol = sg.Policy(dev, np.fromfile(weights, dtype=np.float32))
-            e2, w2, _, _ = run_policy(sg, G, sources, sg.POLICY_LEARNED, 15.0, 24.0, a.steps, 1, False, learned=pol)
+            e2, w2, _, _, _ = run_policy(sg, G, sources, sg.POLICY_LEARNED, 15.0, 24.0, a.steps, 1, False, learned=pol)
             extra["gteps_learned"] = round(e2 / w2 / 1e9, 3)
             log(f"learned: {e2 / w2 / 1e9:.2f} GTEPS wall (sygraph_amd/data/bfs_policy.f32)")
 
@@ -381,7 +392,13 @@ def main():
         "config": {"workload": f"BFS {a.policy} (alpha=15, beta=24) from 16 sources, RMAT scale {a.scale} edge factor "
                                f"16 a/b/c/d=.57/.19/.19/.05 seed 2, int64 offsets, int32 ids",
                    "vertices": G.n, "arcs": G.m, "sources": len(sources), "policy": a.policy,
-                   "gteps_device_time": round(gteps_kernel, 3) if gteps_kernel else None, **extra},
+                   "gteps_device_time": round(gteps_kernel, 3) if gteps_kernel else None,
+                   # per traversal, device time (Graph500 style: harmonic mean over the sources)
+                   "gteps_per_source": {
+                       "hmean": round(len(per_trav) / sum(1.0 / x for x in per_trav), 3) if per_trav else None,
+                       "min": round(min(per_trav), 3) if per_trav else None,
+                       "max": round(max(per_trav), 3) if per_trav else None},
+                   **extra},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -274,12 +274,13 @@ typedef struct sg_level_stat {
   int64_t max_degree;        /* largest degree among the discovered vertices */
   int32_t action;            /* direction * 3 + work-mapping variant (0 bucketing, 1 edge-balanced /
                                 short pull head, 2 wave-split / long pull head) of this level */
-  float kernel_ms;           /* device time of the level's advance kernel (record_levels) */
-  float convert_ms;          /* device time of a queue<->bitmap conversion after it */
-  int32_t burst;             /* 1: the level ran inside a single-workgroup burst of small push levels;
-                                the burst's device time is split evenly over the levels it completed
-                                (kernel_ms); the level at which a burst stops is counted and decided
-                                by the next step, whose times it carries */
+  float kernel_ms;           /* device time of the level's traversal kernel (record_levels; 0 for the
+                                levels of a burst) */
+  float convert_ms;          /* device time of what follows it in the same step: the count, scan,
+                                decision and queue write, and the burst of small levels if one ran */
+  int32_t burst;             /* 1: the level ran inside a single-workgroup burst of small push levels,
+                                timed in convert_ms of the level before it; the level at which a burst
+                                stops is counted and decided by the next step, whose times it carries */
 } sg_level_stat;
 
 typedef struct sg_bfs_stats {

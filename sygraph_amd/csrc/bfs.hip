@@ -209,27 +209,25 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, 
 }
 
 /* Device-resident control block of the level loop.  Every kernel of a step
- * reads it (kernel boundaries order the accesses); count_dyn_kernel,
- * finish_kernel and burst_kernel write it. */
+ * reads it (kernel boundaries, and the last-workgroup hand-offs inside the
+ * count and write passes, order the accesses); count_dyn_kernel (finish_level)
+ * and the bursts write it. */
 struct BfsCtl {
   int32_t level;    /* the level about to run */
   int32_t dir;      /* its direction: 0 push, 1 pull */
   int32_t done;     /* a level discovered nothing: later steps are no-ops */
-  int32_t mode;     /* count pass of this level: 0 none, 1 visited & ~prev, 2 frontier bitmap + prev sync */
   int32_t write;    /* queues for the next (push) level must be written */
-  int32_t next_act; /* decided after a pull level by count_dyn_kernel: direction * 3 + variant */
   int32_t lb;       /* work-mapping variant of the level about to run */
-  int32_t burst;    /* the next level is small enough for burst_kernel */
-  int32_t skip;     /* burst_kernel ran this level: the next step only counts and decides it */
+  int32_t burst;    /* the next level is small enough for a burst */
+  int32_t skip;     /* a burst ran this level: the next step only counts and decides it */
   int32_t n_push, n_pull;
   int64_t f_vertices, f_edges; /* frontier entering the level */
   unsigned long long visited_deg, reached;
-  unsigned long long disc, disc_deg, edges, scanned, misses, max_deg; /* staged pull-level statistics */
 };
 
 /* One level's record, written into pinned host memory (levels below
- * SG_MAX_LEVELS_RECORDED) by finish_kernel or, inside a burst, by
- * burst_kernel. */
+ * SG_MAX_LEVELS_RECORDED) by finish_level or, inside a burst, by
+ * burst_body. */
 struct alignas(64) LevelLog {
   int32_t level, dir, done, next_dir, action, burst;
   int64_t f_vertices, f_edges, disc, disc_deg, edges, scanned, misses, max_deg, unexplored;
@@ -254,7 +252,7 @@ struct DecideParams {
   int32_t lb_fixed;     /* variant under the push / pull / hybrid policies */
   const float* mlp;     /* learned policy parameters (SG_POLICY_LEARNED) */
   float f6, f7;         /* the graph's constant features (average degree, vertex count) */
-  int64_t burst_arcs;   /* frontier arcs up to which push levels run in burst_kernel (0: never) */
+  int64_t burst_arcs;   /* frontier arcs up to which push levels run in a burst (0: never) */
 };
 
 /* evaluateHeuristic (algorithms/bfs.hpp:341-350): push -> pull when the new
@@ -330,65 +328,77 @@ __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const Bf
   else pull_level(a, ctl->lb, level);
 }
 
-/* Count pass of the level.  After a push it counts visited & ~prev.  After a
- * pull the next action must be known first (queues are only built when the
- * next level pushes): wave 0 of every workgroup reduces the pull's 64 stat
- * slots and decides -- the same inputs, so the same answer everywhere, with no
- * extra launch -- and workgroup 0 records the decision for finish_kernel. */
-template<typename OffT>
-__global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, BfsCtl* ctl, const StatSlot* slots,
-                                                           DecideParams p) {
-  __shared__ int32_t s_mode;
-  __shared__ sg::policy::Staged s_pol;
-  if (ctl->done) return;
-  if (ctl->dir == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->mode = 1;
-    compact_count_body<OffT, kFromDiff>(a, 0, ctl->level + 1); /* the push left the depths to this pass */
-    return;
-  }
-  stage_policy(p, &s_pol);
+/* True in exactly one workgroup of the grid, the last to get here; it also
+ * resets the counter for the next launch.  All threads call.  What the last
+ * workgroup may then read of the grid's earlier writes:
+ *  - kCoherent: what was written with agent-coherent stores (every wave
+ *    waits for its stores to complete before the arrival is counted), read
+ *    with agent-coherent loads; nothing is written back or invalidated;
+ *  - otherwise: the writes of threads that pass `publish` (they release at
+ *    agent scope, an L2 write-back, so only threads with data for the last
+ *    workgroup should), read with ordinary loads after the last workgroup's
+ *    acquire (an L2 invalidate). */
+constexpr int kArriveGroups = 8;  /* workgroups are dealt to the 8 XCDs round robin */
+constexpr int kArriveStride = 16; /* counters 64 B apart */
+constexpr int kArriveWords = (kArriveGroups + 1) * kArriveStride;
+
+/* Arrivals are counted in two stages, per group of workgroups (blockIdx.x %
+ * kArriveGroups: one counter per XCD) and then per group, so no single
+ * counter takes more than 1/8 of the grid's atomics.  `ctr` holds
+ * kArriveWords zeroed words. */
+template<bool kCoherent>
+__device__ __forceinline__ bool last_to_arrive(uint32_t* ctr, bool publish) {
+  __shared__ uint32_t s_last;
+  if (kCoherent) wait_stores_complete();
+  else if (publish) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
-  const int32_t level = ctl->level;
-  if (threadIdx.x < 64) {
-    const StatSlot& x = slots[threadIdx.x];
-    const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
-    const unsigned long long max_deg = wave_max_u64(x.max_deg);
-    const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
-    const int32_t act = decide_action(p, &s_pol, 1, disc, disc_deg, unexplored, level + 1, max_deg);
-    const int32_t mode = (disc > 0 && act / 3 == 0) ? 2 : 0;
-    const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
-    const unsigned long long misses = wave_sum_u64(x.misses);
-    if (threadIdx.x == 0) {
-      s_mode = mode;
-      if (blockIdx.x == 0) {
-        ctl->disc = disc;
-        ctl->disc_deg = disc_deg;
-        ctl->edges = edges;
-        ctl->scanned = scanned;
-        ctl->misses = misses;
-        ctl->max_deg = max_deg;
-        ctl->next_act = act;
-        ctl->mode = mode;
-      }
+  if (threadIdx.x == 0) {
+    const uint32_t grid = gridDim.x, g = blockIdx.x % kArriveGroups;
+    const uint32_t ngroups = grid < (uint32_t)kArriveGroups ? grid : (uint32_t)kArriveGroups;
+    const uint32_t in_group = (grid - g + kArriveGroups - 1) / kArriveGroups;
+    uint32_t* gc = ctr + (1 + g) * kArriveStride;
+    uint32_t last = 0;
+    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1) {
+      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1 ? 1u : 0u;
+      if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    s_last = last;
   }
   __syncthreads();
-  if (s_mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
+  const bool last = s_last != 0;
+  if (!kCoherent && last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return last;
 }
+
+/* Totals of a pull level's 64 stat slots and the decision taken on them. */
+struct PullTotals {
+  unsigned long long disc, disc_deg, edges, scanned, misses, max_deg;
+  int32_t act;
+};
+
+/* Where the last workgroup of a step's count pass writes its results. */
+struct FinishArgs {
+  TileCount* tiles; /* the count pass's per-workgroup sums, scanned in place */
+  int64_t ntiles;
+  QueueHdr* hdr_out;
+  int32_t step;
+  LevelLog* levlog;
+  StepLog* steplog;
+  uint32_t* arrive; /* zero between launches */
+};
 
 /* Scan of the count pass, the push-level decision (wave 0, so the learned
  * policy can run on it), the level record, and the control block for the
- * next level.  Unless burst_kernel takes over the next levels, this is the
- * step's last working kernel and writes the step record. */
-__global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, int64_t ntiles, QueueHdr* hdr_out,
-                                                              BfsCtl* ctl, DecideParams p, int32_t step,
-                                                              LevelLog* levlog, StepLog* steplog) {
+ * next level.  Unless a burst takes over the next levels, this ends the
+ * step's work and writes the step record.  Run by the last workgroup of the
+ * count pass; all kBlock threads call.  `mode` is the count pass's mode (0:
+ * nothing counted, the next level pulls or the traversal ends). */
+__device__ __forceinline__ void finish_level(const FinishArgs& f, BfsCtl* ctl, const DecideParams& p,
+                                             const sg::policy::Staged* s_pol, int32_t mode, const PullTotals* pt) {
   __shared__ QueueHdr tot;
-  __shared__ sg::policy::Staged s_pol;
-  if (ctl->done) return;
-  const int32_t mode = ctl->mode;
-  if (ctl->dir == 0) stage_policy(p, &s_pol);
-  if (mode != 0) compact_scan_body(tiles, ntiles, hdr_out, &tot);
+  /* the workgroup sums were written with agent-coherent stores during this launch */
+  if (mode != 0) compact_scan_body_t<kBlock, 8, true>(f.tiles, f.ntiles, f.hdr_out, &tot);
   __syncthreads();
   if (threadIdx.x >= 64) return;
   const int32_t dir = ctl->dir, level = ctl->level;
@@ -402,16 +412,16 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     edges = (unsigned long long)ctl->f_edges;
     scanned = (unsigned long long)ctl->f_vertices;
     misses = 0;
-    act = decide_action(p, &s_pol, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1,
+    act = decide_action(p, s_pol, 0, disc, disc_deg, p.total_degree - (vdeg_before + disc_deg), level + 1,
                         max_deg);
   } else {
-    disc = ctl->disc;
-    disc_deg = ctl->disc_deg;
-    max_deg = ctl->max_deg;
-    edges = ctl->edges;
-    scanned = ctl->scanned;
-    misses = ctl->misses;
-    act = ctl->next_act;
+    disc = pt->disc;
+    disc_deg = pt->disc_deg;
+    max_deg = pt->max_deg;
+    edges = pt->edges;
+    scanned = pt->scanned;
+    misses = pt->misses;
+    act = pt->act;
   }
   if (threadIdx.x != 0) return;
   const int32_t next = act / 3;
@@ -434,7 +444,7 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
     e.misses = (int64_t)misses;
     e.max_deg = (int64_t)max_deg;
     e.unexplored = (int64_t)(p.total_degree - vdeg);
-    levlog[level] = e;
+    f.levlog[level] = e;
   }
   const int32_t n_push = ctl->n_push + (dir == 0 ? 1 : 0), n_pull = ctl->n_pull + (dir == 0 ? 0 : 1);
   const unsigned long long reached = ctl->reached + disc;
@@ -454,29 +464,69 @@ __global__ void __launch_bounds__(kScanThreads) finish_kernel(TileCount* tiles, 
   ctl->skip = 0;
   if (!burst) {
     StepLog sl;
-    sl.step = step;
+    sl.step = f.step;
     sl.last_level = level;
     sl.done = done;
     sl.n_push = n_push;
     sl.n_pull = n_pull;
     sl.reached_v = (int64_t)reached;
     sl.reached_e = (int64_t)vdeg;
-    steplog[step % kRing] = sl;
+    f.steplog[f.step % kRing] = sl;
   }
   __threadfence_system();
 }
 
+/* Count pass of the level.  After a push it counts visited & ~prev.  After a
+ * pull the next action must be known first (queues are only built when the
+ * next level pushes): wave 0 of every workgroup reduces the pull's 64 stat
+ * slots and decides -- the same inputs, so the same answer everywhere, with no
+ * extra launch.  The last workgroup to finish then scans, decides (after a
+ * push) and updates the control block (finish_level). */
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) write_dyn_kernel(CompactArgs<OffT> a, const BfsCtl* ctl) {
-  if (!ctl->write) return;
-  compact_write_body<OffT>(a);
+__global__ void __launch_bounds__(kBlock) count_dyn_kernel(CompactArgs<OffT> a, BfsCtl* ctl, const StatSlot* slots,
+                                                           DecideParams p, FinishArgs f) {
+  __shared__ int32_t s_mode;
+  __shared__ sg::policy::Staged s_pol;
+  __shared__ PullTotals s_pt;
+  if (ctl->done) return;
+  const int32_t dir = ctl->dir, level = ctl->level;
+  if (dir == 0) {
+    compact_count_body<OffT, kFromDiff>(a, 0, level + 1); /* the push left the depths to this pass */
+    if (!last_to_arrive<true>(f.arrive, false)) return;
+    stage_policy(p, &s_pol);
+    __syncthreads();
+    finish_level(f, ctl, p, &s_pol, 1, nullptr);
+    return;
+  }
+  stage_policy(p, &s_pol);
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const StatSlot& x = slots[threadIdx.x];
+    const unsigned long long disc = wave_sum_u64(x.disc), disc_deg = wave_sum_u64(x.disc_deg);
+    const unsigned long long max_deg = wave_max_u64(x.max_deg);
+    const unsigned long long unexplored = p.total_degree - (ctl->visited_deg + disc_deg);
+    const int32_t act = decide_action(p, &s_pol, 1, disc, disc_deg, unexplored, level + 1, max_deg);
+    const int32_t mode = (disc > 0 && act / 3 == 0) ? 2 : 0;
+    const unsigned long long edges = wave_sum_u64(x.edges), scanned = wave_sum_u64(x.scanned);
+    const unsigned long long misses = wave_sum_u64(x.misses);
+    if (threadIdx.x == 0) {
+      s_mode = mode;
+      s_pt = PullTotals{disc, disc_deg, edges, scanned, misses, max_deg, act};
+    }
+  }
+  __syncthreads();
+  const int32_t mode = s_mode;
+  if (mode == 2) compact_count_body<OffT, kFromBitmapSyncPrev>(a);
+  if (!last_to_arrive<true>(f.arrive, false)) return;
+  finish_level(f, ctl, p, &s_pol, mode, &s_pt);
 }
 
 /* ------------------------------------------------------------ bursts --- */
 /* Small push levels are bound by launch and dispatch cost, not bandwidth: a
- * level with a few thousand frontier arcs takes four grid-wide launches
- * (about 40 us) for microseconds of memory work.  burst_kernel, one
- * 512-thread workgroup, runs such levels back to back in one launch with
+ * level with a few thousand frontier arcs takes three grid-wide launches
+ * (about 25 us) for microseconds of memory work.  A burst, one 512-thread
+ * workgroup (the last of the write pass, or burst_kernel before the first
+ * step), runs such levels back to back with
  * workgroup barriers between them, for as long as the policy keeps pushing and
  * the frontier keeps at most burst_arcs arcs.  The frontier lives in two
  * lists of (first arc, vertex, degree) entries; each wave flattens 64 entries
@@ -649,12 +699,11 @@ __device__ __forceinline__ void burst_visit_big(const BurstArgs<OffT>& a, const 
  * still read this level's sums; every wave evaluates the (deterministic)
  * decision itself. */
 template<typename OffT>
-__global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a, BfsCtl* ctl, DecideParams p,
-                                                              int32_t step, LevelLog* levlog, StepLog* steplog) {
+__device__ __forceinline__ void burst_body(const BurstArgs<OffT>& a, BfsCtl* ctl, const DecideParams& p,
+                                           int32_t step, LevelLog* levlog, StepLog* steplog) {
   __shared__ sg::policy::Staged s_pol;
-  __shared__ uint32_t s_count[2][2], s_cls[3];
+  __shared__ uint32_t s_count[2][2];
   __shared__ unsigned long long s_red[2][kBurstWaves][5];
-  if (ctl->done || !ctl->burst) return;
   stage_policy(p, &s_pol);
   if (threadIdx.x == 0) s_count[0][0] = s_count[0][1] = 0;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -775,6 +824,34 @@ __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a,
   }
 }
 
+/* A burst from the source, before the first step (all kBurstThreads threads
+ * of one workgroup); returns at once unless the control block asks for one. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a, BfsCtl* ctl, DecideParams p,
+                                                              int32_t step, LevelLog* levlog, StepLog* steplog) {
+  if (ctl->done || !ctl->burst) return;
+  burst_body(a, ctl, p, step, levlog, steplog);
+}
+
+/* Queue write of a level that the next one pushes.  A workgroup of
+ * kBurstThreads threads writes the tiles of two count-pass workgroups; when
+ * the control block asks for a burst, the last workgroup to finish runs it on
+ * the queues the grid has just written. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBurstThreads) write_dyn_kernel(CompactArgs<OffT> a, BurstArgs<OffT> ba,
+                                                                  BfsCtl* ctl, DecideParams p, int32_t step,
+                                                                  LevelLog* levlog, StepLog* steplog,
+                                                                  uint32_t* arrive) {
+  static_assert(kBurstThreads == 2 * kBlock, "a write workgroup covers two count-pass workgroups");
+  if (!ctl->write) return;
+  const bool burst = ctl->burst != 0;
+  const bool wrote =
+      compact_write_body_at<OffT>(a, 0, (int64_t)blockIdx.x * 2 + (threadIdx.x >> 8), (int)(threadIdx.x & (kBlock - 1)));
+  /* a burst level has at most burst_arcs arcs, so few waves write entries and publish them */
+  if (!burst || !last_to_arrive<false>(arrive, wrote)) return;
+  burst_body(ba, ctl, p, step, levlog, steplog);
+}
+
 __global__ void copy_words_kernel(uint64_t* dst, uint64_t* dst2, const uint64_t* src, int64_t nw) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t x = src[i];
@@ -797,10 +874,12 @@ struct BfsWorkspace {
   StatSlot* stats = nullptr; /* 2 areas of kSlots */
   BfsCtl* ctl = nullptr;
   QEntry* burst_list[4] = {nullptr, nullptr, nullptr, nullptr}; /* kBurstMaxArcs entries each */
+  uint32_t* arrive = nullptr; /* arrival counters of the count and write passes (kArriveWords each), zero between launches */
   LevelLog* levlog = nullptr; /* SG_MAX_LEVELS_RECORDED entries, pinned coherent host memory */
   StepLog* steplog = nullptr; /* kRing entries, pinned coherent host memory */
-  hipEvent_t ev_start[kRing] = {}, ev_mid[kRing] = {}, ev_burst[kRing] = {}, ev_done[kRing] = {};
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  /* per step: after its traversal kernel (record_levels only) and at its end */
+  hipEvent_t ev_mid[kRing] = {}, ev_done[kRing] = {};
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_first = nullptr; /* ev_first: before step 0 */
 };
 
 void ws_free(BfsWorkspace* w) {
@@ -813,13 +892,14 @@ void ws_free(BfsWorkspace* w) {
   (void)hipFree(w->stats);
   (void)hipFree(w->ctl);
   for (QEntry* l : w->burst_list) (void)hipFree(l);
+  (void)hipFree(w->arrive);
   if (w->levlog) (void)hipHostFree(w->levlog);
   if (w->steplog) (void)hipHostFree(w->steplog);
   for (int r = 0; r < kRing; ++r)
-    for (hipEvent_t e : {w->ev_start[r], w->ev_mid[r], w->ev_burst[r], w->ev_done[r]})
+    for (hipEvent_t e : {w->ev_mid[r], w->ev_done[r]})
       if (e) (void)hipEventDestroy(e);
-  if (w->ev_begin) (void)hipEventDestroy(w->ev_begin);
-  if (w->ev_end) (void)hipEventDestroy(w->ev_end);
+  for (hipEvent_t e : {w->ev_begin, w->ev_end, w->ev_first})
+    if (e) (void)hipEventDestroy(e);
   delete w;
 }
 
@@ -842,6 +922,8 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
       (st = sg::alloc_device(dev, (void**)&w->burst_list[1], sizeof(QEntry) * kBurstMaxArcs)) ||
       (st = sg::alloc_device(dev, (void**)&w->burst_list[2], sizeof(QEntry) * kBurstMaxArcs)) ||
       (st = sg::alloc_device(dev, (void**)&w->burst_list[3], sizeof(QEntry) * kBurstMaxArcs)) ||
+      (st = sg::alloc_device(dev, (void**)&w->arrive, 8 * kArriveWords)) ||
+      (st = sg::zero_async(dev, w->arrive, 8 * kArriveWords)) ||
       (st = w->qs.init(dev, g->n, g->m, default_tuning()))) {
     ws_free(w);
     return st;
@@ -850,9 +932,9 @@ int workspace_for(sg_graph_t g, BfsWorkspace** out) {
                 hipSuccess &&
             hipHostMalloc((void**)&w->steplog, sizeof(StepLog) * kRing, hipHostMallocCoherent) == hipSuccess;
   for (int r = 0; r < kRing && ok; ++r)
-    ok = hipEventCreate(&w->ev_start[r]) == hipSuccess && hipEventCreate(&w->ev_mid[r]) == hipSuccess &&
-         hipEventCreate(&w->ev_burst[r]) == hipSuccess && hipEventCreate(&w->ev_done[r]) == hipSuccess;
-  ok = ok && hipEventCreate(&w->ev_begin) == hipSuccess && hipEventCreate(&w->ev_end) == hipSuccess;
+    ok = hipEventCreate(&w->ev_mid[r]) == hipSuccess && hipEventCreate(&w->ev_done[r]) == hipSuccess;
+  ok = ok && hipEventCreate(&w->ev_begin) == hipSuccess && hipEventCreate(&w->ev_end) == hipSuccess &&
+       hipEventCreate(&w->ev_first) == hipSuccess;
   if (!ok) {
     ws_free(w);
     sg::set_error("bfs workspace: event or pinned log allocation failed");
@@ -875,8 +957,9 @@ void bfs_workspace_release(sg_graph_t g) {
 }
 } // namespace sg
 
-/* Host side of the level loop.  Each step is a fixed sequence of five
- * launches (traverse, decide+count, scan+finish, write, burst) whose
+/* Host side of the level loop.  Each step is a fixed sequence of three
+ * launches (traverse; decide + count, then scan + finish in the last
+ * workgroup; write, then a burst in the last workgroup) whose
  * behaviour the device-resident control block selects, so the host never
  * has to decide anything between levels: it keeps `lookahead` steps enqueued
  * beyond the one it waits for and only reads each step's record (pinned host
@@ -974,15 +1057,15 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     CompactArgs<OffT> ca = make_compact_args<OffT>(Q, nxt, off, ws->visited, ws->prev, ws->bm[nxt], tune);
     ca.depth = depth; /* depths of the vertices a push level discovered */
     const BurstArgs<OffT> ba = burst_args(nxt);
-    if (timed) SG_HIP(hipEventRecord(ws->ev_start[r], s));
+    /* timing: the traversal kernel runs from the previous step's end (or
+     * ev_first) to ev_mid; each event is a stream barrier, so only these */
+    if (timed && step == 0) SG_HIP(hipEventRecord(ws->ev_first, s));
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
-    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P);
-    finish_kernel<<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr + nxt, ws->ctl, P, step, ws->levlog,
-                                             ws->steplog);
-    write_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
-    if (timed) SG_HIP(hipEventRecord(ws->ev_burst[r], s));
-    burst_kernel<OffT><<<1, kBurstThreads, 0, s>>>(ba, ws->ctl, P, step, ws->levlog, ws->steplog);
+    FinishArgs fa{Q.blocks, Q.nblocks, Q.hdr + nxt, step, ws->levlog, ws->steplog, ws->arrive};
+    count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P, fa);
+    write_dyn_kernel<OffT><<<(Q.grid_tiles() + 1) / 2, kBurstThreads, 0, s>>>(ca, ba, ws->ctl, P, step, ws->levlog,
+                                                                             ws->steplog, ws->arrive + kArriveWords);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev_done[r], s));
     return SG_OK;
@@ -1008,13 +1091,12 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       return SG_ERR_STATE;
     }
     if (stats) {
-      float k_ms = 0.f, c_ms = 0.f, b_ms = 0.f;
+      float k_ms = 0.f, c_ms = 0.f;
       if (timed) {
-        SG_HIP(hipEventElapsedTime(&k_ms, ws->ev_start[r], ws->ev_mid[r]));
-        SG_HIP(hipEventElapsedTime(&c_ms, ws->ev_mid[r], ws->ev_burst[r]));
-        SG_HIP(hipEventElapsedTime(&b_ms, ws->ev_burst[r], ws->ev_done[r]));
+        const hipEvent_t start = step == 0 ? ws->ev_first : ws->ev_done[(step + kRing - 1) % kRing];
+        SG_HIP(hipEventElapsedTime(&k_ms, start, ws->ev_mid[r]));
+        SG_HIP(hipEventElapsedTime(&c_ms, ws->ev_mid[r], ws->ev_done[r]));
       }
-      const int32_t nb = last.last_level - first_level; /* levels of the burst */
       for (int32_t L = first_level; L <= last.last_level && L < SG_MAX_LEVELS_RECORDED; ++L) {
         LevelLog e;
         std::memcpy(&e, (const void*)&ws->levlog[L], sizeof(e));
@@ -1032,11 +1114,9 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
         ls.unexplored_edges = e.unexplored;
         ls.action = e.action;
         ls.burst = e.burst;
-        if (L == first_level) {
+        if (L == first_level) { /* the step's later levels ran in a burst, timed with the count and write */
           ls.kernel_ms = k_ms;
           ls.convert_ms = c_ms;
-        } else {
-          ls.kernel_ms = b_ms / (float)nb;
         }
       }
     }

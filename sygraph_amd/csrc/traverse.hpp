@@ -46,6 +46,36 @@ struct alignas(32) TileCount {
   unsigned long long maxdeg;
 };
 
+/* Agent-coherent stores and loads: relaxed atomics at agent scope, which
+ * write through to / read from the point of coherence that every XCD's L2
+ * shares, so one workgroup can hand data to another within a launch without
+ * an L2 write-back or invalidate. */
+__device__ __forceinline__ void store_coherent(TileCount* p, const TileCount& t) {
+  uint64_t w[4];
+  __builtin_memcpy(w, &t, sizeof(w));
+  uint64_t* q = reinterpret_cast<uint64_t*>(p);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) __hip_atomic_store(q + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ TileCount load_coherent(const TileCount* p) {
+  uint64_t w[4];
+  const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  TileCount t;
+  __builtin_memcpy(&t, w, sizeof(w));
+  return t;
+}
+
+/* Wait until this wave's stores have completed (s_waitcnt vmcnt(0)); the
+ * signal fences keep the compiler from moving memory operations across. */
+__device__ __forceinline__ void wait_stores_complete() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70); /* gfx9 encoding: vmcnt 0; expcnt 7 and lgkmcnt 15 are no-waits */
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 struct QueueHdr {
   uint32_t count[3];
   uint32_t pad;
@@ -271,7 +301,7 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, i
       b.deg += s_wave[w].deg;
       b.maxdeg = s_wave[w].maxdeg > b.maxdeg ? s_wave[w].maxdeg : b.maxdeg;
     }
-    a.blocks[blockIdx.x] = b;
+    store_coherent(&a.blocks[blockIdx.x], b); /* may be scanned by the last workgroup of this launch */
   }
 }
 
@@ -289,22 +319,23 @@ __global__ void __launch_bounds__(kBlock) compact_count_kernel(CompactArgs<OffT>
 constexpr int kScanThreads = 1024;
 constexpr int kScanPer = 2;
 
-/* Must be called by all kScanThreads threads; thread 0 gets the totals in
- * *out (also stored to *hdr). */
-__device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntiles, QueueHdr* hdr, QueueHdr* out) {
-  __shared__ uint32_t wave_tot[kScanThreads / 64][3];
-  __shared__ unsigned long long red[kScanThreads / 64][3];
+/* Must be called by all kThreads threads of the workgroup; thread 0 gets the
+ * totals in *out (also stored to *hdr).  kThreads * kPer entries per pass. */
+template<int kThreads, int kPer, bool kCoherent = false>
+__device__ __forceinline__ void compact_scan_body_t(TileCount* tiles, int64_t ntiles, QueueHdr* hdr, QueueHdr* out) {
+  __shared__ uint32_t wave_tot[kThreads / 64][3];
+  __shared__ unsigned long long red[kThreads / 64][3];
   uint32_t carry[3] = {0, 0, 0};
   unsigned long long nv = 0, deg = 0, mx = 0;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (int64_t base = 0; base < ntiles; base += (int64_t)kScanThreads * kScanPer) {
-    const int64_t first = base + (int64_t)threadIdx.x * kScanPer;
-    uint32_t loc[kScanPer][3];
+  for (int64_t base = 0; base < ntiles; base += (int64_t)kThreads * kPer) {
+    const int64_t first = base + (int64_t)threadIdx.x * kPer;
+    uint32_t loc[kPer][3];
     uint32_t sum[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < kScanPer; ++k) {
+    for (int k = 0; k < kPer; ++k) {
       TileCount t{};
-      if (first + k < ntiles) t = tiles[first + k];
+      if (first + k < ntiles) t = kCoherent ? load_coherent(&tiles[first + k]) : tiles[first + k];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         loc[k][c] = t.c[c];
@@ -324,13 +355,13 @@ __device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntil
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       uint32_t before = 0, tot = 0;
-      for (int w = 0; w < kScanThreads / 64; ++w) {
+      for (int w = 0; w < kThreads / 64; ++w) {
         if (w < wv) before += wave_tot[w][c];
         tot += wave_tot[w][c];
       }
       uint32_t run = carry[c] + before + incl[c] - sum[c];
 #pragma unroll
-      for (int k = 0; k < kScanPer; ++k) {
+      for (int k = 0; k < kPer; ++k) {
         if (first + k < ntiles) tiles[first + k].c[c] = run;
         run += loc[k][c];
       }
@@ -352,7 +383,7 @@ __device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntil
     h.count[0] = carry[0];
     h.count[1] = carry[1];
     h.count[2] = carry[2];
-    for (int w = 0; w < kScanThreads / 64; ++w) {
+    for (int w = 0; w < kThreads / 64; ++w) {
       h.nvert += red[w][0];
       h.deg += red[w][1];
       h.maxdeg = red[w][2] > h.maxdeg ? red[w][2] : h.maxdeg;
@@ -362,30 +393,38 @@ __device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntil
   }
 }
 
+__device__ __forceinline__ void compact_scan_body(TileCount* tiles, int64_t ntiles, QueueHdr* hdr, QueueHdr* out) {
+  compact_scan_body_t<kScanThreads, kScanPer>(tiles, ntiles, hdr, out);
+}
+
 static __global__ void __launch_bounds__(kScanThreads) compact_scan_kernel(TileCount* tiles, int64_t ntiles,
                                                                            QueueHdr* hdr) {
   compact_scan_body(tiles, ntiles, hdr, nullptr);
 }
 
-/* pass 3: emit the class queues in vertex order at the scanned offsets */
+/* pass 3: emit the class queues in vertex order at the scanned offsets.
+ * `blk` and `tid` are the workgroup and thread of the count pass's layout
+ * (kBlock threads per workgroup), so a wider workgroup can act as several.
+ * Returns whether the wave wrote any entry. */
 template<typename OffT>
-__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, int phase = 0) {
-  const int64_t tile = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (tile >= a.ntiles) return;
+__device__ __forceinline__ bool compact_write_body_at(const CompactArgs<OffT>& a, int phase, int64_t blk, int tid) {
+  const int64_t tile = (blk * kBlock + tid) >> 6;
+  if (tile >= a.ntiles) return false;
   const int lane = lane_id();
   const int64_t myw = tile * 64 + lane;
   const uint64_t mynew = myw < a.w0 ? a.frontier[myw] : 0ull;
   /* this tile's offsets: its workgroup's exclusive offsets plus the counts of
    * the workgroup's earlier tiles */
-  const TileCount bc = a.blocks[blockIdx.x];
+  const TileCount bc = a.blocks[blk];
   uint32_t pos0 = bc.c[0], pos1 = bc.c[1], pos2 = bc.c[2];
-  for (int64_t k = (int64_t)blockIdx.x * (kBlock / 64); k < tile; ++k) {
+  for (int64_t k = blk * (kBlock / 64); k < tile; ++k) {
     const TileCount tc = a.tiles[k];
     pos0 += tc.c[0];
     pos1 += tc.c[1];
     pos2 += tc.c[2];
   }
   uint64_t todo = ballot(mynew != 0);
+  const bool wrote = todo != 0;
   while (todo) {
     const int k = __ffsll((unsigned long long)todo) - 1;
     todo &= todo - 1;
@@ -419,6 +458,12 @@ __device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, i
     }
     pos2 += __shfl(incl, 63, 64);
   }
+  return wrote;
+}
+
+template<typename OffT>
+__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, int phase = 0) {
+  compact_write_body_at(a, phase, (int64_t)blockIdx.x, (int)threadIdx.x);
 }
 
 template<typename OffT>
