@@ -341,6 +341,12 @@ typedef struct sg_sssp_options {
   double delta;          /* bucket width of the near/far split: 0 = automatic (half the mean arc
                             weight), negative or infinite = none, i.e. every dirty vertex is relaxed
                             each round (the reference's frontier Bellman-Ford, sssp.hpp:152-210) */
+  int32_t heavy_pull;    /* direction of a bucket's heavy round on an undirected graph: 0 = automatic
+                            (pull when the heavy arcs of the vertices not yet settled are fewer than
+                            those of the bucket's settled set), 1 = always pull, -1 = always push.
+                            Pull: every vertex not yet settled takes the minimum over its heavy arcs
+                            from the settled set; push: every heavy arc of the settled set is
+                            relaxed with an atomic minimum.  Directed graphs always push. */
 } sg_sssp_options;
 void sg_sssp_options_default(sg_sssp_options* opt);
 /* sg_sssp with options; the distances do not depend on delta. */

@@ -87,7 +87,7 @@ class SsspStats(C.Structure):
 
 
 class SsspOptions(C.Structure):
-    _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double)]
+    _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double), ("heavy_pull", C.c_int32)]
 
 
 class BfsStats(C.Structure):
@@ -584,15 +584,17 @@ class SSSP:
     def reset(self):
         self.source = None
 
-    def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0) -> SsspStats:
+    def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0) -> SsspStats:
         """delta: bucket width of the near/far split (0 automatic, negative or
-        inf for the reference's frontier Bellman-Ford)."""
+        inf for the reference's frontier Bellman-Ford).  heavy_pull: direction
+        of a bucket's heavy round (0 automatic, 1 pull, -1 push)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
         o = SsspOptions()
         lib().sg_sssp_options_default(C.byref(o))
         o.load_balancer = load_balancer
         o.delta = float(delta)
+        o.heavy_pull = int(heavy_pull)
         check(lib().sg_sssp_ex(self.g.h, self.source, C.byref(o), C.c_void_p(self.dist.ptr), C.byref(self.stats)))
         return self.stats
 

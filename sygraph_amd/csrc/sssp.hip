@@ -57,10 +57,15 @@ struct SsspArgs {
   const W* w;
   uint32_t* dist; /* distance bits */
   uint64_t* next_bm;
-  uint64_t* zero_bm; /* frontier bitmap of the previous round, cleared here */
   int64_t w0;
   QEntry* qin[3];
   const QueueHdr* qin_hdr;
+  /* heavy rounds that pull (light-first adjacency of an undirected graph) */
+  const OffT* off;
+  const int32_t* light;      /* light arcs per vertex */
+  const uint64_t* settled;   /* every vertex settled by a finished bucket (including the current one) */
+  const uint64_t* frontier;  /* the current bucket's settled set */
+  int64_t n;
 };
 
 /* Device-resident control block of the round loop (written only by
@@ -73,8 +78,9 @@ struct SsspCtl {
   int32_t heavy;      /* the current iteration relaxes the heavy arcs of the bucket's settled set */
   int32_t heavy_next; /* the next iteration is the heavy one */
   int32_t r_pending;  /* vertices were relaxed in this bucket since its heavy iteration */
-  int32_t pad;
-  unsigned long long relaxed; /* arcs relaxed so far */
+  int32_t pull;       /* the current (heavy) iteration pulls into the vertices not yet settled */
+  unsigned long long relaxed;    /* arcs relaxed so far */
+  unsigned long long heavy_done; /* heavy arcs of every settled set so far */
 };
 
 struct alignas(64) RoundLog {
@@ -90,26 +96,114 @@ constexpr int kRing = 8;
  * from a settled bucket cannot land inside it, so it is relaxed once).  The
  * adjacency is the light-first copy, and the queue entries already cover only
  * the phase's part of each list, so every visited arc is relaxed. */
+constexpr int kPullHead = 32; /* heavy arcs a lane scans alone before the wave helps */
+
+/* Heavy round by pull (undirected graphs; in-arcs are the out-arcs and the
+ * weights are symmetric): a heavy arc (u, v) can only lower dist[v] if v is
+ * not settled yet (dist[u] >= T - delta and w >= delta give at least T), so
+ * each vertex not yet settled takes the minimum of dist[u] + w over its heavy
+ * arcs from the bucket's settled set and becomes dirty if that is below its
+ * distance.  Every such vertex belongs to one lane, so its distance is
+ * written without an atomic, and the settled distances it reads are final.
+ * A wave loads 64 words of the settled bitmap and compacts their clear bits;
+ * lists longer than kPullHead are finished by the whole wave.  Returns the
+ * arcs from the settled set that were relaxed. */
+template<typename OffT, typename W>
+__device__ __forceinline__ unsigned long long heavy_pull(const SsspArgs<OffT, W>& a) {
+  unsigned long long cnt = 0;
+  const int lane = lane_id();
+  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  const int tail_bits = (int)(a.n & 63);
+  for (int64_t base = gwave * 64; base < a.w0; base += nwaves * 64) {
+    const int64_t myw = base + lane;
+    uint64_t myun = myw < a.w0 ? ~a.settled[myw] : 0ull;
+    if (myw == a.w0 - 1 && tail_bits) myun &= (1ull << tail_bits) - 1ull;
+    const uint32_t c = (uint32_t)__popcll(myun);
+    const uint32_t incl = wave_inclusive_scan(c);
+    const uint32_t total = __shfl(incl, 63, 64);
+    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      const int owner = scan_owner(incl, t);
+      const uint32_t before = __shfl(incl - c, owner, 64);
+      const uint64_t un = shfl_u64(myun, owner);
+      const bool active = t < total;
+      int64_t v = 0, j = 0, e = 0;
+      uint32_t dv = 0, best = 0xFFFFFFFFu;
+      if (active) {
+        v = (base + owner) * 64 + select_bit(un, t - before);
+        j = (int64_t)a.off[v] + a.light[v];
+        e = (int64_t)a.off[v + 1];
+        dv = a.dist[v];
+        best = dv;
+        const int64_t he = e - j > kPullHead ? j + kPullHead : e;
+        for (; j < he; j += 4) { /* four ids, then four bitmap probes, then the hits' distances */
+          uint32_t u[4];
+          bool h[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) u[k] = j + k < he ? (uint32_t)a.cols[j + k] : 0u;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) h[k] = j + k < he && test_bit(a.frontier, u[k]);
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (h[k]) {
+              ++cnt;
+              const uint32_t nb = DistOps<W>::bits(DistOps<W>::value(a.dist[u[k]]) + a.w[j + k]);
+              best = nb < best ? nb : best;
+            }
+        }
+        if (j > he) j = he;
+      }
+      uint64_t pending = ballot(active && j < e);
+      while (pending) {
+        const int k2 = __ffsll((unsigned long long)pending) - 1;
+        pending &= pending - 1;
+        const int64_t jb = shfl_i64(j, k2), je = shfl_i64(e, k2);
+        uint32_t m = 0xFFFFFFFFu;
+        for (int64_t jj = jb + lane; jj < je; jj += 64) {
+          const uint32_t u = (uint32_t)a.cols[jj];
+          if (test_bit(a.frontier, u)) {
+            ++cnt;
+            const uint32_t nb = DistOps<W>::bits(DistOps<W>::value(a.dist[u]) + a.w[jj]);
+            m = nb < m ? nb : m;
+          }
+        }
+        m = ~(uint32_t)wave_max_u64(~(unsigned long long)m); /* wave minimum */
+        if (lane == k2) best = m < best ? m : best;
+      }
+      if (active && best < dv) {
+        a.dist[v] = best;
+        atomic_or(&a.next_bm[v >> 6], 1ull << (v & 63));
+      }
+    }
+  }
+  return cnt;
+}
+
 template<typename OffT, typename W>
 __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, const SsspCtl* ctl,
                                                        unsigned long long* relax_slots) {
   if (ctl->done || !ctl->relax) return;
   unsigned long long cnt = 0;
-  for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
-    const W wt = a.w[e];
-    ++cnt;
-    /* a stale dist[u] is an upper bound: u is dirty again if it improved */
-    const W du = DistOps<W>::value(a.dist[u]);
-    const W nd = du + wt;
-    const uint32_t nb = DistOps<W>::bits(nd);
-    if (nb < a.dist[v]) {
-      const uint32_t old = atomic_min(&a.dist[v], nb);
-      if (nb < old) {
-        const uint64_t bit = 1ull << (v & 63);
-        if (!(a.next_bm[v >> 6] & bit)) atomic_or(&a.next_bm[v >> 6], bit);
+  if (ctl->pull) {
+    cnt = heavy_pull(a);
+  } else {
+    for_each_queued_arc<OffT>(a.qin, a.qin_hdr, a.cols, [&](uint32_t u, uint32_t v, int64_t e) {
+      const W wt = a.w[e];
+      ++cnt;
+      /* a stale dist[u] is an upper bound: u is dirty again if it improved */
+      const W du = DistOps<W>::value(a.dist[u]);
+      const W nd = du + wt;
+      const uint32_t nb = DistOps<W>::bits(nd);
+      if (nb < a.dist[v]) {
+        const uint32_t old = atomic_min(&a.dist[v], nb);
+        if (nb < old) {
+          const uint64_t bit = 1ull << (v & 63);
+          if (!(a.next_bm[v >> 6] & bit)) atomic_or(&a.next_bm[v >> 6], bit);
+        }
       }
-    }
-  });
+    });
+  }
   /* arcs relaxed in this phase: one atomic per workgroup into 64 slots */
   __shared__ unsigned long long red[kBlock / 64];
   cnt = wave_sum_u64(cnt);
@@ -128,10 +222,11 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(SsspArgs<OffT, W> a, cons
 template<typename OffT, typename W>
 __global__ void __launch_bounds__(kBlock) split_build_kernel(const OffT* off, const int32_t* cols, const W* w,
                                                              int64_t n, W dlt, int32_t* scols, W* sw,
-                                                             int32_t* light) {
+                                                             int32_t* light, unsigned long long* heavy_total) {
   const int lane = lane_id();
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  unsigned long long heavy = 0; /* lane 0: heavy arcs of the wave's vertices */
   for (int64_t v = gwave; v < n; v += nwaves) {
     const int64_t b = (int64_t)off[v], e = (int64_t)off[v + 1];
     int64_t front = b, back = e;
@@ -159,8 +254,12 @@ __global__ void __launch_bounds__(kBlock) split_build_kernel(const OffT* off, co
       front += __popcll(ml);
       back -= __popcll(mh);
     }
-    if (lane == 0) light[v] = (int32_t)(front - b);
+    if (lane == 0) {
+      light[v] = (int32_t)(front - b);
+      heavy += (unsigned long long)(e - front);
+    }
   }
+  if (lane == 0 && heavy) atomic_add(heavy_total, heavy);
 }
 
 /* per-block counters of the split, one 128-byte line per slot */
@@ -176,14 +275,17 @@ struct alignas(128) SplitSlot {
 
 template<typename W>
 __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, uint64_t* settled,
-                                                       const uint32_t* dist, const SsspCtl* ctl, int64_t w0,
-                                                       SplitSlot* slots) {
+                                                       uint64_t* settled_all, const uint32_t* dist,
+                                                       const SsspCtl* ctl, int64_t w0, SplitSlot* slots) {
   if (ctl->done) return;
   if (ctl->heavy_next) { /* the heavy iteration: its frontier is the bucket's settled set */
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
       const uint64_t r = settled[i];
       near_bm[i] = r;
-      if (r) settled[i] = 0;
+      if (r) {
+        settled[i] = 0;
+        settled_all[i] |= r;
+      }
     }
     return;
   }
@@ -235,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) near_count_kernel(CompactArgs<OffT> a,
 
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) near_write_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
-  if (ctl->done || !ctl->relax) return;
+  if (ctl->done || !ctl->relax || ctl->pull) return;
   compact_write_body<OffT>(a, ctl->heavy ? 1 : 0);
 }
 
@@ -255,12 +357,17 @@ __device__ __forceinline__ uint32_t next_threshold(uint32_t fmin_bits, double de
 }
 
 /* Scan of the near set's counts; relax / advance the threshold / stop; the
- * iteration's record into the pinned ring. */
+ * iteration's record into the pinned ring.  A heavy iteration pulls
+ * (pull_mode 1), pushes (-1), or (0) pulls when the heavy arcs of the vertices
+ * not yet settled -- all heavy arcs less those of every settled set so far,
+ * in-arcs being out-arcs on an undirected graph -- are fewer than those of
+ * the bucket's settled set. */
 template<typename W>
 __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* blocks, int64_t nblocks, QueueHdr* hdr,
                                                                    SsspCtl* ctl, const SplitSlot* slots,
                                                                    double delta, double w_max, int32_t iter,
-                                                                   RoundLog* log) {
+                                                                   RoundLog* log, int32_t pull_mode,
+                                                                   const unsigned long long* heavy_total) {
   __shared__ QueueHdr tot;
   if (ctl->done) return;
   compact_scan_body(blocks, nblocks, hdr, &tot);
@@ -270,14 +377,20 @@ __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* bl
   e.iter = iter;
   e.near_v = (int64_t)tot.nvert;
   e.near_e = (int64_t)tot.deg;
+  ctl->pull = 0;
   if (ctl->heavy_next) { /* this iteration relaxes the heavy arcs of the settled set */
     ctl->heavy_next = 0;
     ctl->r_pending = 0;
     ctl->heavy = 1;
     ctl->relax = tot.nvert > 0 ? 1 : 0;
+    const unsigned long long done_arcs = ctl->heavy_done + tot.deg;
+    ctl->heavy_done = done_arcs;
     if (tot.nvert > 0) {
+      const unsigned long long pull_arcs = *heavy_total - done_arcs;
+      const bool pull = pull_mode > 0 || (pull_mode == 0 && pull_arcs < tot.deg);
+      ctl->pull = pull ? 1 : 0;
       ctl->rounds += 1;
-      ctl->relaxed += tot.deg;
+      ctl->relaxed += pull ? pull_arcs : tot.deg;
     }
   } else if (tot.nvert > 0) {
     ctl->heavy = 0;
@@ -363,7 +476,8 @@ __global__ void reached_kernel(const OffT* off, const uint32_t* dist, uint32_t u
 }
 
 struct SsspWorkspace {
-  uint64_t* bm[3] = {nullptr, nullptr, nullptr}; /* dirty vertices, near set of the round, bucket's settled set */
+  /* dirty vertices, near set of the round, bucket's settled set, every vertex settled so far */
+  uint64_t* bm[4] = {nullptr, nullptr, nullptr, nullptr};
   unsigned long long* acc = nullptr;
   SplitSlot* slots = nullptr;
   SsspCtl* ctl = nullptr;
@@ -379,6 +493,7 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->bm[0]);
   (void)hipFree(w->bm[1]);
   (void)hipFree(w->bm[2]);
+  (void)hipFree(w->bm[3]);
   (void)hipFree(w->acc);
   (void)hipFree(w->slots);
   (void)hipFree(w->ctl);
@@ -403,6 +518,7 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
     if ((st = sg::alloc_device(dev, (void**)&ws->bm[0], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->bm[1], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->bm[2], 8 * (ws->w0 + 1))) ||
+        (st = sg::alloc_device(dev, (void**)&ws->bm[3], 8 * (ws->w0 + 1))) ||
         (st = sg::alloc_device(dev, (void**)&ws->acc, 16)) ||
         (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
         (st = sg::alloc_device(dev, (void**)&ws->ctl, sizeof(SsspCtl))) ||
@@ -436,9 +552,12 @@ int ensure_split(sg_graph_t g, W dlt) {
     SG_TRY(sg::alloc_device(dev, (void**)&g->sssp_w, 4 * (size_t)(g->m + 1)));
     SG_TRY(sg::alloc_device(dev, (void**)&g->sssp_light, 4 * (size_t)(g->n + 1)));
   }
+  unsigned long long* heavy_total = nullptr; /* heavy arcs of the split, read by the heavy rounds' decision */
+  SG_TRY(sg::graph_buffer(g, "sssp_heavy_total", 8, &heavy_total));
+  SG_TRY(sg::zero_async(dev, heavy_total, 8));
   split_build_kernel<OffT, W><<<dev->cus * 8, kBlock, 0, dev->stream>>>(
       (const OffT*)g->offsets, g->cols, (const W*)g->weights, g->n, dlt, g->sssp_cols, (W*)g->sssp_w,
-      g->sssp_light);
+      g->sssp_light, heavy_total);
   SG_HIP(hipGetLastError());
   g->sssp_split_bits = bits;
   g->sssp_split_valid = true;
@@ -446,7 +565,7 @@ int ensure_split(sg_graph_t g, W dlt) {
 }
 
 template<typename OffT, typename W>
-int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats* stats) {
+int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, W* dist_out, sg_sssp_stats* stats) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   const int64_t n = g->n;
@@ -468,8 +587,15 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
   uint64_t* dirty = ws->bm[0];
   uint64_t* near_bm = ws->bm[1];
   uint64_t* settled = ws->bm[2];
+  uint64_t* settled_all = ws->bm[3];
   SG_TRY(sg::zero_async(dev, dirty, 8 * ws->w0));
   SG_TRY(sg::zero_async(dev, settled, 8 * ws->w0));
+  /* vertices without arcs never change: a pull skips them from the start */
+  const bool directed = (g->flags & SG_GRAPH_DIRECTED) != 0;
+  if (!directed) {
+    SG_TRY(sg::graph_ensure_isolated(g));
+    SG_HIP(hipMemcpyAsync(settled_all, g->isolated, 8 * ws->w0, hipMemcpyDeviceToDevice, s));
+  }
   SG_TRY(sg::zero_async(dev, ws->relax_slots, 8 * 16 * kSlots));
   sssp_init_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(off, dist, unreached, n, src, Q.q[0][0],
                                                                        Q.q[0][1], Q.q[0][2], Q.hdr, tune);
@@ -498,14 +624,25 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, W* dist_out, sg_sssp_stats
   a.w0 = ws->w0;
   for (int k = 0; k < 3; ++k) a.qin[k] = Q.q[0][k];
   a.qin_hdr = Q.hdr;
+  a.off = off;
+  a.light = g->sssp_light;
+  a.settled = settled_all;
+  a.frontier = near_bm;
+  a.n = n;
+  /* a heavy round pulls only on an undirected graph (in-arcs = out-arcs, symmetric weights) */
+  const int32_t pull_mode = directed ? -1 : heavy_pull;
+  unsigned long long* heavy_total = nullptr;
+  if (!bellman) SG_TRY(sg::graph_buffer(g, "sssp_heavy_total", 8, &heavy_total));
   /* one iteration: split, near-set compaction, decide (relax / next bucket /
    * stop), relax; everything the host needs is in the iteration's record */
   auto enqueue = [&](int32_t it) -> int {
     split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl);
-    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, settled, dist, ws->ctl, ws->w0, ws->slots);
+    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, settled, settled_all, dist, ws->ctl, ws->w0,
+                                              ws->slots);
     near_count_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     sssp_finish_kernel<W><<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr, ws->ctl, ws->slots,
-                                                     bellman ? 0.0 : delta, w_max, it, ws->log);
+                                                     bellman ? 0.0 : delta, w_max, it, ws->log, pull_mode,
+                                                     heavy_total);
     near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
@@ -603,13 +740,13 @@ int mean_weight(sg_graph_t g, double* out) {
 }
 
 template<typename OffT, typename W>
-int run_sssp_auto(sg_graph_t g, int32_t source, double delta, W* dist, sg_sssp_stats* stats) {
+int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, W* dist, sg_sssp_stats* stats) {
   if (delta == 0.0) {
     double mw = 1.0;
     SG_TRY(mean_weight<W>(g, &mw));
     delta = std::max(kAutoDeltaFactor * mw, DistOps<W>::is_float ? 1e-30 : 1.0);
   }
-  return run_sssp<OffT, W>(g, source, delta, dist, stats);
+  return run_sssp<OffT, W>(g, source, delta, heavy_pull, dist, stats);
 }
 
 extern "C" {
@@ -631,6 +768,8 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void
   else sg_sssp_options_default(&opt);
   SG_CHECK(opt.load_balancer >= SG_LB_WORKITEM && opt.load_balancer <= SG_LB_BUCKETING, SG_ERR_INVALID,
            "sg_sssp: unknown load balancer %d", opt.load_balancer);
+  SG_CHECK(opt.heavy_pull >= -1 && opt.heavy_pull <= 1, SG_ERR_INVALID, "sg_sssp: heavy_pull %d not in {-1, 0, 1}",
+           opt.heavy_pull);
   if (stats) {
     stats->rounds = 0;
     stats->reached_vertices = stats->reached_edges = stats->relaxations = 0;
@@ -639,10 +778,10 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void
   SG_HIP(hipSetDevice(g->dev->ordinal));
   const bool fw = (g->flags & SG_GRAPH_WEIGHT_FLOAT) != 0;
   if (g->offset_bytes == 8)
-    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, (float*)dist, stats)
-              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, (uint32_t*)dist, stats);
-  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, (float*)dist, stats)
-            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, (uint32_t*)dist, stats);
+    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, opt.heavy_pull, (float*)dist, stats)
+              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, (uint32_t*)dist, stats);
+  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, opt.heavy_pull, (float*)dist, stats)
+            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, (uint32_t*)dist, stats);
 }
 
 int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats) {

@@ -123,6 +123,70 @@ def test_sssp_float_weights_buckets(sg, device, oracle_mod, delta):
     assert np.allclose(got[reach], ref[reach] / 7.0, rtol=1e-5)
 
 
+@pytest.mark.parametrize("heavy_pull", [1, -1, 0])
+def test_sssp_heavy_round_pull_and_push(sg, device, oracle_mod, heavy_pull):
+    """A bucket's heavy round either pulls into the vertices not yet settled
+    or pushes from the bucket's settled set; the distances are the same.  The
+    ER graph has isolated vertices and several components, the grid a long
+    diameter (many buckets)."""
+    graphs = [oracle_mod.rmat(12, 16, seed=48), oracle_mod.grid(40, 50, 0.1, seed=49),
+              oracle_mod.erdos_renyi(3000, 9000, seed=50)]
+    for g in graphs:
+        oracle_mod.fill_weights(g, 11)
+        G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+        s = sg.SSSP(G)
+        for src, delta in [(0, 0.0), (g.n // 2, 20.0), (int(np.argmax(g.degrees())), 3.0)]:
+            s.init(src)
+            s.run(delta=delta, heavy_pull=heavy_pull)
+            assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src)), (g.n, src, delta)
+
+
+def test_sssp_heavy_pull_relaxes_fewer_arcs_on_rmat(sg, device, oracle_mod):
+    # after the first bucket almost every vertex is settled: pulling into the
+    # rest relaxes far fewer heavy arcs than pushing all of the settled set's
+    g = oracle_mod.rmat(13, 16, seed=52)
+    oracle_mod.fill_weights(g, 12)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+    s = sg.SSSP(G)
+    src = int(np.argmax(g.degrees()))
+    counts = {}
+    for mode in (1, -1):
+        s.init(src)
+        counts[mode] = s.run(heavy_pull=mode).relaxations
+        assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src))
+    assert counts[1] < counts[-1], counts
+
+
+def test_sssp_float_heavy_pull_matches_push(sg, device, oracle_mod):
+    # each final distance is the minimum of fl(dist[u] + w) over final
+    # dist[u], whichever direction relaxed it: pull and push agree bit for bit
+    g = oracle_mod.rmat(11, 8, seed=51)
+    w = oracle_mod.fill_weights(g, 5)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=(w / 7.0).astype(np.float32), weight_float=True)
+    s = sg.SSSP(G)
+    got = {}
+    for mode in (1, -1):
+        s.init(9)
+        s.run(heavy_pull=mode)
+        got[mode] = s.get_distances()
+    assert np.array_equal(got[1].view(np.uint32), got[-1].view(np.uint32))
+    ref = oracle_mod.sssp(g, 9).astype(np.float64)
+    reach = ref != g.n + 1
+    assert np.allclose(got[1].astype(np.float64)[reach], ref[reach] / 7.0, rtol=1e-5)
+
+
+def test_sssp_heavy_pull_directed_pushes(sg, device, oracle_mod):
+    # a directed graph's heavy rounds push whatever is asked
+    g = _fixture(oracle_mod, "weighted_directed_5")
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights, directed=True)
+    s = sg.SSSP(G)
+    s.init(0)
+    s.run(delta=2.0, heavy_pull=1)
+    assert s.get_distances().tolist() == _cases()["expect"]["sssp_weighted_directed_5"]
+    with pytest.raises(sg.SygraphError):
+        s.run(heavy_pull=2)
+
+
 def test_sssp_float_weights(sg, device, oracle_mod):
     g = oracle_mod.rmat(11, 8, seed=41)
     w = oracle_mod.fill_weights(g, 5)
