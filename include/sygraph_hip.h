@@ -338,9 +338,11 @@ int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_
 
 typedef struct sg_sssp_options {
   int32_t load_balancer; /* sg_load_balancer (the bucketing bins in every case) */
-  double delta;          /* bucket width of the near/far split: 0 = automatic (half the mean arc
-                            weight), negative or infinite = none, i.e. every dirty vertex is relaxed
-                            each round (the reference's frontier Bellman-Ford, sssp.hpp:152-210) */
+  double delta;          /* bucket width of the near/far split: 0 = automatic (none below 2^25 arcs,
+                            where an iteration's launches cost more than the relaxations buckets
+                            save; half the mean arc weight from there on), negative or infinite =
+                            none, i.e. every dirty vertex is relaxed each round (the reference's
+                            frontier Bellman-Ford, sssp.hpp:152-210) */
   int32_t heavy_pull;    /* direction of a bucket's heavy round on an undirected graph: 0 = automatic
                             (pull when the heavy arcs of the vertices not yet settled are fewer than
                             those of the bucket's settled set), 1 = always pull, -1 = always push.

@@ -585,9 +585,10 @@ class SSSP:
         self.source = None
 
     def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0) -> SsspStats:
-        """delta: bucket width of the near/far split (0 automatic, negative or
-        inf for the reference's frontier Bellman-Ford).  heavy_pull: direction
-        of a bucket's heavy round (0 automatic, 1 pull, -1 push)."""
+        """delta: bucket width of the near/far split (0 automatic: none below
+        2^25 arcs, half the mean arc weight above; negative or inf for the
+        reference's frontier Bellman-Ford).  heavy_pull: direction of a
+        bucket's heavy round (0 automatic, 1 pull, -1 push)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
         o = SsspOptions()

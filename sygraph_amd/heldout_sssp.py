@@ -1,0 +1,98 @@
+"""SSSP on the held-out graph set: the reference's schedule against this build's.
+
+    python -m sygraph_amd.heldout_sssp [--quick] [--out sygraph_amd/data/sssp_heldout.json]
+
+The graphs are the held-out set of the BFS policy (seeds 300-349, families and
+sizes as in train_policy.make_graph), each carrying u32 weights U[1,255]
+(weight seed = graph seed), from the same four sources per graph.  Per run the
+device time of the whole SSSP (best of three) under
+
+  plain     frontier Bellman-Ford, every round a push (delta < 0: the schedule
+            of algorithms/sssp.hpp:152-210);
+  buckets   delta-stepping with width 64 (half the mean weight), heavy rounds
+            pushing;
+  adaptive  the defaults: the automatic schedule (plain below 2^25 arcs,
+            buckets of half the mean weight above) with heavy rounds pulling
+            when cheaper.
+
+The three distance arrays of a run are compared on the device (sg_digest) and
+must be equal.  Writes the per-run times and the totals / geometric means.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(HERE) not in sys.path:
+    sys.path.insert(0, os.path.dirname(HERE))
+
+import sygraph_amd as sg  # noqa: E402
+from sygraph_amd.train_policy import make_graph, sources_of  # noqa: E402
+
+MODES = [("plain", dict(delta=-1.0, heavy_pull=-1)), ("buckets", dict(delta=64.0, heavy_pull=-1)),
+         ("adaptive", dict(delta=0.0, heavy_pull=0))]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def evaluate(dev, seeds, quick, nsrc=4, reps=3):
+    rows = []
+    for seed in seeds:
+        name, G = make_graph(dev, seed, quick, weight_seed=seed)
+        s = sg.SSSP(G)
+        for src in sources_of(G, seed, nsrc):
+            t, digests = {}, set()
+            for key, kw in MODES:
+                best = None
+                for _ in range(reps):
+                    s.init(src)
+                    st = s.run(**kw)
+                    best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+                t[key] = best
+                digests.add(s.dist.digest(G.n, 4))
+            if len(digests) != 1:
+                raise RuntimeError(f"seed {seed} source {src}: distances differ between schedules")
+            rows.append({"seed": seed, "graph": name, "n": G.n, "m": G.m, "source": src, **t})
+        last = rows[-1]
+        log(f"  seed {seed}: {name}: plain {last['plain']:.2f} buckets {last['buckets']:.2f} "
+            f"adaptive {last['adaptive']:.2f} ms")
+        G.destroy()
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "data", "sssp_heldout.json"))
+    ap.add_argument("--quick", action="store_true", help="smaller graphs and set (smoke run)")
+    ap.add_argument("--seeds", default="300-349", help="graph seeds, first-last (training graphs: 100-299)")
+    ap.add_argument("--sources", type=int, default=4)
+    ap.add_argument("--reps", type=int, default=3)
+    a = ap.parse_args()
+    dev = sg.Device(0)
+    lo, hi = (int(x) for x in a.seeds.split("-"))
+    seeds = list(range(lo, hi + 1)) if not a.quick else list(range(lo, min(hi, lo + 5) + 1))
+    rows = evaluate(dev, seeds, a.quick, a.sources, a.reps)
+    tot = {k: float(sum(r[k] for r in rows)) for k, _ in MODES}
+    geo = lambda num, den: float(np.exp(np.mean([np.log(r[num] / r[den]) for r in rows])))  # noqa: E731
+    summary = {"total_ms": tot,
+               "adaptive_speedup_vs_plain_total": tot["plain"] / tot["adaptive"],
+               "adaptive_speedup_vs_plain_geomean": geo("plain", "adaptive"),
+               "adaptive_speedup_vs_buckets_total": tot["buckets"] / tot["adaptive"],
+               "adaptive_speedup_vs_buckets_geomean": geo("buckets", "adaptive")}
+    log(json.dumps(summary, indent=1))
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    with open(a.out, "w") as f:
+        json.dump({"seeds": [seeds[0], seeds[-1]], "quick": a.quick, "weights": "u32 U[1,255], weight seed = graph seed",
+                   "summary": summary, "runs": rows}, f, indent=1)
+    print(json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()

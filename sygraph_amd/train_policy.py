@@ -48,8 +48,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_graph(dev, seed: int, quick: bool):
-    """The graph of a seed (family and size drawn from the seed)."""
+def make_graph(dev, seed: int, quick: bool, weight_seed: int = 0):
+    """The graph of a seed (family and size drawn from the seed); with
+    weight_seed != 0 it carries u32 weights U[1,255]."""
     rng = np.random.default_rng(seed)
     kind = rng.choice(["rmat", "rmat", "rmat", "grid", "er"])
     top = 19 if quick else 22
@@ -58,15 +59,16 @@ def make_graph(dev, seed: int, quick: bool):
         ef = int(rng.choice([8, 16, 32]))
         a = float(rng.uniform(0.45, 0.60))
         bc = (1.0 - a) / 2.6
-        return f"rmat s{scale} ef{ef} a{a:.3f}", sg.Graph.rmat(dev, scale, ef, a, bc, bc, seed=seed)
+        return f"rmat s{scale} ef{ef} a{a:.3f}", sg.Graph.rmat(dev, scale, ef, a, bc, bc, seed=seed,
+                                                              weight_seed=weight_seed)
     if kind == "grid":
         side = int(rng.choice([128, 256, 512] if quick else [256, 512, 1024]))
         p = float(rng.uniform(0.0, 0.3))
-        return f"grid {side}x{side} p{p:.2f}", sg.Graph.grid(dev, side, side, p, seed=seed)
+        return f"grid {side}x{side} p{p:.2f}", sg.Graph.grid(dev, side, side, p, seed=seed, weight_seed=weight_seed)
     lg = int(rng.integers(16, (19 if quick else 21) + 1))
     deg = int(rng.choice([4, 8, 16, 32]))
     n = 1 << lg
-    return f"er 2^{lg} d{deg}", sg.Graph.erdos_renyi(dev, n, n * deg // 2, seed=seed)
+    return f"er 2^{lg} d{deg}", sg.Graph.erdos_renyi(dev, n, n * deg // 2, seed=seed, weight_seed=weight_seed)
 
 
 def sources_of(G, seed: int, k: int = 4):

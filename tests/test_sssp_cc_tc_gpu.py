@@ -102,9 +102,15 @@ def test_sssp_buckets_relax_fewer_arcs(sg, device, oracle_mod):
     s.init(src)
     bf = s.run(delta=-1.0).relaxations
     s.init(src)
-    auto = s.run().relaxations
+    buckets = s.run(delta=64.0).relaxations  # half the mean of weights U[1,255]
     assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src))
-    assert auto < bf, (auto, bf)
+    assert buckets < bf, (buckets, bf)
+    # the automatic schedule of a graph this small (under 2^25 arcs) is the
+    # reference's (relaxation counts vary a little from run to run: a round's
+    # atomics race with each other)
+    s.init(src)
+    auto = s.run().relaxations
+    assert abs(auto - bf) < 0.05 * bf and abs(auto - bf) < abs(auto - buckets), (auto, bf, buckets)
 
 
 @pytest.mark.parametrize("delta", [0.0, 0.3, -1.0])
@@ -152,7 +158,7 @@ def test_sssp_heavy_pull_relaxes_fewer_arcs_on_rmat(sg, device, oracle_mod):
     counts = {}
     for mode in (1, -1):
         s.init(src)
-        counts[mode] = s.run(heavy_pull=mode).relaxations
+        counts[mode] = s.run(delta=64.0, heavy_pull=mode).relaxations
         assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src))
     assert counts[1] < counts[-1], counts
 
@@ -167,7 +173,7 @@ def test_sssp_float_heavy_pull_matches_push(sg, device, oracle_mod):
     got = {}
     for mode in (1, -1):
         s.init(9)
-        s.run(heavy_pull=mode)
+        s.run(delta=9.0, heavy_pull=mode)  # about half the mean of the weights w / 7
         got[mode] = s.get_distances()
     assert np.array_equal(got[1].view(np.uint32), got[-1].view(np.uint32))
     ref = oracle_mod.sssp(g, 9).astype(np.float64)
