@@ -328,49 +328,6 @@ __global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const Bf
   else pull_level(a, ctl->lb, level);
 }
 
-/* True in exactly one workgroup of the grid, the last to get here; it also
- * resets the counter for the next launch.  All threads call.  What the last
- * workgroup may then read of the grid's earlier writes:
- *  - kCoherent: what was written with agent-coherent stores (every wave
- *    waits for its stores to complete before the arrival is counted), read
- *    with agent-coherent loads; nothing is written back or invalidated;
- *  - otherwise: the writes of threads that pass `publish` (they release at
- *    agent scope, an L2 write-back, so only threads with data for the last
- *    workgroup should), read with ordinary loads after the last workgroup's
- *    acquire (an L2 invalidate). */
-constexpr int kArriveGroups = 8;  /* workgroups are dealt to the 8 XCDs round robin */
-constexpr int kArriveStride = 16; /* counters 64 B apart */
-constexpr int kArriveWords = (kArriveGroups + 1) * kArriveStride;
-
-/* Arrivals are counted in two stages, per group of workgroups (blockIdx.x %
- * kArriveGroups: one counter per XCD) and then per group, so no single
- * counter takes more than 1/8 of the grid's atomics.  `ctr` holds
- * kArriveWords zeroed words. */
-template<bool kCoherent>
-__device__ __forceinline__ bool last_to_arrive(uint32_t* ctr, bool publish) {
-  __shared__ uint32_t s_last;
-  if (kCoherent) wait_stores_complete();
-  else if (publish) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t grid = gridDim.x, g = blockIdx.x % kArriveGroups;
-    const uint32_t ngroups = grid < (uint32_t)kArriveGroups ? grid : (uint32_t)kArriveGroups;
-    const uint32_t in_group = (grid - g + kArriveGroups - 1) / kArriveGroups;
-    uint32_t* gc = ctr + (1 + g) * kArriveStride;
-    uint32_t last = 0;
-    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == in_group - 1) {
-      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ngroups - 1 ? 1u : 0u;
-      if (last) __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s_last = last;
-  }
-  __syncthreads();
-  const bool last = s_last != 0;
-  if (!kCoherent && last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return last;
-}
-
 /* Totals of a pull level's 64 stat slots and the decision taken on them. */
 struct PullTotals {
   unsigned long long disc, disc_deg, edges, scanned, misses, max_deg;

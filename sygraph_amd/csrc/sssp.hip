@@ -69,7 +69,8 @@ struct SsspArgs {
 };
 
 /* Device-resident control block of the round loop (written only by
- * sssp_finish_kernel; kernel boundaries order the accesses). */
+ * the last workgroup of sssp_select_kernel; kernel boundaries order the
+ * accesses). */
 struct SsspCtl {
   uint32_t T_bits;    /* threshold: near = distance below T */
   int32_t done;       /* no dirty vertex left: later iterations are no-ops */
@@ -271,15 +272,15 @@ struct alignas(128) SplitSlot {
 };
 
 /* near = dirty vertices with distance < T (moved out of the dirty bitmap into
- * the round's frontier bitmap); count and smallest distance of the rest */
-
+ * the round's frontier bitmap); count and smallest distance of the rest.  One
+ * word per thread (word i); the heavy iteration's frontier is the bucket's
+ * settled set instead.  All threads of the workgroup call. */
 template<typename W>
-__global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t* near_bm, uint64_t* settled,
-                                                       uint64_t* settled_all, const uint32_t* dist,
-                                                       const SsspCtl* ctl, int64_t w0, SplitSlot* slots) {
-  if (ctl->done) return;
-  if (ctl->heavy_next) { /* the heavy iteration: its frontier is the bucket's settled set */
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
+__device__ __forceinline__ void split_word(int64_t i, int64_t w0, bool heavy_next, uint32_t T_bits, uint64_t* dirty,
+                                           uint64_t* near_bm, uint64_t* settled, uint64_t* settled_all,
+                                           const uint32_t* dist, SplitSlot* slots) {
+  if (heavy_next) {
+    if (i < w0) {
       const uint64_t r = settled[i];
       near_bm[i] = r;
       if (r) {
@@ -289,10 +290,10 @@ __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t
     }
     return;
   }
-  const W T = DistOps<W>::value(ctl->T_bits);
+  const W T = DistOps<W>::value(T_bits);
   unsigned long long far = 0;
   unsigned int fmin = 0xFFFFFFFFu;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < w0; i += (int64_t)gridDim.x * kBlock) {
+  if (i < w0) {
     const uint64_t f = dirty[i];
     uint64_t nb = 0;
     for (uint64_t rest = f; rest; rest &= rest - 1) {
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) split_kernel(uint64_t* dirty, uint64_t
       dirty[i] = f & ~nb;
       settled[i] |= nb;
     }
-    far += (unsigned long long)__popcll(f & ~nb);
+    far = (unsigned long long)__popcll(f & ~nb);
   }
   far = wave_sum_u64(far);
   unsigned long long m = wave_max_u64(~(unsigned long long)fmin); /* min via max of complement */
@@ -325,14 +326,6 @@ __global__ void split_slots_reset_kernel(SplitSlot* slots, const SsspCtl* ctl) {
     slots[i].far_count = 0;
     slots[i].far_min = 0xFFFFFFFFu;
   }
-}
-
-/* entries over the light part (light iterations) or the heavy part (the
- * heavy iteration, decided before the count and recorded by the finish) */
-template<typename OffT>
-__global__ void __launch_bounds__(kBlock) near_count_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
-  if (ctl->done) return;
-  compact_count_body<OffT, kFromBitmap>(a, ctl->heavy_next ? 1 : 0);
 }
 
 template<typename OffT>
@@ -356,21 +349,43 @@ __device__ __forceinline__ uint32_t next_threshold(uint32_t fmin_bits, double de
   return DistOps<W>::bits(tn);
 }
 
-/* Scan of the near set's counts; relax / advance the threshold / stop; the
- * iteration's record into the pinned ring.  A heavy iteration pulls
- * (pull_mode 1), pushes (-1), or (0) pulls when the heavy arcs of the vertices
- * not yet settled -- all heavy arcs less those of every settled set so far,
- * in-arcs being out-arcs on an undirected graph -- are fewer than those of
- * the bucket's settled set. */
-template<typename W>
-__global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* blocks, int64_t nblocks, QueueHdr* hdr,
-                                                                   SsspCtl* ctl, const SplitSlot* slots,
-                                                                   double delta, double w_max, int32_t iter,
-                                                                   RoundLog* log, int32_t pull_mode,
-                                                                   const unsigned long long* heavy_total) {
+/* Where a selection launch keeps its scratch and writes its decision. */
+struct SelectArgs {
+  uint64_t *dirty, *near_bm, *settled, *settled_all;
+  const uint32_t* dist;
+  SplitSlot* slots;
+  int64_t w0, nblocks; /* bitmap words; count-pass workgroups (sums to scan) */
+  uint32_t* arrive;    /* kArriveWords zeroed words */
+  RoundLog* log;
+  const unsigned long long* heavy_total;
+  double delta, w_max;
+  int32_t pull_mode;
+};
+
+/* One iteration's selection, count and decision in one launch.  Each
+ * workgroup splits the 256 words of the dirty bitmap that its four count
+ * tiles cover (thread t splits the word that it then counts, so no other
+ * workgroup's near bits are needed), counts the near set's queue entries,
+ * and the last workgroup to finish scans the counts and decides: relax /
+ * advance the threshold / stop, and the iteration's record into the pinned
+ * ring.  A heavy iteration pulls (pull_mode 1), pushes (-1), or (0) pulls
+ * when the heavy arcs of the vertices not yet settled -- all heavy arcs less
+ * those of every settled set so far, in-arcs being out-arcs on an undirected
+ * graph -- are fewer than those of the bucket's settled set.  The far
+ * statistics are added into `slots` with atomics; the last workgroup reads
+ * them through agent-coherent loads and clears them for the next
+ * iteration. */
+template<typename OffT, typename W>
+__global__ void __launch_bounds__(kBlock) sssp_select_kernel(CompactArgs<OffT> ca, SelectArgs sa, SsspCtl* ctl,
+                                                             int32_t iter) {
   __shared__ QueueHdr tot;
   if (ctl->done) return;
-  compact_scan_body(blocks, nblocks, hdr, &tot);
+  const bool heavy_next = ctl->heavy_next != 0;
+  split_word<W>((int64_t)blockIdx.x * kBlock + threadIdx.x, sa.w0, heavy_next, ctl->T_bits, sa.dirty, sa.near_bm,
+                sa.settled, sa.settled_all, sa.dist, sa.slots);
+  compact_count_body<OffT, kFromBitmap>(ca, heavy_next ? 1 : 0);
+  if (!last_to_arrive<true>(sa.arrive, false)) return;
+  compact_scan_body_t<kBlock, 8, true>(ca.blocks, sa.nblocks, ca.hdr, &tot);
   __syncthreads();
   if (threadIdx.x != 0) return;
   RoundLog e{};
@@ -378,7 +393,7 @@ __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* bl
   e.near_v = (int64_t)tot.nvert;
   e.near_e = (int64_t)tot.deg;
   ctl->pull = 0;
-  if (ctl->heavy_next) { /* this iteration relaxes the heavy arcs of the settled set */
+  if (heavy_next) { /* this iteration relaxes the heavy arcs of the settled set */
     ctl->heavy_next = 0;
     ctl->r_pending = 0;
     ctl->heavy = 1;
@@ -386,8 +401,9 @@ __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* bl
     const unsigned long long done_arcs = ctl->heavy_done + tot.deg;
     ctl->heavy_done = done_arcs;
     if (tot.nvert > 0) {
-      const unsigned long long pull_arcs = *heavy_total - done_arcs;
-      const bool pull = pull_mode > 0 || (pull_mode == 0 && pull_arcs < tot.deg);
+      const unsigned long long ht = *sa.heavy_total;
+      const unsigned long long pull_arcs = ht > done_arcs ? ht - done_arcs : 0ull;
+      const bool pull = sa.pull_mode > 0 || (sa.pull_mode == 0 && pull_arcs < tot.deg);
       ctl->pull = pull ? 1 : 0;
       ctl->rounds += 1;
       ctl->relaxed += pull ? pull_arcs : tot.deg;
@@ -401,25 +417,30 @@ __global__ void __launch_bounds__(kScanThreads) sssp_finish_kernel(TileCount* bl
   } else {
     ctl->heavy = 0;
     ctl->relax = 0;
-    if (ctl->r_pending && delta > 0.0) {
+    if (ctl->r_pending && sa.delta > 0.0) {
       ctl->heavy_next = 1; /* the bucket's light phase is over */
     } else {
       unsigned long long far = 0;
       uint32_t fmin = 0xFFFFFFFFu;
       for (int k = 0; k < kSlots; ++k) {
-        far += slots[k].far_count;
-        fmin = slots[k].far_min < fmin ? slots[k].far_min : fmin;
+        far += __hip_atomic_load(&sa.slots[k].far_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t mk = __hip_atomic_load(&sa.slots[k].far_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fmin = mk < fmin ? mk : fmin;
       }
       if (far == 0) ctl->done = 1;
-      else ctl->T_bits = next_threshold<W>(fmin, delta, w_max);
+      else ctl->T_bits = next_threshold<W>(fmin, sa.delta, sa.w_max);
       ctl->r_pending = 0;
     }
+  }
+  for (int k = 0; k < kSlots; ++k) { /* the next iteration's split adds into cleared slots */
+    sa.slots[k].far_count = 0;
+    sa.slots[k].far_min = 0xFFFFFFFFu;
   }
   e.done = ctl->done;
   e.relax = ctl->relax;
   e.rounds = ctl->rounds;
   e.relaxed = ctl->relaxed;
-  log[iter % kRing] = e;
+  sa.log[iter % kRing] = e;
   __threadfence_system();
 }
 
@@ -482,6 +503,7 @@ struct SsspWorkspace {
   SplitSlot* slots = nullptr;
   SsspCtl* ctl = nullptr;
   unsigned long long* relax_slots = nullptr; /* kSlots x 16 (one cache line each) */
+  uint32_t* arrive = nullptr;                /* arrival counters of the selection launch, zero between launches */
   RoundLog* log = nullptr; /* kRing entries, pinned coherent host memory */
   hipEvent_t ev[kRing] = {};
   QueueSet qs;
@@ -498,6 +520,7 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->slots);
   (void)hipFree(w->ctl);
   (void)hipFree(w->relax_slots);
+  (void)hipFree(w->arrive);
   if (w->log) (void)hipHostFree(w->log);
   for (hipEvent_t e : w->ev)
     if (e) (void)hipEventDestroy(e);
@@ -523,6 +546,8 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
         (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
         (st = sg::alloc_device(dev, (void**)&ws->ctl, sizeof(SsspCtl))) ||
         (st = sg::alloc_device(dev, (void**)&ws->relax_slots, 8 * 16 * kSlots)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->arrive, 4 * kArriveWords)) ||
+        (st = sg::zero_async(dev, ws->arrive, 4 * kArriveWords)) ||
         (st = ws->qs.init(dev, n, g->m, default_tuning()))) {
       sssp_ws_free(ws);
       return st;
@@ -608,7 +633,6 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, W* dis
     c.T_bits = DistOps<W>::bits((W)(bellman ? w_max : std::min(delta, w_max)));
     SG_HIP(hipMemcpyAsync(ws->ctl, &c, sizeof(c), hipMemcpyHostToDevice, s));
   }
-  const int grid_w = sg::grid_for(dev, ws->w0, kBlock, 4);
   /* arcs of weight >= dlt are heavy (none under the reference's schedule) */
   const W dlt = bellman ? (W)w_max : (W)std::min(std::max(delta, DistOps<W>::is_float ? 0.0 : 1.0), w_max);
   /* bucketed runs read the light-first adjacency, the entries covering one
@@ -633,16 +657,27 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, W* dis
   const int32_t pull_mode = directed ? -1 : heavy_pull;
   unsigned long long* heavy_total = nullptr;
   if (!bellman) SG_TRY(sg::graph_buffer(g, "sssp_heavy_total", 8, &heavy_total));
-  /* one iteration: split, near-set compaction, decide (relax / next bucket /
-   * stop), relax; everything the host needs is in the iteration's record */
+  SelectArgs sa{};
+  sa.dirty = dirty;
+  sa.near_bm = near_bm;
+  sa.settled = settled;
+  sa.settled_all = settled_all;
+  sa.dist = dist;
+  sa.slots = ws->slots;
+  sa.w0 = ws->w0;
+  sa.nblocks = Q.nblocks;
+  sa.arrive = ws->arrive;
+  sa.log = ws->log;
+  sa.heavy_total = heavy_total;
+  sa.delta = bellman ? 0.0 : delta;
+  sa.w_max = w_max;
+  sa.pull_mode = pull_mode;
+  split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl); /* later iterations clear them as they decide */
+  /* one iteration, three launches: split + near-set count + decision (relax /
+   * next bucket / stop); queue write; relax.  Everything the host needs is in
+   * the iteration's record. */
   auto enqueue = [&](int32_t it) -> int {
-    split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl);
-    split_kernel<W><<<grid_w, kBlock, 0, s>>>(dirty, near_bm, settled, settled_all, dist, ws->ctl, ws->w0,
-                                              ws->slots);
-    near_count_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
-    sssp_finish_kernel<W><<<1, kScanThreads, 0, s>>>(Q.blocks, Q.nblocks, Q.hdr, ws->ctl, ws->slots,
-                                                     bellman ? 0.0 : delta, w_max, it, ws->log, pull_mode,
-                                                     heavy_total);
+    sssp_select_kernel<OffT, W><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, sa, ws->ctl, it);
     near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
