@@ -349,6 +349,9 @@ typedef struct sg_sssp_options {
                             Pull: every vertex not yet settled takes the minimum over its heavy arcs
                             from the settled set; push: every heavy arc of the settled set is
                             relaxed with an atomic minimum.  Directed graphs always push. */
+  int32_t burst_arcs;    /* rounds of the reference's schedule (no buckets) with at most this many
+                            arcs run back to back inside one workgroup while they stay that small:
+                            0 = default (4,096), negative = never, at most 8,192 */
 } sg_sssp_options;
 void sg_sssp_options_default(sg_sssp_options* opt);
 /* sg_sssp with options; the distances do not depend on delta. */

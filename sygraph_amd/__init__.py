@@ -87,7 +87,8 @@ class SsspStats(C.Structure):
 
 
 class SsspOptions(C.Structure):
-    _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double), ("heavy_pull", C.c_int32)]
+    _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double), ("heavy_pull", C.c_int32),
+                ("burst_arcs", C.c_int32)]
 
 
 class BfsStats(C.Structure):
@@ -584,11 +585,14 @@ class SSSP:
     def reset(self):
         self.source = None
 
-    def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0) -> SsspStats:
+    def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0,
+            burst_arcs: int = 0) -> SsspStats:
         """delta: bucket width of the near/far split (0 automatic: none below
         2^25 arcs, half the mean arc weight above; negative or inf for the
         reference's frontier Bellman-Ford).  heavy_pull: direction of a
-        bucket's heavy round (0 automatic, 1 pull, -1 push)."""
+        bucket's heavy round (0 automatic, 1 pull, -1 push).  burst_arcs:
+        small rounds of the reference's schedule run inside one workgroup (0
+        default 4096, negative never)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
         o = SsspOptions()
@@ -596,6 +600,7 @@ class SSSP:
         o.load_balancer = load_balancer
         o.delta = float(delta)
         o.heavy_pull = int(heavy_pull)
+        o.burst_arcs = int(burst_arcs)
         check(lib().sg_sssp_ex(self.g.h, self.source, C.byref(o), C.c_void_p(self.dist.ptr), C.byref(self.stats)))
         return self.stats
 

@@ -80,6 +80,9 @@ struct SsspCtl {
   int32_t heavy_next; /* the next iteration is the heavy one */
   int32_t r_pending;  /* vertices were relaxed in this bucket since its heavy iteration */
   int32_t pull;       /* the current (heavy) iteration pulls into the vertices not yet settled */
+  int32_t burst;      /* the current (Bellman-Ford) round is small: the write pass's last workgroup runs it
+                         and the following small rounds */
+  int32_t pad1;
   unsigned long long relaxed;    /* arcs relaxed so far */
   unsigned long long heavy_done; /* heavy arcs of every settled set so far */
 };
@@ -328,10 +331,147 @@ __global__ void split_slots_reset_kernel(SplitSlot* slots, const SsspCtl* ctl) {
   }
 }
 
-template<typename OffT>
-__global__ void __launch_bounds__(kBlock) near_write_kernel(CompactArgs<OffT> a, const SsspCtl* ctl) {
+/* ------------------------------------------------------------ bursts --- */
+/* Rounds of the reference's schedule (every dirty vertex relaxed each round)
+ * on a graph of long diameter have small frontiers, and each costs a
+ * selection, a write and a relax launch.  When a round has at most
+ * burst_arcs arcs, the last workgroup of the write pass relaxes it and the
+ * following rounds back to back, one workgroup barrier per round, for as
+ * long as they stay small.  The frontier moves between two lists of (first
+ * arc, vertex, degree) entries; a round lays the list's arcs end to end (an
+ * exclusive scan of the lengths in LDS) and each thread takes every
+ * kBlock-th arc, finding its entry by binary search.  A relaxation that
+ * lowers dist[v] sets v's dirty bit; the lane that set it appends v to the
+ * next list.  Before a round the bits of its list are cleared (the split's
+ * job in the launched rounds), so a vertex lowered again during its own
+ * round is relaxed again in the next.  The burst stops when a round lowers
+ * nothing or the next list is too large: its vertices keep their dirty bits
+ * and the next iteration's selection takes them from the bitmap. */
+constexpr int kSsspBurstDefault = 4096;
+constexpr int kSsspBurstMax = 8192; /* entries (and arcs) of a burst round; LDS for its scan */
+
+template<typename OffT, typename W>
+struct SsspBurst {
+  const OffT* off;
+  const int32_t* cols;
+  const W* w;
+  uint32_t* dist;
+  uint64_t* dirty;
+  QEntry* q[3]; /* the round's queues (written by this launch) */
+  const QueueHdr* hdr;
+  QEntry* list[2]; /* kSsspBurstMax entries each */
+  int64_t burst_arcs;
+  unsigned long long* relax_slots;
+  uint32_t* arrive; /* kArriveWords zeroed words */
+};
+
+template<typename OffT, typename W>
+__device__ __forceinline__ void sssp_burst(const SsspBurst<OffT, W>& b, SsspCtl* ctl) {
+  __shared__ uint32_t s_pref[kSsspBurstMax + 1];
+  __shared__ uint32_t s_wtot[kBlock / 64];
+  __shared__ uint32_t s_next, s_next_arcs;
+  const QueueHdr h = *b.hdr;
+  const uint32_t n0 = h.count[0], n1 = h.count[1];
+  uint32_t cnt = h.count[0] + h.count[1] + h.count[2];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  /* the first round's entries, the three class queues, gathered into the
+   * list that the second round writes */
+  for (uint32_t i = threadIdx.x; i < cnt; i += kBlock)
+    b.list[1][i] = i < n0 ? b.q[0][i] : (i < n0 + n1 ? b.q[1][i - n0] : b.q[2][i - n0 - n1]);
+  __syncthreads();
+  unsigned long long arcs = 0;
+  int32_t rounds = 0;
+  for (int it = 0;; ++it) {
+    const QEntry* in = b.list[(it + 1) & 1];
+    QEntry* out = b.list[it & 1];
+    auto entry = [&](uint32_t i) -> QEntry { return in[i]; };
+    /* exclusive scan of the lengths: thread t owns a run of kPer entries */
+    constexpr int kPer = kSsspBurstMax / kBlock;
+    const uint32_t first = threadIdx.x * kPer;
+    uint32_t run = 0;
+    for (int k = 0; k < kPer; ++k)
+      if (first + k < cnt) run += (uint32_t)entry(first + k).len;
+    const uint32_t incl = wave_inclusive_scan(run);
+    if (lane == 63) s_wtot[wv] = incl;
+    if (threadIdx.x == 0) s_next = s_next_arcs = 0;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      base += w < wv ? s_wtot[w] : 0u;
+      total += s_wtot[w];
+    }
+    uint32_t pos = base + incl - run;
+    for (int k = 0; k < kPer; ++k)
+      if (first + k < cnt) {
+        s_pref[first + k] = pos;
+        pos += (uint32_t)entry(first + k).len;
+      }
+    if (threadIdx.x == 0) s_pref[cnt] = total;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < total; t += kBlock) {
+      uint32_t lo = 0, hi = cnt; /* s_pref[lo] <= t < s_pref[hi] */
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pref[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const QEntry e = entry(lo);
+      const int64_t j = e.start + (int64_t)(t - s_pref[lo]);
+      const uint32_t u = (uint32_t)e.v, v = (uint32_t)b.cols[j];
+      /* distances lowered earlier in the burst were lowered by atomics at L2:
+       * read them with agent-scope loads, past any older copy in L1 */
+      const uint32_t du = __hip_atomic_load(&b.dist[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t nb = DistOps<W>::bits(DistOps<W>::value(du) + b.w[j]);
+      if (nb < __hip_atomic_load(&b.dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+          nb < atomic_min(&b.dist[v], nb)) {
+        const uint64_t bit = 1ull << (v & 63);
+        if (!(atomic_or(&b.dirty[v >> 6], bit) & bit)) {
+          const int64_t s0 = (int64_t)b.off[v], d = (int64_t)(b.off[v + 1] - b.off[v]);
+          if (d > 0) { /* a vertex without arcs stays dirty for the selection */
+            const uint32_t p = atomicAdd(&s_next, 1u);
+            if (p < (uint32_t)kSsspBurstMax) {
+              QEntry q;
+              q.start = s0;
+              q.v = (int32_t)v;
+              q.len = (int32_t)d;
+              out[p] = q;
+            }
+            atomicAdd(&s_next_arcs, (uint32_t)(d < (int64_t)kSsspBurstMax ? d : kSsspBurstMax));
+          }
+        }
+      }
+    }
+    __syncthreads();
+    arcs += total;
+    ++rounds;
+    const uint32_t nc = s_next, na = s_next_arcs;
+    /* hand back (the next list keeps its dirty bits) or stop */
+    if (nc == 0 || nc > (uint32_t)kSsspBurstMax || (int64_t)na > b.burst_arcs) break;
+    for (uint32_t i = threadIdx.x; i < nc; i += kBlock) {
+      const uint32_t v = (uint32_t)out[i].v;
+      atomic_and(&b.dirty[v >> 6], ~(1ull << (v & 63)));
+    }
+    __syncthreads();
+    cnt = nc;
+  }
+  if (threadIdx.x == 0) {
+    /* the selection counted the first round; the relax launch has nothing left */
+    ctl->rounds += rounds - 1;
+    ctl->relaxed += arcs - (unsigned long long)h.deg;
+    ctl->relax = 0;
+    ctl->burst = 0;
+    atomic_add(&b.relax_slots[0], arcs);
+  }
+}
+
+template<typename OffT, typename W>
+__global__ void __launch_bounds__(kBlock) near_write_kernel(CompactArgs<OffT> a, SsspBurst<OffT, W> b, SsspCtl* ctl) {
   if (ctl->done || !ctl->relax || ctl->pull) return;
-  compact_write_body<OffT>(a, ctl->heavy ? 1 : 0);
+  const bool burst = ctl->burst != 0;
+  const bool wrote = compact_write_body<OffT>(a, ctl->heavy ? 1 : 0);
+  /* a burst round has at most burst_arcs arcs, so few waves write entries and publish them */
+  if (!burst || !last_to_arrive<false>(b.arrive, wrote)) return;
+  sssp_burst(b, ctl);
 }
 
 /* Threshold after a dry iteration: the smallest deferred distance plus
@@ -360,6 +500,8 @@ struct SelectArgs {
   const unsigned long long* heavy_total;
   double delta, w_max;
   int32_t pull_mode;
+  int32_t bellman;    /* the reference's schedule (no buckets) */
+  int64_t burst_arcs; /* rounds of the reference's schedule with at most this many arcs run as a burst (0: never) */
 };
 
 /* One iteration's selection, count and decision in one launch.  Each
@@ -393,6 +535,7 @@ __global__ void __launch_bounds__(kBlock) sssp_select_kernel(CompactArgs<OffT> c
   e.near_v = (int64_t)tot.nvert;
   e.near_e = (int64_t)tot.deg;
   ctl->pull = 0;
+  ctl->burst = 0;
   if (heavy_next) { /* this iteration relaxes the heavy arcs of the settled set */
     ctl->heavy_next = 0;
     ctl->r_pending = 0;
@@ -414,6 +557,7 @@ __global__ void __launch_bounds__(kBlock) sssp_select_kernel(CompactArgs<OffT> c
     ctl->r_pending = 1;
     ctl->rounds += 1;
     ctl->relaxed += tot.deg;
+    ctl->burst = (sa.bellman && (int64_t)tot.deg <= sa.burst_arcs) ? 1 : 0;
   } else {
     ctl->heavy = 0;
     ctl->relax = 0;
@@ -503,7 +647,9 @@ struct SsspWorkspace {
   SplitSlot* slots = nullptr;
   SsspCtl* ctl = nullptr;
   unsigned long long* relax_slots = nullptr; /* kSlots x 16 (one cache line each) */
-  uint32_t* arrive = nullptr;                /* arrival counters of the selection launch, zero between launches */
+  uint32_t* arrive = nullptr; /* arrival counters of the selection and write launches (kArriveWords each), zero
+                                 between launches */
+  QEntry* burst_list[2] = {nullptr, nullptr}; /* kSsspBurstMax entries each */
   RoundLog* log = nullptr; /* kRing entries, pinned coherent host memory */
   hipEvent_t ev[kRing] = {};
   QueueSet qs;
@@ -521,6 +667,8 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->ctl);
   (void)hipFree(w->relax_slots);
   (void)hipFree(w->arrive);
+  (void)hipFree(w->burst_list[0]);
+  (void)hipFree(w->burst_list[1]);
   if (w->log) (void)hipHostFree(w->log);
   for (hipEvent_t e : w->ev)
     if (e) (void)hipEventDestroy(e);
@@ -546,8 +694,10 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
         (st = sg::alloc_device(dev, (void**)&ws->slots, sizeof(SplitSlot) * kSlots)) ||
         (st = sg::alloc_device(dev, (void**)&ws->ctl, sizeof(SsspCtl))) ||
         (st = sg::alloc_device(dev, (void**)&ws->relax_slots, 8 * 16 * kSlots)) ||
-        (st = sg::alloc_device(dev, (void**)&ws->arrive, 4 * kArriveWords)) ||
-        (st = sg::zero_async(dev, ws->arrive, 4 * kArriveWords)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->arrive, 8 * kArriveWords)) ||
+        (st = sg::zero_async(dev, ws->arrive, 8 * kArriveWords)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->burst_list[0], sizeof(QEntry) * kSsspBurstMax)) ||
+        (st = sg::alloc_device(dev, (void**)&ws->burst_list[1], sizeof(QEntry) * kSsspBurstMax)) ||
         (st = ws->qs.init(dev, n, g->m, default_tuning()))) {
       sssp_ws_free(ws);
       return st;
@@ -590,7 +740,8 @@ int ensure_split(sg_graph_t g, W dlt) {
 }
 
 template<typename OffT, typename W>
-int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, W* dist_out, sg_sssp_stats* stats) {
+int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_t burst_arcs, W* dist_out,
+             sg_sssp_stats* stats) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   const int64_t n = g->n;
@@ -672,13 +823,28 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, W* dis
   sa.delta = bellman ? 0.0 : delta;
   sa.w_max = w_max;
   sa.pull_mode = pull_mode;
+  sa.bellman = bellman ? 1 : 0;
+  sa.burst_arcs = burst_arcs;
+  SsspBurst<OffT, W> sb{};
+  sb.off = off;
+  sb.cols = g->cols;
+  sb.w = (const W*)g->weights;
+  sb.dist = dist;
+  sb.dirty = dirty;
+  for (int k = 0; k < 3; ++k) sb.q[k] = Q.q[0][k];
+  sb.hdr = Q.hdr;
+  sb.list[0] = ws->burst_list[0];
+  sb.list[1] = ws->burst_list[1];
+  sb.burst_arcs = burst_arcs;
+  sb.relax_slots = ws->relax_slots;
+  sb.arrive = ws->arrive + kArriveWords;
   split_slots_reset_kernel<<<1, 64, 0, s>>>(ws->slots, ws->ctl); /* later iterations clear them as they decide */
   /* one iteration, three launches: split + near-set count + decision (relax /
    * next bucket / stop); queue write; relax.  Everything the host needs is in
    * the iteration's record. */
   auto enqueue = [&](int32_t it) -> int {
     sssp_select_kernel<OffT, W><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, sa, ws->ctl, it);
-    near_write_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl);
+    near_write_kernel<OffT, W><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, sb, ws->ctl);
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
@@ -783,7 +949,8 @@ int mean_weight(sg_graph_t g, double* out) {
 constexpr int64_t kAutoBucketArcs = int64_t(1) << 25;
 
 template<typename OffT, typename W>
-int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, W* dist, sg_sssp_stats* stats) {
+int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, int64_t burst_arcs, W* dist,
+                  sg_sssp_stats* stats) {
   if (delta == 0.0 && g->m < kAutoBucketArcs) {
     delta = -1.0;
   } else if (delta == 0.0) {
@@ -791,7 +958,7 @@ int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull
     SG_TRY(mean_weight<W>(g, &mw));
     delta = std::max(kAutoDeltaFactor * mw, DistOps<W>::is_float ? 1e-30 : 1.0);
   }
-  return run_sssp<OffT, W>(g, source, delta, heavy_pull, dist, stats);
+  return run_sssp<OffT, W>(g, source, delta, heavy_pull, burst_arcs, dist, stats);
 }
 
 extern "C" {
@@ -815,6 +982,8 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void
            "sg_sssp: unknown load balancer %d", opt.load_balancer);
   SG_CHECK(opt.heavy_pull >= -1 && opt.heavy_pull <= 1, SG_ERR_INVALID, "sg_sssp: heavy_pull %d not in {-1, 0, 1}",
            opt.heavy_pull);
+  SG_CHECK(opt.burst_arcs <= kSsspBurstMax, SG_ERR_INVALID, "sg_sssp: burst_arcs %d above %d", opt.burst_arcs,
+           kSsspBurstMax);
   if (stats) {
     stats->rounds = 0;
     stats->reached_vertices = stats->reached_edges = stats->relaxations = 0;
@@ -822,11 +991,12 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void
   }
   SG_HIP(hipSetDevice(g->dev->ordinal));
   const bool fw = (g->flags & SG_GRAPH_WEIGHT_FLOAT) != 0;
+  const int64_t ba = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kSsspBurstDefault : opt.burst_arcs);
   if (g->offset_bytes == 8)
-    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, opt.heavy_pull, (float*)dist, stats)
-              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, (uint32_t*)dist, stats);
-  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, opt.heavy_pull, (float*)dist, stats)
-            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, (uint32_t*)dist, stats);
+    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, opt.heavy_pull, ba, (float*)dist, stats)
+              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, (uint32_t*)dist, stats);
+  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, opt.heavy_pull, ba, (float*)dist, stats)
+            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, (uint32_t*)dist, stats);
 }
 
 int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats) {

@@ -505,8 +505,8 @@ __device__ __forceinline__ bool compact_write_body_at(const CompactArgs<OffT>& a
 }
 
 template<typename OffT>
-__device__ __forceinline__ void compact_write_body(const CompactArgs<OffT>& a, int phase = 0) {
-  compact_write_body_at(a, phase, (int64_t)blockIdx.x, (int)threadIdx.x);
+__device__ __forceinline__ bool compact_write_body(const CompactArgs<OffT>& a, int phase = 0) {
+  return compact_write_body_at(a, phase, (int64_t)blockIdx.x, (int)threadIdx.x);
 }
 
 template<typename OffT>

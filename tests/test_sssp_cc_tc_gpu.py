@@ -193,6 +193,41 @@ def test_sssp_heavy_pull_directed_pushes(sg, device, oracle_mod):
         s.run(heavy_pull=2)
 
 
+@pytest.mark.parametrize("burst_arcs", [-1, 0, 64, 8192])
+def test_sssp_bursts_keep_distances(sg, device, oracle_mod, burst_arcs):
+    """Small rounds of the reference's schedule run inside one workgroup
+    (never, default, tiny and largest thresholds): same distances on a long
+    grid, a weighted path, a small RMAT (hubs in small rounds) and an ER graph
+    with several components."""
+    path = oracle_mod.grid(1, 3000, 0.0, seed=1)
+    graphs = [oracle_mod.grid(120, 90, 0.1, seed=53), path, oracle_mod.rmat(11, 16, seed=54),
+              oracle_mod.erdos_renyi(4000, 5000, seed=55)]
+    for g in graphs:
+        oracle_mod.fill_weights(g, 13)
+        G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=g.weights)
+        s = sg.SSSP(G)
+        for src in (0, g.n // 3, int(np.argmax(g.degrees()))):
+            s.init(src)
+            st = s.run(delta=-1.0, burst_arcs=burst_arcs)
+            assert np.array_equal(s.get_distances(), oracle_mod.sssp(g, src)), (g.n, src)
+            assert st.rounds >= 1
+
+
+def test_sssp_float_bursts_match_launched_rounds(sg, device, oracle_mod):
+    g = oracle_mod.grid(80, 80, 0.05, seed=56)
+    w = oracle_mod.fill_weights(g, 14)
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, weights=(w / 3.0).astype(np.float32), weight_float=True)
+    s = sg.SSSP(G)
+    got = {}
+    for ba in (-1, 0):
+        s.init(17)
+        s.run(delta=-1.0, burst_arcs=ba)
+        got[ba] = s.get_distances()
+    assert np.array_equal(got[-1].view(np.uint32), got[0].view(np.uint32))
+    with pytest.raises(sg.SygraphError):
+        s.run(burst_arcs=8193)
+
+
 def test_sssp_float_weights(sg, device, oracle_mod):
     g = oracle_mod.rmat(11, 8, seed=41)
     w = oracle_mod.fill_weights(g, 5)
