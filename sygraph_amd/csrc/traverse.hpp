@@ -297,13 +297,18 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, i
   }
   uint32_t c0 = 0, c1 = 0, c2 = 0, nv = 0;
   uint64_t deg = 0, mx = 0;
-  uint64_t todo = ballot(mynew != 0);
-  while (todo) {
-    const int k = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t word = shfl_u64(mynew, k);
-    if ((word >> lane) & 1ull) {
-      const int64_t v = (tile * 64 + k) * 64 + lane;
+  /* the tile's set bits, flattened in vertex order (a scan of the words'
+   * popcounts): 64 vertices per step however sparse the words are */
+  const uint32_t pc = (uint32_t)__popcll(mynew);
+  const uint32_t incl = wave_inclusive_scan(pc);
+  const uint32_t total = __shfl(incl, 63, 64);
+  for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const int owner = scan_owner(incl, t);
+    const uint32_t before = __shfl(incl - pc, owner, 64);
+    const uint64_t word = shfl_u64(mynew, owner);
+    if (t < total) {
+      const int64_t v = (tile * 64 + owner) * 64 + select_bit(word, t - before);
       if (kMode == kFromDiff && a.depth) a.depth[v] = depth_val;
       int64_t s0, d;
       entry_range(a, v, phase, s0, d);
@@ -466,14 +471,18 @@ __device__ __forceinline__ bool compact_write_body_at(const CompactArgs<OffT>& a
     pos1 += tc.c[1];
     pos2 += tc.c[2];
   }
-  uint64_t todo = ballot(mynew != 0);
-  const bool wrote = todo != 0;
-  while (todo) {
-    const int k = __ffsll((unsigned long long)todo) - 1;
-    todo &= todo - 1;
-    const uint64_t word = shfl_u64(mynew, k);
-    const int64_t v = (tile * 64 + k) * 64 + lane;
-    const bool has = (word >> lane) & 1ull;
+  /* the same flattened vertex order as the count pass */
+  const uint32_t pc = (uint32_t)__popcll(mynew);
+  const uint32_t incl_b = wave_inclusive_scan(pc);
+  const uint32_t total = __shfl(incl_b, 63, 64);
+  const bool wrote = total != 0;
+  for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+    const uint32_t t = t0 + lane;
+    const int owner = scan_owner(incl_b, t);
+    const uint32_t before = __shfl(incl_b - pc, owner, 64);
+    const uint64_t word = shfl_u64(mynew, owner);
+    const bool has = t < total;
+    const int64_t v = has ? (tile * 64 + owner) * 64 + select_bit(word, t - before) : 0;
     int64_t s = 0, d = 0;
     if (has) entry_range(a, v, phase, s, d);
     const int cls = (has && d > 0) ? degree_class(a.tune, d) : -1;
