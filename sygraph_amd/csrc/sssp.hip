@@ -276,8 +276,10 @@ struct alignas(128) SplitSlot {
 
 /* near = dirty vertices with distance < T (moved out of the dirty bitmap into
  * the round's frontier bitmap); count and smallest distance of the rest.  One
- * word per thread (word i); the heavy iteration's frontier is the bucket's
- * settled set instead.  All threads of the workgroup call. */
+ * word per thread (word i), its dirty bits walked by the lane itself: on the
+ * dense bands of a grid this beats spreading them over the wave.  The heavy
+ * iteration's frontier is the bucket's settled set instead.  All threads of
+ * the workgroup call. */
 template<typename W>
 __device__ __forceinline__ void split_word(int64_t i, int64_t w0, bool heavy_next, uint32_t T_bits, uint64_t* dirty,
                                            uint64_t* near_bm, uint64_t* settled, uint64_t* settled_all,

@@ -229,6 +229,51 @@ __device__ __forceinline__ int degree_class(const Tuning& t, int64_t d) {
   return d >= t.t_large ? 2 : (d >= t.t_medium ? 1 : 0);
 }
 
+/* A wave's walk over the set bits of its 64 words (lane i holds word i), in
+ * vertex order, one vertex per lane and step.  Word by word (a step per
+ * non-empty word, lane = bit) suits dense words; flattened (a step per 64 set
+ * bits, the owner word found by a search of the popcount scan) suits sparse
+ * ones; the walk takes the one with fewer steps.  All lanes call. */
+struct TileWalk {
+  uint64_t words;  /* ballot of the non-empty words (word by word) */
+  uint32_t incl;   /* inclusive scan of the popcounts (flattened) */
+  uint32_t pc;     /* this lane's popcount */
+  uint32_t total;  /* set bits of the tile */
+  uint32_t steps;
+  bool flat;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(uint64_t mine) {
+  TileWalk w;
+  w.words = ballot(mine != 0);
+  w.pc = (uint32_t)__popcll(mine);
+  w.incl = wave_inclusive_scan(w.pc);
+  w.total = __shfl(w.incl, 63, 64);
+  const uint32_t flat_steps = (w.total + 63) / 64, word_steps = (uint32_t)__popcll(w.words);
+  w.flat = flat_steps < word_steps;
+  w.steps = w.flat ? flat_steps : word_steps;
+  return w;
+}
+
+/* Vertex of this lane at `step` (false: none); steps in increasing order. */
+__device__ __forceinline__ bool tile_walk_item(TileWalk& w, uint64_t mine, int64_t tile, uint32_t step, int64_t& v) {
+  const int lane = lane_id();
+  if (w.flat) {
+    const uint32_t t = step * 64 + lane;
+    const int owner = scan_owner(w.incl, t);
+    const uint32_t before = __shfl(w.incl - w.pc, owner, 64);
+    const uint64_t word = shfl_u64(mine, owner);
+    if (t >= w.total) return false;
+    v = (tile * 64 + owner) * 64 + select_bit(word, t - before);
+    return true;
+  }
+  const int k = __ffsll((unsigned long long)w.words) - 1; /* the step-th non-empty word */
+  w.words &= w.words - 1;
+  const uint64_t word = shfl_u64(mine, k);
+  v = (tile * 64 + k) * 64 + lane;
+  return (word >> lane) & 1ull;
+}
+
 /* ------------------------------------------------------- compaction --- */
 enum CompactMode {
   kFromDiff = 0, /* frontier = visited & ~prev; writes frontier bitmap and prev */
@@ -297,18 +342,13 @@ __device__ __forceinline__ void compact_count_body(const CompactArgs<OffT>& a, i
   }
   uint32_t c0 = 0, c1 = 0, c2 = 0, nv = 0;
   uint64_t deg = 0, mx = 0;
-  /* the tile's set bits, flattened in vertex order (a scan of the words'
-   * popcounts): 64 vertices per step however sparse the words are */
-  const uint32_t pc = (uint32_t)__popcll(mynew);
-  const uint32_t incl = wave_inclusive_scan(pc);
-  const uint32_t total = __shfl(incl, 63, 64);
-  for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-    const uint32_t t = t0 + lane;
-    const int owner = scan_owner(incl, t);
-    const uint32_t before = __shfl(incl - pc, owner, 64);
-    const uint64_t word = shfl_u64(mynew, owner);
-    if (t < total) {
-      const int64_t v = (tile * 64 + owner) * 64 + select_bit(word, t - before);
+  /* the tile's set bits in vertex order, walked word by word (lane = bit) or,
+   * when that takes fewer steps, flattened (a scan of the words' popcounts:
+   * 64 vertices per step however sparse the words are) */
+  TileWalk tw = tile_walk(mynew);
+  for (uint32_t step = 0; step < tw.steps; ++step) {
+    int64_t v;
+    if (tile_walk_item(tw, mynew, tile, step, v)) {
       if (kMode == kFromDiff && a.depth) a.depth[v] = depth_val;
       int64_t s0, d;
       entry_range(a, v, phase, s0, d);
@@ -471,18 +511,12 @@ __device__ __forceinline__ bool compact_write_body_at(const CompactArgs<OffT>& a
     pos1 += tc.c[1];
     pos2 += tc.c[2];
   }
-  /* the same flattened vertex order as the count pass */
-  const uint32_t pc = (uint32_t)__popcll(mynew);
-  const uint32_t incl_b = wave_inclusive_scan(pc);
-  const uint32_t total = __shfl(incl_b, 63, 64);
-  const bool wrote = total != 0;
-  for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-    const uint32_t t = t0 + lane;
-    const int owner = scan_owner(incl_b, t);
-    const uint32_t before = __shfl(incl_b - pc, owner, 64);
-    const uint64_t word = shfl_u64(mynew, owner);
-    const bool has = t < total;
-    const int64_t v = has ? (tile * 64 + owner) * 64 + select_bit(word, t - before) : 0;
+  /* the same walk, so the same vertex order, as the count pass */
+  TileWalk tw = tile_walk(mynew);
+  const bool wrote = tw.total != 0;
+  for (uint32_t step = 0; step < tw.steps; ++step) {
+    int64_t v = 0;
+    const bool has = tile_walk_item(tw, mynew, tile, step, v);
     int64_t s = 0, d = 0;
     if (has) entry_range(a, v, phase, s, d);
     const int cls = (has && d > 0) ? degree_class(a.tune, d) : -1;
