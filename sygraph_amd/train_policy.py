@@ -11,7 +11,8 @@
    once per action with that action forced at every level (push or pull x
    work mapping variant 0/1/2).  The frontier sequence of a BFS does not
    depend on the actions, so level L of every run sees the same state and
-   cost(L, a) = traversal + conversion time of level L in the run forced to a.
+   cost(L, a) = traversal + conversion time of level L in the run forced to a
+   (bursts off, so every level is timed on its own).
    Each level L >= 1 becomes one sample: the features the device computes
    when it decides level L (policy.hpp), labelled with argmin_a cost(L, a).
 3. Training: minibatch Adam on the device (sg_policy_train_step) on
@@ -19,7 +20,8 @@
    action time), so the levels that dominate traversal time dominate the loss.
 4. Held-out evaluation: the summed device time of the BFS under the plain
    policy (push, bucketing), the reference heuristic (alpha 15, beta 24), the
-   learned policy, and the per-level best action (a lower bound).
+   learned policy, and the per-level best action (a lower bound), all with
+   bursts off; the three policies also with the default bursts.
 
 Writes <out>/bfs_policy.f32 (1,542 float32 parameters) and
 <out>/bfs_policy.json (sets, sample counts, training loss, held-out times).
@@ -78,14 +80,19 @@ def sources_of(G, seed: int, k: int = 4):
     return [int(x) for x in rng.choice(cand, min(k, len(cand)), replace=False)]
 
 
+NO_BURSTS = {"burst_arcs": -1}  # every level its own launches, so each level's time is its own
+
+
 def level_costs(G, src: int):
     """cost[L][a] for every action a forced at every level, and the stats of
-    the first run (the frontier sequence is the same in all runs)."""
+    the first run (the frontier sequence is the same in all runs).  Bursts
+    are off: a burst times its levels together, so their costs could not be
+    told apart."""
     b = sg.BFS(G)
     costs, stats0 = [], None
     for pol, lb in ACTIONS:
         b.init(src)
-        st = b.run(pol, 15.0, 24.0, load_balancer=lb)
+        st = b.run(pol, 15.0, 24.0, load_balancer=lb, tuning=NO_BURSTS)
         c = [st.level[i].kernel_ms + st.level[i].convert_ms for i in range(st.levels)]
         costs.append(c)
         if stats0 is None:
@@ -148,6 +155,9 @@ def train(dev, X, Y, C, steps: int, lr: float, batch: int = 8192, seed: int = 0)
 
 
 def evaluate(dev, pol, seeds, quick):
+    """Device time of whole traversals (best of three) under each policy,
+    without bursts (the setting of the per-level costs, so per_level_best
+    bounds them) and, as `<policy>_bursts`, with the default bursts."""
     rows = []
     for seed in seeds:
         name, G = make_graph(dev, seed, quick)
@@ -156,12 +166,13 @@ def evaluate(dev, pol, seeds, quick):
             t = {}
             for key, kw in [("plain", dict(policy=sg.POLICY_PUSH)), ("heuristic", dict(policy=sg.POLICY_HYBRID)),
                             ("learned", dict(policy=sg.POLICY_LEARNED, learned=pol))]:
-                best = None
-                for _ in range(3):  # best of three, device time of the whole traversal
-                    b.init(src)
-                    st = b.run(alpha=15.0, beta=24.0, **kw)
-                    best = st.kernel_ms if best is None else min(best, st.kernel_ms)
-                t[key] = best
+                for suffix, tuning in (("", NO_BURSTS), ("_bursts", None)):
+                    best = None
+                    for _ in range(3):
+                        b.init(src)
+                        st = b.run(alpha=15.0, beta=24.0, tuning=tuning, **kw)
+                        best = st.kernel_ms if best is None else min(best, st.kernel_ms)
+                    t[key + suffix] = best
             cost, _ = level_costs(G, src)
             t["per_level_best"] = float(cost.min(axis=1).sum())
             rows.append({"seed": seed, "graph": name, "source": src, **t})
@@ -192,13 +203,18 @@ def main():
     regret = float((C[np.arange(len(Y)), pred] - C.min(axis=1)).sum() / C.min(axis=1).sum())
     log(f"train: loss {losses[0]:.3f} -> {losses[-1]:.3f}, accuracy {acc:.3f}, time regret {regret:.3%}")
     rows = evaluate(dev, pol, test_seeds, a.quick)
-    tot = {k: float(sum(r[k] for r in rows)) for k in ["plain", "heuristic", "learned", "per_level_best"]}
+    tot = {k: float(sum(r[k] for r in rows)) for k in ["plain", "heuristic", "learned", "per_level_best",
+                                                        "plain_bursts", "heuristic_bursts", "learned_bursts"]}
     geo = lambda num, den: float(np.exp(np.mean([np.log(r[num] / r[den]) for r in rows])))  # noqa: E731
     summary = {"held_out_total_ms": tot,
                "learned_speedup_vs_plain_total": tot["plain"] / tot["learned"],
                "learned_speedup_vs_heuristic_total": tot["heuristic"] / tot["learned"],
                "learned_speedup_vs_plain_geomean": geo("plain", "learned"),
-               "learned_speedup_vs_heuristic_geomean": geo("heuristic", "learned")}
+               "learned_speedup_vs_heuristic_geomean": geo("heuristic", "learned"),
+               "with_bursts": {"learned_speedup_vs_plain_total": tot["plain_bursts"] / tot["learned_bursts"],
+                               "learned_speedup_vs_heuristic_total": tot["heuristic_bursts"] / tot["learned_bursts"],
+                               "learned_speedup_vs_plain_geomean": geo("plain_bursts", "learned_bursts"),
+                               "learned_speedup_vs_heuristic_geomean": geo("heuristic_bursts", "learned_bursts")}}
     log(json.dumps(summary, indent=1))
     os.makedirs(a.out, exist_ok=True)
     pol.get_params().astype(np.float32).tofile(os.path.join(a.out, "bfs_policy.f32"))
