@@ -281,9 +281,9 @@ struct alignas(128) SplitSlot {
  * iteration's frontier is the bucket's settled set instead.  All threads of
  * the workgroup call. */
 template<typename W>
-__device__ __forceinline__ void split_word(int64_t i, int64_t w0, bool heavy_next, uint32_t T_bits, uint64_t* dirty,
-                                           uint64_t* near_bm, uint64_t* settled, uint64_t* settled_all,
-                                           const uint32_t* dist, SplitSlot* slots) {
+__device__ __forceinline__ void split_word(int64_t i, int64_t w0, bool heavy_next, bool bellman, uint32_t T_bits,
+                                           uint64_t* dirty, uint64_t* near_bm, uint64_t* settled,
+                                           uint64_t* settled_all, const uint32_t* dist, SplitSlot* slots) {
   if (heavy_next) {
     if (i < w0) {
       const uint64_t r = settled[i];
@@ -291,6 +291,17 @@ __device__ __forceinline__ void split_word(int64_t i, int64_t w0, bool heavy_nex
       if (r) {
         settled[i] = 0;
         settled_all[i] |= r;
+      }
+    }
+    return;
+  }
+  if (bellman) { /* the reference's schedule: T is the largest weight, so every dirty vertex is near */
+    if (i < w0) {
+      const uint64_t f = dirty[i];
+      near_bm[i] = f;
+      if (f) {
+        dirty[i] = 0;
+        settled[i] |= f;
       }
     }
     return;
@@ -525,8 +536,8 @@ __global__ void __launch_bounds__(kBlock) sssp_select_kernel(CompactArgs<OffT> c
   __shared__ QueueHdr tot;
   if (ctl->done) return;
   const bool heavy_next = ctl->heavy_next != 0;
-  split_word<W>((int64_t)blockIdx.x * kBlock + threadIdx.x, sa.w0, heavy_next, ctl->T_bits, sa.dirty, sa.near_bm,
-                sa.settled, sa.settled_all, sa.dist, sa.slots);
+  split_word<W>((int64_t)blockIdx.x * kBlock + threadIdx.x, sa.w0, heavy_next, sa.bellman != 0, ctl->T_bits,
+                sa.dirty, sa.near_bm, sa.settled, sa.settled_all, sa.dist, sa.slots);
   compact_count_body<OffT, kFromBitmap>(ca, heavy_next ? 1 : 0);
   if (!last_to_arrive<true>(sa.arrive, false)) return;
   compact_scan_body_t<kBlock, 8, true>(ca.blocks, sa.nblocks, ca.hdr, &tot);
