@@ -80,90 +80,148 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t x, uint32_t mask) {
   return ((uint32_t)x * 2654435761u >> 7) & mask; /* Knuth's multiplicative hash */
 }
 
-/* Open-addressing membership in the wave's LDS table (empty slots hold -1). */
-__device__ __forceinline__ bool hash_contains(const int32_t* table, uint32_t mask, int32_t x) {
-  for (uint32_t h = hash_slot(x, mask);; h = (h + 1) & mask) {
-    const int32_t y = table[h];
-    if (y == x) return true;
-    if (y < 0) return false;
-  }
-}
+/* One wave per vertex u.  N+(u) goes into an open-addressing LDS hash table
+ * (linear probing, empty slots hold -1) of at least twice its size (lists up
+ * to kStage ids; longer ones are binary-searched in global memory, where they
+ * are sorted).  The wave then looks up every w of the lists N+(v), v in
+ * N+(u): w in N+(u) closes the triangle (u, v, w), counted once at its
+ * lowest-ranked corner.  Lists of 64 or more arcs are walked one at a time by
+ * the whole wave; the shorter ones, 64 at a time laid end to end (merge path).
+ * Waves claim vertices kTcChunk at a time from a counter (next[0], zero at
+ * launch): the work per vertex is skewed, and a fixed assignment leaves the
+ * busiest wave with several times the mean. */
+constexpr int kTcChunk = 4;
 
-/* One wave per vertex u.  N+(u) goes into an LDS hash table of at least
- * twice its size (lists up to kStage ids; longer ones are binary-searched in
- * global memory, where they are sorted).  The wave then walks the lists
- * N+(v), v in N+(u), 64 of them laid end to end at a time (merge path), a lane
- * per arc (v, w): w in N+(u) closes the triangle (u, v, w), counted once at
- * its lowest-ranked corner. */
 __global__ void __launch_bounds__(kBlock) intersect_kernel(const int64_t* dag_off, const int32_t* dag_cols, int64_t n,
-                                                           unsigned long long* per_vertex, unsigned long long* total) {
+                                                           unsigned long long* per_vertex, unsigned long long* total,
+                                                           unsigned long long* next) {
   __shared__ int32_t table_all[kBlock / 64][2 * kStage];
   const int lane = lane_id();
-  const int wv = threadIdx.x >> 6;
-  int32_t* table = table_all[wv];
-  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  int32_t* table = table_all[threadIdx.x >> 6];
   uint64_t acc_total = 0;
-  for (int64_t u = gwave; u < n; u += nwaves) {
-    const int64_t ub = dag_off[u], ue = dag_off[u + 1];
-    const int32_t du = (int32_t)(ue - ub);
-    if (du < 2) {
-      if (per_vertex && lane == 0) per_vertex[u] = 0;
-      continue;
-    }
-    const bool hashed = du <= kStage;
-    uint32_t size = 64;
-    while (size < 2u * (uint32_t)du) size <<= 1;
-    const uint32_t mask = size - 1;
-    if (hashed) {
-      for (uint32_t i = lane; i < size; i += 64) table[i] = -1;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      for (int32_t i = lane; i < du; i += 64) {
-        const int32_t x = dag_cols[ub + i];
-        for (uint32_t h = hash_slot(x, mask);; h = (h + 1) & mask)
-          if (atomicCAS(&table[h], -1, x) == -1) break;
+  for (;;) {
+    unsigned long long claim = 0;
+    if (lane == 0) claim = atomicAdd(next, (unsigned long long)kTcChunk);
+    const int64_t u0 = (int64_t)shfl_u64(claim, 0);
+    if (u0 >= n) break;
+    const int64_t u1 = u0 + kTcChunk < n ? u0 + kTcChunk : n;
+    for (int64_t u = u0; u < u1; ++u) {
+      const int64_t ub = dag_off[u], ue = dag_off[u + 1];
+      const int32_t du = (int32_t)(ue - ub);
+      if (du < 2) {
+        if (per_vertex && lane == 0) per_vertex[u] = 0;
+        continue;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-    const int32_t* list = dag_cols + ub;
-    uint64_t cnt = 0;
-    for (int32_t vbase = 0; vbase < du; vbase += 64) {
-      int64_t vb = 0, vlen = 0;
-      if (vbase + lane < du) {
-        const int32_t v = list[vbase + lane];
-        vb = dag_off[v];
-        vlen = dag_off[v + 1] - vb;
+      const bool hashed = du <= kStage;
+      /* at most 1/8 full where the table allows (2 * kStage slots), so most
+       * lookups end at their first slot; at least 1/2 empty in every case */
+      uint32_t size = 64;
+      while (size < 8u * (uint32_t)du && size < 2u * kStage) size <<= 1;
+      const uint32_t mask = size - 1;
+      if (hashed) {
+        for (uint32_t i = lane; i < size; i += 64) table[i] = -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int32_t i = lane; i < du; i += 64) {
+          const int32_t x = dag_cols[ub + i];
+          for (uint32_t h = hash_slot(x, mask);; h = (h + 1) & mask)
+            if (atomicCAS(&table[h], -1, x) == -1) break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       }
-      const uint32_t incl = wave_inclusive_scan((uint32_t)vlen);
-      const uint32_t tot = __shfl(incl, 63, 64);
-      const uint32_t excl = incl - (uint32_t)vlen;
-      for (uint32_t t0 = 0; t0 < tot; t0 += kTcUnroll * 64) {
+      const int32_t* list = dag_cols + ub;
+      uint64_t cnt = 0;
+      /* members of N+(u) among kTcUnroll ids per lane */
+      auto count_hits = [&](const int32_t (&w)[kTcUnroll], const bool (&ok)[kTcUnroll]) {
+        if (hashed) {
+          /* probe rounds: the pending lookups of all kTcUnroll ids read their
+           * next slot together (independent LDS reads), so the wave waits for
+           * its longest probe sequence once instead of once per id */
+          uint32_t h[kTcUnroll];
+          uint32_t pend = 0;
+#pragma unroll
+          for (int k = 0; k < kTcUnroll; ++k) {
+            h[k] = hash_slot(w[k], mask);
+            pend |= ok[k] ? 1u << k : 0u;
+          }
+          while (pend) {
+            int32_t y[kTcUnroll];
+#pragma unroll
+            for (int k = 0; k < kTcUnroll; ++k) y[k] = (pend >> k) & 1u ? table[h[k]] : -1;
+#pragma unroll
+            for (int k = 0; k < kTcUnroll; ++k) {
+              if (!((pend >> k) & 1u)) continue;
+              if (y[k] == w[k]) {
+                ++cnt;
+                pend &= ~(1u << k);
+              } else if (y[k] < 0) {
+                pend &= ~(1u << k);
+              } else {
+                h[k] = (h[k] + 1) & mask;
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < kTcUnroll; ++k)
+            if (ok[k]) cnt += search(list, du, w[k]) ? 1u : 0u;
+        }
+      };
+      for (int32_t vbase = 0; vbase < du; vbase += 64) {
+        int64_t vb = 0, vlen = 0;
+        if (vbase + lane < du) {
+          const int32_t v = list[vbase + lane];
+          vb = dag_off[v];
+          vlen = dag_off[v + 1] - vb;
+        }
         int32_t w[kTcUnroll];
         bool ok[kTcUnroll];
+        /* long lists: the whole wave walks each in turn, lanes side by side */
+        const bool is_long = vlen >= 64;
+        uint64_t longs = ballot(is_long);
+        while (longs) {
+          const int k2 = __ffsll((unsigned long long)longs) - 1;
+          longs &= longs - 1;
+          const int64_t lb = shfl_i64(vb, k2);
+          const uint32_t ll = (uint32_t)__shfl((uint32_t)vlen, k2, 64);
+          for (uint32_t j0 = 0; j0 < ll; j0 += kTcUnroll * 64) {
 #pragma unroll
-        for (int k = 0; k < kTcUnroll; ++k) {
-          const uint32_t t = t0 + 64 * k + lane;
-          const int owner = scan_owner(incl, t);
-          const uint32_t before = __shfl(excl, owner, 64);
-          const int64_t st = shfl_i64(vb, owner);
-          ok[k] = t < tot;
-          w[k] = ok[k] ? dag_cols[st + (t - before)] : 0;
+            for (int k = 0; k < kTcUnroll; ++k) {
+              const uint32_t j = j0 + 64 * k + lane;
+              ok[k] = j < ll;
+              w[k] = ok[k] ? dag_cols[lb + j] : 0;
+            }
+            count_hits(w, ok);
+          }
         }
+        /* short lists: laid end to end, a lane per arc */
+        const uint32_t slen = is_long ? 0u : (uint32_t)vlen;
+        const uint32_t incl = wave_inclusive_scan(slen);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        const uint32_t excl = incl - slen;
+        for (uint32_t t0 = 0; t0 < tot; t0 += kTcUnroll * 64) {
 #pragma unroll
-        for (int k = 0; k < kTcUnroll; ++k)
-          if (ok[k]) cnt += (hashed ? hash_contains(table, mask, w[k]) : search(list, du, w[k])) ? 1u : 0u;
+          for (int k = 0; k < kTcUnroll; ++k) {
+            const uint32_t t = t0 + 64 * k + lane;
+            const int owner = scan_owner(incl, t);
+            const uint32_t before = __shfl(excl, owner, 64);
+            const int64_t st = shfl_i64(vb, owner);
+            ok[k] = t < tot;
+            w[k] = ok[k] ? dag_cols[st + (t - before)] : 0;
+          }
+          count_hits(w, ok);
+        }
       }
+      cnt = wave_sum_u64(cnt);
+      if (lane == 0 && per_vertex) per_vertex[u] = cnt;
+      acc_total += cnt;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    cnt = wave_sum_u64(cnt);
-    if (lane == 0 && per_vertex) per_vertex[u] = cnt;
-    acc_total += cnt;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
   if (lane == 0 && acc_total) atomicAdd(total, (unsigned long long)acc_total);
 }
@@ -186,7 +244,7 @@ extern "C" int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex) {
   size_t tmp_bytes = 0;
   SG_TRY(sg::graph_buffer(g, "tc_outdeg", 4 * (n + 1), &outdeg));
   SG_TRY(sg::graph_buffer(g, "tc_dag_off", 8 * (n + 1), &dag_off));
-  SG_TRY(sg::graph_buffer(g, "tc_total", 8, &total));
+  SG_TRY(sg::graph_buffer(g, "tc_total", 16, &total)); /* the count, then the claim counter */
   const int grid = dev->cus * 8;
   if (g->offset_bytes == 8)
     orient_kernel<int64_t, false><<<grid, kBlock, 0, s>>>((const int64_t*)g->offsets, g->cols, n, outdeg, nullptr, nullptr);
@@ -212,8 +270,8 @@ extern "C" int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex) {
   else
     orient_kernel<uint32_t, true><<<grid, kBlock, 0, s>>>((const uint32_t*)g->offsets, g->cols, n, nullptr, dag_off,
                                                           dag_cols);
-  SG_TRY(sg::zero_async(dev, total, 8));
-  intersect_kernel<<<grid, kBlock, 0, s>>>(dag_off, dag_cols, n, (unsigned long long*)per_vertex, total);
+  SG_TRY(sg::zero_async(dev, total, 16));
+  intersect_kernel<<<grid, kBlock, 0, s>>>(dag_off, dag_cols, n, (unsigned long long*)per_vertex, total, total + 1);
   SG_HIP(hipGetLastError());
   unsigned long long h = 0;
   SG_HIP(hipMemcpyAsync(&h, total, 8, hipMemcpyDeviceToHost, s));
