@@ -28,7 +28,7 @@ using namespace sg::trav;
 namespace {
 
 constexpr int kStage = 1024; /* longest N+(u) hashed in LDS (table of 2 * kStage slots per wave) */
-constexpr int kTcUnroll = 8;  /* arcs (v, w) per lane in flight */
+constexpr int kTcUnroll = 4;  /* arcs (v, w) per lane in flight */
 
 template<typename OffT>
 __device__ __forceinline__ bool ranks_higher(const OffT* off, uint32_t u, int64_t du, uint32_t v) {
@@ -90,7 +90,7 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t x, uint32_t mask) {
  * Waves claim vertices kTcChunk at a time from a counter (next[0], zero at
  * launch): the work per vertex is skewed, and a fixed assignment leaves the
  * busiest wave with several times the mean. */
-constexpr int kTcChunk = 4;
+constexpr int kTcChunk = 1;
 
 __global__ void __launch_bounds__(kBlock) intersect_kernel(const int64_t* dag_off, const int32_t* dag_cols, int64_t n,
                                                            unsigned long long* per_vertex, unsigned long long* total,
