@@ -113,10 +113,10 @@ __global__ void __launch_bounds__(kBlock) intersect_kernel(const int64_t* dag_of
         continue;
       }
       const bool hashed = du <= kStage;
-      /* at most 1/8 full where the table allows (2 * kStage slots), so most
+      /* at most 1/16 full where the table allows (2 * kStage slots), so most
        * lookups end at their first slot; at least 1/2 empty in every case */
       uint32_t size = 64;
-      while (size < 8u * (uint32_t)du && size < 2u * kStage) size <<= 1;
+      while (size < 16u * (uint32_t)du && size < 2u * kStage) size <<= 1;
       const uint32_t mask = size - 1;
       if (hashed) {
         for (uint32_t i = lane; i < size; i += 64) table[i] = -1;
