@@ -407,12 +407,81 @@ uint64_t or_tc(int64_t n, const int64_t* offsets, const int32_t* cols) {
 /* and path counts sigma, then dependencies                                   */
 /*   delta(v) = sum_{w in N(v), d(w)=d(v)+1} sigma(v)/sigma(w) * (1+delta(w))  */
 /* are accumulated from the deepest level up; bc(v) += delta(v) for v != s.   */
-/* sigma is stored scaled by a power of two per level so that the largest     */
-/* value of each level lies in [0.5,1): with S_d the product of the scales up  */
-/* to level d, sigma(v)/sigma(w) = sig(v)/sig(w) * scale_{d+1}.  Power-of-two */
-/* scaling is exact, so this equals unscaled FP64 Brandes wherever the latter  */
-/* does not overflow, and stays finite on high-diameter grids where it would. */
+/* Path counts outgrow every floating-point range on long-diameter graphs     */
+/* (about 2^8190 on a 4096x4096 grid) and differ by more than 2^1074 within   */
+/* one level, so sigma is kept per vertex as m * 2^e (m in [0.5,1), e an      */
+/* int64); sigma(v)/sigma(w) = m(v)/m(w) * 2^(e(v)-e(w)).  Powers of two are  */
+/* exact, so this equals unscaled FP64 Brandes wherever that is finite.       */
+/* Sources are taken in chunks of 16, each chunk summed in source order into  */
+/* its own array, the arrays added in chunk order: the result does not depend */
+/* on the number of threads.                                                  */
 /* ------------------------------------------------------------------------- */
+static void bc_one_source(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t* in_offsets,
+                          const int32_t* in_cols, int32_t s, int32_t* depth, int32_t* order, int64_t* lvl_start,
+                          double* sm, int64_t* se, double* dl, double* bc) {
+  for (int64_t i = 0; i < n; ++i) {
+    depth[i] = -1;
+    dl[i] = 0.0;
+  }
+  int64_t head = 0, tail = 0;
+  int32_t nl = 0;
+  depth[s] = 0;
+  sm[s] = 0.5;
+  se[s] = 1;
+  order[tail++] = s;
+  lvl_start[0] = 0;
+  while (head < tail) {
+    int64_t lend = tail;
+    lvl_start[++nl] = lend; /* level nl-1 occupies [lvl_start[nl-1], lend) */
+    for (; head < lend; ++head) {
+      int32_t x = order[head];
+      for (int64_t e = offsets[x]; e < offsets[x + 1]; ++e) {
+        int32_t y = cols[e];
+        if (depth[y] < 0) {
+          depth[y] = depth[x] + 1;
+          order[tail++] = y;
+        }
+      }
+    }
+    /* sigma of the new level, pulled from the previous level in CSR order:
+     * a running sum S * 2^E, rescaled when a larger exponent arrives */
+    for (int64_t i = lend; i < tail; ++i) {
+      int32_t y = order[i];
+      double acc = 0.0;
+      int64_t E = INT64_MIN;
+      for (int64_t e = in_offsets[y]; e < in_offsets[y + 1]; ++e) {
+        int32_t u = in_cols[e];
+        if (depth[u] != depth[y] - 1) continue;
+        if (se[u] > E) {
+          acc = E == INT64_MIN ? 0.0 : ldexp(acc, (int)(E - se[u] < -2000 ? -2000 : E - se[u]));
+          E = se[u];
+        }
+        acc += ldexp(sm[u], (int)(se[u] - E < -2000 ? -2000 : se[u] - E));
+      }
+      int ex = 0;
+      sm[y] = frexp(acc, &ex);
+      se[y] = E + ex;
+    }
+  }
+  /* levels 0..nl-1; level L occupies [lvl_start[L], lvl_start[L+1]) */
+  for (int32_t L = nl - 2; L >= 0; --L) {
+    for (int64_t i = lvl_start[L]; i < lvl_start[L + 1]; ++i) {
+      int32_t x = order[i];
+      double acc = 0.0;
+      for (int64_t e = offsets[x]; e < offsets[x + 1]; ++e) {
+        int32_t y = cols[e];
+        if (depth[y] == L + 1) {
+          int64_t d = se[x] - se[y];
+          acc += ldexp((1.0 + dl[y]) / sm[y], (int)(d < -2000 ? -2000 : d));
+        }
+      }
+      dl[x] = sm[x] * acc;
+    }
+  }
+  for (int64_t i = 0; i < tail; ++i)
+    if (order[i] != s) bc[order[i]] += dl[order[i]];
+}
+
 void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t* in_offsets, const int32_t* in_cols,
            int32_t nsrc, const int32_t* srcs, double* bc) {
   /* sigma is pulled over in-arcs; an undirected graph is its own inverse */
@@ -420,80 +489,36 @@ void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t
     in_offsets = offsets;
     in_cols = cols;
   }
-  int32_t* depth = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
-  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
-  int64_t* lvl_start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 2));
-  double* sig = (double*)malloc(sizeof(double) * (size_t)n);
-  double* dl = (double*)malloc(sizeof(double) * (size_t)n);
-  double* scale = (double*)malloc(sizeof(double) * (size_t)(n + 2));
-  for (int64_t i = 0; i < n; ++i) bc[i] = 0.0;
-  for (int32_t si = 0; si < nsrc; ++si) {
-    int32_t s = srcs[si];
-    for (int64_t i = 0; i < n; ++i) {
-      depth[i] = -1;
-      sig[i] = 0.0;
-      dl[i] = 0.0;
-    }
-    int64_t head = 0, tail = 0;
-    int32_t nl = 0;
-    depth[s] = 0;
-    sig[s] = 1.0;
-    order[tail++] = s;
-    scale[0] = 1.0;
-    lvl_start[0] = 0;
-    while (head < tail) {
-      int64_t lend = tail;
-      lvl_start[++nl] = lend; /* level nl-1 occupies [lvl_start[nl-1], lend) */
-      for (; head < lend; ++head) {
-        int32_t x = order[head];
-        for (int64_t e = offsets[x]; e < offsets[x + 1]; ++e) {
-          int32_t y = cols[e];
-          if (depth[y] < 0) {
-            depth[y] = depth[x] + 1;
-            order[tail++] = y;
-          }
-        }
-      }
-      if (tail > lend) {
-        /* sigma of the new level, pulled from the previous level in CSR order */
-        double mx = 0.0;
-        for (int64_t i = lend; i < tail; ++i) {
-          int32_t y = order[i];
-          double acc = 0.0;
-          for (int64_t e = in_offsets[y]; e < in_offsets[y + 1]; ++e)
-            if (depth[in_cols[e]] == depth[y] - 1) acc += sig[in_cols[e]];
-          sig[y] = acc;
-          if (acc > mx) mx = acc;
-        }
-        int ex = 0;
-        frexp(mx, &ex);
-        double sc = ldexp(1.0, -ex);
-        for (int64_t i = lend; i < tail; ++i) sig[order[i]] *= sc;
-        scale[nl] = sc; /* scale applied at level nl */
-      }
-    }
-    /* levels 0..nl-1; level L occupies [lvl_start[L], lvl_start[L+1]) */
-    for (int32_t L = nl - 2; L >= 0; --L) {
-      double sc = scale[L + 1];
-      for (int64_t i = lvl_start[L]; i < lvl_start[L + 1]; ++i) {
-        int32_t x = order[i];
-        double acc = 0.0;
-        for (int64_t e = offsets[x]; e < offsets[x + 1]; ++e) {
-          int32_t y = cols[e];
-          if (depth[y] == L + 1) acc += (1.0 + dl[y]) / sig[y];
-        }
-        dl[x] = sig[x] * acc * sc;
-      }
-    }
-    for (int64_t i = 0; i < n; ++i)
-      if (i != s && depth[i] >= 0) bc[i] += dl[i];
+  const int32_t kChunk = 16;
+  const int32_t nchunks = (nsrc + kChunk - 1) / kChunk;
+  double* part = (double*)calloc((size_t)(nchunks > 0 ? nchunks : 1) * (size_t)n, sizeof(double));
+#pragma omp parallel
+  {
+    int32_t* depth = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int64_t* lvl_start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 2));
+    double* sm = (double*)malloc(sizeof(double) * (size_t)n);
+    int64_t* se = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    double* dl = (double*)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(dynamic, 1)
+    for (int32_t c = 0; c < nchunks; ++c)
+      for (int32_t si = c * kChunk; si < nsrc && si < (c + 1) * kChunk; ++si)
+        bc_one_source(n, offsets, cols, in_offsets, in_cols, srcs[si], depth, order, lvl_start, sm, se, dl,
+                      part + (size_t)c * (size_t)n);
+    free(depth);
+    free(order);
+    free(lvl_start);
+    free(sm);
+    free(se);
+    free(dl);
   }
-  free(depth);
-  free(order);
-  free(lvl_start);
-  free(sig);
-  free(dl);
-  free(scale);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double acc = 0.0;
+    for (int32_t c = 0; c < nchunks; ++c) acc += part[(size_t)c * (size_t)n + i];
+    bc[i] = acc;
+  }
+  free(part);
 }
 
 /* Bitmap probes of one pull level (the byte model of the pull kernel).  Each

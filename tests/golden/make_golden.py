@@ -3,6 +3,7 @@
     python tests/golden/make_golden.py            # small fixtures (seconds)
     python tests/golden/make_golden.py --rmat24   # + the RMAT scale-24 summary (minutes)
     python tests/golden/make_golden.py --c3       # + grid / RMAT-20 CC and TC summaries
+    python tests/golden/make_golden.py --c4       # + batched BC / MS-BFS fixtures (minutes)
 
 reference_cases.json -- the adjacency-matrix fixtures of the reference's own
   tests (tests/include/test_utils.hpp:21-45, include/sygraph/io/matrices.hpp:21-41)
@@ -16,6 +17,13 @@ rmat14_seed1.npz -- the CPU-device configuration (RMAT scale 14, edge factor 16,
 c3_cc_tc.json -- CC labels digest / component count on the 4096x4096 grid with 10%
   of edges deleted (seed 4) and on RMAT scale 20 edge factor 32 (seed 5); exact
   triangle count of that RMAT graph.
+c4_bc.json -- batched BC / multi-source BFS configurations, 256 sources each:
+  (a) RMAT scale 20, edge factor 56, a/b/c/d = .55/.20/.20/.05, seed 6 (1,048,576
+  vertices, average degree 101.5) and (b) the 4096x4096 grid with 10% of edges
+  deleted (seed 4).  Per graph: CSR digests, the sources (uniform over vertices
+  with arcs, seed 6), the FP64 oracle's BC sum, the path-length identity
+  sum_s sum_t (d(s,t) - 1), the BC values at 2,048 sampled vertices and at the
+  64 largest, and per source the oracle BFS depth digest.
 rmat24_seed2.json -- RMAT scale 24 (seed 2): vertex/arc counts, CSR digests, 16
   sources drawn uniformly (seed 2) from the largest component, and per source
   the BFS level count, reached vertices/edges and the depth-array digest; SSSP
@@ -179,10 +187,61 @@ def write_rmat24():
         json.dump(out, f, indent=1)
 
 
+C4 = {
+    "rmat": dict(kind="rmat", scale=20, edge_factor=56, a=0.55, b=0.20, c=0.20, seed=6),
+    "grid": dict(kind="grid", rows=4096, cols=4096, p_delete=0.1, seed=4),
+}
+
+
+def c4_graph(key: str):
+    p = C4[key]
+    if p["kind"] == "rmat":
+        return oracle.rmat(p["scale"], p["edge_factor"], p["a"], p["b"], p["c"], seed=p["seed"])
+    return oracle.grid(p["rows"], p["cols"], p["p_delete"], seed=p["seed"])
+
+
+def c4_sources(g, count: int = 256, seed: int = 6) -> list[int]:
+    cand = np.flatnonzero(g.degrees() > 0)
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in cand[rng.integers(0, len(cand), size=count)]]
+
+
+def write_c4():
+    """Batched BC / MS-BFS fixtures (256 sources) for both C4 graphs."""
+    from concurrent.futures import ThreadPoolExecutor
+    out = {}
+    for key in ("rmat", "grid"):
+        t0 = time.time()
+        g = c4_graph(key)
+        srcs = c4_sources(g)
+        with ThreadPoolExecutor(8) as ex:
+            bfs = list(ex.map(lambda s: oracle.bfs(g, s), srcs))
+        ident = 0
+        for d, _ in bfs:
+            r = d[d != g.n + 1].astype(np.int64)
+            ident += int(np.maximum(r - 1, 0).sum())
+        b = oracle.bc(g, srcs)
+        rng = np.random.default_rng(7)
+        sample = np.unique(rng.integers(0, g.n, size=2048))
+        top = np.argsort(-b, kind="stable")[:64]
+        out[key] = {"params": C4[key], "n": g.n, "m": g.m, "avg_degree": g.m / g.n,
+                    "digest": [str(x) for x in g.digest()], "sources": srcs,
+                    "bc_sum": float(b.sum()), "identity_sum": ident,
+                    "sample": [int(x) for x in sample], "sample_bc": [float(x) for x in b[sample]],
+                    "top": [int(x) for x in top], "top_bc": [float(x) for x in b[top]],
+                    "levels": [int(lv) for _, lv in bfs],
+                    "depth_digest": [str(oracle.digest_i32(d)) for d, _ in bfs]}
+        print(f"c4 {key}: n={g.n} m={g.m} bc sum {b.sum():.6e} identity {ident} in {time.time() - t0:.1f}s",
+              flush=True)
+    with open(os.path.join(OUT, "c4_bc.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--rmat24", action="store_true")
     ap.add_argument("--c3", action="store_true")
+    ap.add_argument("--c4", action="store_true")
     a = ap.parse_args()
     write_reference_cases()
     write_rmat14()
@@ -190,3 +249,5 @@ if __name__ == "__main__":
         write_rmat24()
     if a.c3:
         write_c3()
+    if a.c4:
+        write_c4()

@@ -344,3 +344,30 @@ def test_golden_rmat14(oracle_mod):
     assert np.array_equal(oracle_mod.cc(g), z["cc"])
     assert oracle_mod.tc(g) == int(z["tc"])
     assert np.allclose(oracle_mod.bc(g, z["bc_sources"]), z["bc"], rtol=1e-12, atol=1e-12)
+
+
+def test_bc_path_length_identity_on_long_grids(oracle_mod):
+    """Sum over v != s of delta_s(v) is the number of interior vertices of the
+    shortest paths, sum_t (d(s,t) - 1) over reached t != s (each s-t path
+    passes through d(s,t) - 1 vertices, weighted 1/sigma_st per path).  On a
+    1024x1024 grid from a corner path counts reach ~2^2040 and differ by more
+    than 2^1074 within a level, beyond FP64 with one scale per level; the
+    per-vertex exponent keeps every score finite and the identity exact to
+    rounding.  A dropped term or a wrong exponent sign breaks the identity."""
+    g = oracle_mod.grid(1024, 1024, 0.1, seed=4)
+    srcs = [0, 123457, 777777]
+    b = oracle_mod.bc(g, srcs)
+    assert np.all(np.isfinite(b)) and b.min() >= 0.0
+    want = 0.0
+    for s in srcs:
+        d, _ = oracle_mod.bfs(g, s)
+        reached = d <= g.n
+        want += float(np.maximum(d[reached].astype(np.int64) - 1, 0).sum())
+    assert abs(b.sum() - want) <= 1e-9 * want
+    # and chunking over sources (16 per chunk, summed in chunk order) does not
+    # change a sum over 20 sources beyond rounding
+    srcs20 = list(range(0, 20 * 997, 997))
+    g2 = oracle_mod.grid(150, 150)
+    b20 = oracle_mod.bc(g2, srcs20)
+    ref = sum(oracle_mod.bc(g2, [s]) for s in srcs20)
+    assert np.allclose(b20, ref, rtol=1e-12, atol=1e-12)
