@@ -36,6 +36,9 @@ public:
     if (sg_alloc(_g.getQueue().handle(), sizeof(uint64_t) * (_g.getVertexCount() + 1), SG_SPACE_DEVICE, &p) != SG_OK)
       throw std::runtime_error(sg_last_error());
     _per_vertex = static_cast<uint64_t*>(p);
+    /* zero until run() counts, as the reference's TCInstance fills its array (tc.hpp:35-36) */
+    if (sg_fill(_g.getQueue().handle(), _per_vertex, 0, 8, _g.getVertexCount() + 1) != SG_OK)
+      throw std::runtime_error(sg_last_error());
     _ready = true;
   }
   void reset() {

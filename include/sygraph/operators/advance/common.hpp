@@ -30,7 +30,6 @@
 #include <sygraph/operators/config.hpp>
 #include <sygraph/sycl/event.hpp>
 #include <sygraph/utils/device.hpp>
-#include <sygraph/utils/kernel.hpp>
 #include <sygraph/utils/types.hpp>
 
 namespace sygraph {

@@ -1,7 +1,10 @@
 /*
  * utils/profiler.hpp -- per-tag kernel timing and MTEPS report (reference
- * utils/profiler.hpp:18-69); same print format, so reference-style logs stay
- * comparable.  Active when ENABLE_PROFILING is defined.
+ * utils/profiler.hpp:18-69).  The printed lines keep the reference's layout
+ * (" Kernel [tag x count] ... Time: t ms", "Total GPU Time: t ms",
+ * "Total Edge-Througput (MTEPS): x MTEPS"), so logs of both stay comparable;
+ * tags are reported in the order they were first used.  Active when
+ * ENABLE_PROFILING is defined.
  */
 #pragma once
 
@@ -9,7 +12,7 @@
 
 #include <iostream>
 #include <string>
-#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include <sygraph/sycl/event.hpp>
@@ -17,44 +20,51 @@
 namespace sygraph {
 
 namespace detail {
-inline std::unordered_map<std::string, std::vector<sygraph::Event>>& events() {
-  static auto* e = new std::unordered_map<std::string, std::vector<sygraph::Event>>();
-  return *e;
-}
-inline size_t& numVisitedEdges() {
-  static auto* v = new size_t(0);
-  return *v;
-}
+/* The process-wide log: events per tag and the TEPS numerator. */
+struct ProfileLog {
+  std::vector<std::pair<std::string, std::vector<sygraph::Event>>> by_tag;
+  size_t visited_edges = 0;
+
+  std::vector<sygraph::Event>& tag(const std::string& name) {
+    for (auto& entry : by_tag)
+      if (entry.first == name) return entry.second;
+    by_tag.emplace_back(name, std::vector<sygraph::Event>{});
+    return by_tag.back().second;
+  }
+
+  static ProfileLog& instance() {
+    static ProfileLog log;
+    return log;
+  }
+};
 } // namespace detail
 
 class Profiler {
 public:
   static void addEvent(sygraph::Event event, std::string tag = "") {
-    if (tag.empty()) tag = "default";
-    detail::events()[tag].push_back(event);
+    detail::ProfileLog::instance().tag(tag.empty() ? std::string("default") : tag).push_back(event);
   }
-  static void addVisitedEdges(size_t visited_edges) { detail::numVisitedEdges() += visited_edges; }
-  static void clear() {
-    detail::events().clear();
-    detail::numVisitedEdges() = 0;
-  }
+  static void addVisitedEdges(size_t visited_edges) { detail::ProfileLog::instance().visited_edges += visited_edges; }
+  static void clear() { detail::ProfileLog::instance() = detail::ProfileLog{}; }
+
   static void print(bool detailed = false) {
+    const detail::ProfileLog& log = detail::ProfileLog::instance();
     double total_ms = 0.0;
-    for (auto& [tag, evs] : detail::events()) {
-      double ms = 0.0;
-      size_t i = 0;
-      std::cout << " Kernel [" << tag << " x " << evs.size() << "] ";
-      for (auto& e : evs) {
-        const double t = e.getRuntime();
-        if (detailed) std::cout << "(" << i++ << ") " << t << " ms ";
-        ms += t;
+    for (const auto& [name, evs] : log.by_tag) {
+      std::cout << " Kernel [" << name << " x " << evs.size() << "] ";
+      double tag_ms = 0.0;
+      for (size_t k = 0; k < evs.size(); ++k) {
+        const double t = evs[k].getRuntime();
+        tag_ms += t;
+        if (detailed) std::cout << "(" << k << ") " << t << " ms ";
       }
       if (detailed) std::cout << "| ";
-      std::cout << "Time: " << ms << " ms" << std::endl;
-      total_ms += ms;
+      std::cout << "Time: " << tag_ms << " ms" << std::endl;
+      total_ms += tag_ms;
     }
     std::cout << "Total GPU Time: " << total_ms << " ms" << std::endl;
-    const double mteps = total_ms > 0.0 ? (detail::numVisitedEdges() / 1e6) / (total_ms / 1e3) : 0.0;
+    const double seconds = total_ms / 1e3;
+    const double mteps = total_ms > 0.0 ? static_cast<double>(log.visited_edges) / 1e6 / seconds : 0.0;
     std::cout << "Total Edge-Througput (MTEPS): " << mteps << " MTEPS" << std::endl;
   }
 };
