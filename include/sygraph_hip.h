@@ -85,6 +85,10 @@ int sg_device_info_get(sg_device_t dev, sg_device_info* info);
 int sg_device_stream(sg_device_t dev, void** stream);
 /* queue.wait() (used throughout, e.g. mlb_frontier.hpp:170). */
 int sg_device_synchronize(sg_device_t dev);
+/* Bound on a grid barrier's wait inside the device-resident level loops
+ * (default 30 s); a launch that waits longer ends with an error.  No
+ * counterpart in the reference (its level loops run on the host). */
+int sg_device_set_wait_limit(sg_device_t dev, double seconds);
 
 /* ------------------------------------------------------------------ memory */
 enum sg_space { SG_SPACE_HOST = 0, SG_SPACE_DEVICE = 1, SG_SPACE_SHARED = 2 };
@@ -370,16 +374,18 @@ int sg_tc(sg_graph_t g, uint64_t* count, uint64_t* per_vertex);
  * is added to bc once.  Sources are processed one after another. */
 int sg_bc(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
 /* Batched BC (no reference counterpart): the sums of sg_bc computed 64 sources
- * at a time, one bit-lane per source -- one multi-source BFS per batch, then
- * sigma and delta passes per level with FP64 rows of 64 lanes per vertex.
- * Equal to sg_bc up to FP64 rounding; added into bc (device float[n]).
- * Needs one n-word mask per BFS level (SG_ERR_UNSUPPORTED beyond 16 GiB of
- * masks: long-diameter graphs use sg_bc). */
+ * at a time, one bit-lane per source, each batch's whole level loop (forward
+ * multi-source BFS with path counts, then dependencies deepest level first)
+ * in one cooperative launch.  Path counts are FP32 mantissas with an int16
+ * exponent per (vertex, source), dependencies FP32, so any diameter works;
+ * equal to sg_bc up to FP32 rounding; added into bc (device float[n]).
+ * SG_ERR_UNSUPPORTED if a path count reaches 2^32000. */
 int sg_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc);
 /* Multi-source BFS: depth is device int32[nsrc][n] (row i = sources_host[i]),
  * unreached = n+1.  Sources run 64 per batch, one bit-lane each: every vertex
- * holds 64-bit visited / frontier / next masks, so one pass over the arcs
- * advances all 64 traversals by a level.  No counterpart in the reference
+ * holds 64-bit visited and level masks, so one pass over the arcs advances
+ * all 64 traversals by a level; a batch's level loop runs in one cooperative
+ * launch with a grid barrier per level.  No counterpart in the reference
  * (its BFS takes one source). */
 int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth);
 

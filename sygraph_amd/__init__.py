@@ -111,6 +111,7 @@ _SIGS = {
     "sg_device_info_get": (_I, [_P, C.POINTER(DeviceInfo)]),
     "sg_device_stream": (_I, [_P, C.POINTER(_P)]),
     "sg_device_synchronize": (_I, [_P]),
+    "sg_device_set_wait_limit": (_I, [_P, C.c_double]),
     "sg_alloc": (_I, [_P, C.c_size_t, _I, C.POINTER(_P)]),
     "sg_free": (_I, [_P, _P]),
     "sg_memcpy": (_I, [_P, _P, _P, C.c_size_t]),
@@ -296,10 +297,12 @@ class DeviceBuffer:
     def fill(self, value: int, pattern_bytes: int, count: int):
         check(lib().sg_fill(self.dev.h, C.c_void_p(self.ptr), value, pattern_bytes, count))
 
-    def digest(self, count: int, elem_bytes: int = 4) -> int:
-        """Order-sensitive 64-bit digest of the first `count` elements, on the device."""
+    def digest(self, count: int, elem_bytes: int = 4, first: int = 0) -> int:
+        """Order-sensitive 64-bit digest of `count` elements from element
+        `first`, on the device."""
+        assert (first + count) * elem_bytes <= self.nbytes
         h = C.c_uint64()
-        check(lib().sg_digest(self.dev.h, C.c_void_p(self.ptr), count, elem_bytes, C.byref(h)))
+        check(lib().sg_digest(self.dev.h, C.c_void_p(self.ptr + first * elem_bytes), count, elem_bytes, C.byref(h)))
         return h.value
 
 
@@ -678,8 +681,9 @@ class BC:
         self.values.fill(0, 4, max(self.g.n, 1))
 
     def run(self, batched: bool = False):
-        """batched: 64 sources per bit-lane batch (sg_bc_batched); otherwise
-        one source after another (sg_bc).  Equal up to FP64 rounding."""
+        """batched: 64 sources per bit-lane batch (sg_bc_batched, FP32 path
+        counts with per-vertex exponents); otherwise one source after another
+        (sg_bc, FP64).  Equal up to FP32 rounding."""
         if self.values is None:
             raise SygraphError(SG_ERR_STATE, "BC instance not initialized")
         s = np.ascontiguousarray(np.asarray(self.sources, dtype=np.int32))
@@ -690,11 +694,14 @@ class BC:
         return self.values.download(np.float32, self.g.n)
 
 
-def bfs_multi(g: Graph, sources) -> np.ndarray:
-    """Depths from several sources, row i = sources[i] (unreached = n + 1)."""
+def bfs_multi(g: Graph, sources, on_device: bool = False):
+    """Depths from several sources, row i = sources[i] (unreached = n + 1).
+    on_device: return the device buffer (int32[len(sources)][n]) instead."""
     s = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
     out = DeviceBuffer(g.dev, 4 * max(len(s) * g.n, 1), dtype=np.int32)
     check(lib().sg_bfs_multi(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr)))
+    if on_device:
+        return out
     return out.download(np.int32, len(s) * g.n).reshape(len(s), g.n)
 
 

@@ -54,6 +54,11 @@ struct sg_device_s {
   /* small pinned host scratch for stat read-backs */
   void* host_scratch = nullptr;
   size_t host_scratch_bytes = 0;
+  /* bound on a grid barrier's wait inside a cooperative launch, in ticks of
+   * the device's constant-rate clock (wall_clock64); past it the launch ends
+   * with an error instead of waiting on */
+  uint64_t wait_ticks = 0;
+  int64_t wall_khz = 0;
   /* sg_device_close and every graph, frontier and event on the device hold
    * one reference; the stream and scratch go with the last one, so handles
    * may be released in any order */

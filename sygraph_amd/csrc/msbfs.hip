@@ -1,0 +1,1078 @@
+/*
+ * msbfs.hip -- multi-source BFS and batched betweenness centrality, 64
+ * sources per batch (one bit-lane each), the whole level loop of a batch in
+ * one cooperative launch.
+ *
+ * State per vertex: vis (lanes that reached it) and four rotating level masks
+ * M[d & 3] (lanes for which it lies at depth d).  Each level keeps a list of
+ * its vertices (each once, whatever number of lanes), so no pass ever sweeps
+ * all vertices on a sparse level: a 4096 x 4096 grid runs its ~8,000 levels
+ * at the cost of its frontiers plus one grid barrier per level, not of a
+ * launch or a host round trip per level.
+ *
+ * Level d, one phase between grid barriers:
+ *  - expand: push over the out-arcs of L_d (lane per arc), or pull over the
+ *    in-arcs of every vertex still missing lanes when L_d holds more than
+ *    1/20 of the vertices; for an arc u -> v the new lanes are
+ *    M_d[u] & ~(vis[v] | M_d[v]), ORed into M_{d+1}[v].  The first lane to
+ *    touch v appends it to L_{d+1}.  Adjacency lists longer than kHub arcs
+ *    are split into chunks of kHub arcs, listed when the vertex is appended;
+ *  - visit L_d: vis |= M_d, depth rows (MS-BFS) or the level's masks kept
+ *    for the backward pass (BC), and the masks of L_{d-2} cleared so that
+ *    the array serves as M_{d+2};
+ *  - BC: path counts of L_d, pulled from in-neighbours at depth d - 1.
+ * Depths do not depend on push or pull (the level at which a lane first
+ * reaches a vertex is fixed), so they are bit-exact with one BFS per source.
+ *
+ * Brandes (algorithms/bc.hpp:118-200), lane l = source l:
+ *   sigma_l(v) = sum of sigma_l(u) over in-arcs u -> v with u at depth d-1;
+ *   delta_l(v) = sum over out-arcs v -> w with w at depth d+1 of
+ *                sigma_l(v) / sigma_l(w) * (1 + delta_l(w));
+ *   bc(v) += sum of delta_l(v) over the lanes with v at depth >= 1.
+ * Path counts grow like 2^depth on a grid (~2^8190 on 4096 x 4096) and
+ * differ by thousands of binary orders within one level, so each (vertex,
+ * lane) keeps sigma as an FP32 mantissa in [0.5, 1) and an int16 exponent;
+ * sums align exponents as they go.  The backward pass overwrites sigma with
+ * R = (1 + delta) / sigma in the same form, so an arc's term is one product
+ * and one ldexp; delta itself is at most the vertex count and stays FP32.
+ * The backward pass walks the levels deepest first; membership of level d+1
+ * is rebuilt from its list into one of two mask arrays (the OR of L_d's masks
+ * and the AND-NOT of L_{d+2}'s act on disjoint lanes, so they commute).
+ *
+ * Work mapping of the sigma and delta passes: a wave takes 64 list entries
+ * and walks their (vertex, lane) pairs -- 64 pairs per step from several
+ * vertices when masks are sparse (grids), a vertex per step with lane =
+ * source when they are dense (power-law graphs).  Each pair sums its own
+ * arcs in CSR order; lists longer than kHub are summed per chunk into
+ * partial rows that a following phase adds in chunk order.  Nothing is
+ * summed with atomics, so the results do not depend on scheduling.
+ */
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "traverse.hpp"
+
+using namespace sg::trav;
+
+namespace {
+
+constexpr int kHub = 256;     /* adjacency lists longer than this are chunked */
+constexpr int kLiRing = 8;    /* level records of the MS-BFS ring */
+constexpr int kBarGroups = 8; /* arrival counters, one per XCD */
+constexpr int kBarStride = 32; /* 128 bytes apart */
+constexpr uint32_t kOvList = 1, kOvHubV = 2, kOvHubC = 4, kOvPart = 8, kOvLevels = 16;
+constexpr uint32_t kTimeout = 32, kRange = 64;
+constexpr uint32_t kStop = kOvList | kOvHubV | kOvHubC | kOvPart | kOvLevels | kTimeout | kRange;
+constexpr int kExpMax = 32000; /* |exponent| of sigma and R kept in int16 */
+
+/* One level: its vertices and the chunks of its long lists (out-arcs; in-arcs
+ * as well on directed graphs).  Counts grow with atomics while the level is
+ * being discovered; bases are positions in the list arrays. */
+struct alignas(128) LevelInfo {
+  unsigned long long count, arcs, base;
+  unsigned long long hov, hov_base, hoc, hoc_base; /* out-hubs: vertices, chunks */
+  unsigned long long hiv, hiv_base, hic, hic_base; /* in-hubs (directed BC) */
+  unsigned long long pad[5];
+};
+
+/* a vertex with a long list: its chunks are [first, first + nch) of the level's chunk list */
+struct HubV {
+  int32_t v, nch, first, pos; /* pos: the vertex's index in its level list */
+};
+struct HubC {
+  int32_t v, c, pos, pad;
+};
+
+struct Ctl {
+  uint32_t grp[kBarGroups * kBarStride];
+  uint32_t top[kBarStride];
+  uint32_t gen[kBarStride];
+  uint32_t flags;
+  int32_t levels; /* levels of the batch (depths 0 .. levels-1) */
+  uint32_t pad[30];
+};
+
+template<typename OffT>
+struct MsArgs {
+  int64_t n;
+  uint64_t full;
+  int32_t k;        /* sources in the batch */
+  int32_t bc;       /* 1: batched BC, 0: MS-BFS */
+  int32_t ring;     /* 1: level lists in a ring of four slots (MS-BFS) */
+  int32_t directed; /* in-arcs are not the out-arcs */
+  const OffT* off;
+  const int32_t* cols;
+  const OffT* ioff;
+  const int32_t* icols;
+  uint64_t* vis;
+  uint64_t* M; /* four arrays of n words */
+  int32_t* lv;
+  uint64_t* lm;     /* BC: each list entry's mask */
+  int64_t lcap;     /* list entries (ring: per slot) */
+  HubV* hov;
+  HubC* hoc;
+  HubV* hiv;
+  HubC* hic;
+  int64_t hv_cap, hc_cap; /* per slot in the ring */
+  LevelInfo* li;
+  int64_t li_cap;
+  int32_t* depth; /* MS-BFS: rows of the batch's sources */
+  float* sm;      /* BC: sigma / R mantissas, [n][64] */
+  int16_t* se;    /* BC: exponents, [n][64] */
+  float* dl;      /* BC: delta, [n][64] */
+  float* pS;      /* BC: partial rows of chunked lists, [p_cap][64] */
+  int32_t* pE;
+  int64_t p_cap;
+  double* bc64;
+  const int32_t* src;
+  Ctl* ctl;
+  uint64_t wait_ticks;
+};
+
+/* --------------------------------------------------------- device helpers --- */
+__device__ __forceinline__ uint32_t load_flags(const Ctl* c) { return atomic_load(&c->flags); }
+__device__ __forceinline__ void raise_flag(Ctl* c, uint32_t f) { atomic_or(&c->flags, f); }
+
+/* Grid barrier over a cooperative launch.  Arrival counters only grow (barrier
+ * t completes at t arrivals per member), so nothing is reset between
+ * barriers.  The wait is bounded: past wait_ticks of the constant-rate clock
+ * the kTimeout flag is raised and every workgroup leaves.  Returns false when
+ * the run must stop (a flag is set). */
+__device__ __noinline__ bool grid_sync(Ctl* c, uint32_t& t, uint64_t wait_ticks) {
+  __shared__ uint32_t s_stop;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  ++t;
+  if (threadIdx.x == 0) {
+    const uint32_t grid = gridDim.x, g = blockIdx.x % kBarGroups;
+    const uint32_t ngroups = grid < (uint32_t)kBarGroups ? grid : (uint32_t)kBarGroups;
+    const uint32_t in_group = (grid - g + kBarGroups - 1) / kBarGroups;
+    bool released = false;
+    if (atomic_add(&c->grp[g * kBarStride], 1u) == t * in_group - 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (atomic_add(&c->top[0], 1u) == t * ngroups - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __hip_atomic_store(&c->gen[0], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        released = true;
+      }
+    }
+    if (!released) {
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&c->gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < t) {
+        if (load_flags(c) & kStop) break;
+        if ((uint64_t)(wall_clock64() - t0) > wait_ticks) {
+          raise_flag(c, kTimeout);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    s_stop = (load_flags(c) & kStop) ? 1u : 0u;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_stop == 0;
+}
+
+/* this lane's first slot of k, with one atomic per wave (all lanes call) */
+__device__ __forceinline__ uint64_t wave_reserve(unsigned long long* ctr, uint32_t k) {
+  const uint32_t incl = wave_inclusive_scan(k);
+  const uint32_t total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (total) {
+    if (lane_id() == 63) base = atomic_add(ctr, (unsigned long long)total);
+    base = (unsigned long long)shfl_i64((int64_t)base, 63);
+  }
+  return base + incl - k;
+}
+
+template<typename OffT>
+__device__ __forceinline__ LevelInfo* level(const MsArgs<OffT>& a, int32_t d) {
+  return a.ring ? a.li + (d & (kLiRing - 1)) : a.li + d;
+}
+
+template<typename OffT>
+__device__ __forceinline__ int64_t list_base(const MsArgs<OffT>& a, int32_t d) {
+  return a.ring ? (int64_t)(d & 3) * a.lcap : (int64_t)atomic_load(&level(a, d)->base);
+}
+
+/* Bases of level d+1's lists: the ring's slot, or right after level d's. */
+struct Bases {
+  int64_t l, hov, hoc, hiv, hic;
+};
+template<typename OffT>
+__device__ __forceinline__ Bases next_bases(const MsArgs<OffT>& a, int32_t d) {
+  Bases b;
+  if (a.ring) {
+    const int64_t s = (d + 1) & 3;
+    b.l = s * a.lcap;
+    b.hov = b.hiv = s * a.hv_cap;
+    b.hoc = b.hic = s * a.hc_cap;
+  } else {
+    LevelInfo* L = level(a, d);
+    b.l = (int64_t)(atomic_load(&L->base) + atomic_load(&L->count));
+    b.hov = (int64_t)(atomic_load(&L->hov_base) + atomic_load(&L->hov));
+    b.hoc = (int64_t)(atomic_load(&L->hoc_base) + atomic_load(&L->hoc));
+    b.hiv = (int64_t)(atomic_load(&L->hiv_base) + atomic_load(&L->hiv));
+    b.hic = (int64_t)(atomic_load(&L->hic_base) + atomic_load(&L->hic));
+  }
+  return b;
+}
+
+/* Append the vertices this wave touched first (first == true) to level
+ * nx's list, with their out-degrees and, for long lists, their chunks.
+ * All lanes call. */
+template<typename OffT>
+__device__ __forceinline__ void append(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, bool first,
+                                       int32_t v) {
+  const uint64_t any = ballot(first);
+  if (!any) return;
+  const int64_t cap_l = a.ring ? a.lcap : a.lcap - b.l;
+  const int64_t cap_hv = a.ring ? a.hv_cap : a.hv_cap - b.hov;
+  const int64_t cap_hc = a.ring ? a.hc_cap : a.hc_cap - b.hoc;
+  const uint64_t rel = wave_reserve(&nx->count, first ? 1u : 0u);
+  int64_t deg = 0;
+  if (first) {
+    if ((int64_t)rel < cap_l) a.lv[b.l + (int64_t)rel] = v;
+    else raise_flag(a.ctl, kOvList);
+    deg = (int64_t)(a.off[v + 1] - a.off[v]);
+  }
+  const uint64_t arcs = wave_sum_u64((uint64_t)deg);
+  if (lane_id() == 0 && arcs) atomic_add(&nx->arcs, (unsigned long long)arcs);
+  uint32_t nch = (first && deg > kHub) ? (uint32_t)((deg + kHub - 1) / kHub) : 0u;
+  if (ballot(nch != 0)) {
+    const uint64_t hv = wave_reserve(&nx->hov, nch ? 1u : 0u);
+    const uint64_t hc = wave_reserve(&nx->hoc, nch);
+    if (nch) {
+      if ((int64_t)hv < cap_hv && (int64_t)(hc + nch) <= cap_hc) {
+        a.hov[b.hov + (int64_t)hv] = HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel};
+        for (uint32_t c = 0; c < nch; ++c) a.hoc[b.hoc + (int64_t)hc + c] = HubC{v, (int32_t)c, (int32_t)rel, 0};
+      } else {
+        raise_flag(a.ctl, (int64_t)hv < cap_hv ? kOvHubC : kOvHubV);
+      }
+    }
+  }
+  if (a.bc && a.directed) { /* in-arc chunks for the path counts */
+    const int64_t ideg = first ? (int64_t)(a.ioff[v + 1] - a.ioff[v]) : 0;
+    nch = ideg > kHub ? (uint32_t)((ideg + kHub - 1) / kHub) : 0u;
+    if (ballot(nch != 0)) {
+      const int64_t cap_iv = a.hv_cap - b.hiv, cap_ic = a.hc_cap - b.hic;
+      const uint64_t hv = wave_reserve(&nx->hiv, nch ? 1u : 0u);
+      const uint64_t hc = wave_reserve(&nx->hic, nch);
+      if (nch) {
+        if ((int64_t)hv < cap_iv && (int64_t)(hc + nch) <= cap_ic) {
+          a.hiv[b.hiv + (int64_t)hv] = HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel};
+          for (uint32_t c = 0; c < nch; ++c) a.hic[b.hic + (int64_t)hc + c] = HubC{v, (int32_t)c, (int32_t)rel, 0};
+        } else {
+          raise_flag(a.ctl, (int64_t)hv < cap_iv ? kOvHubC : kOvHubV);
+        }
+      }
+    }
+  }
+}
+
+/* New lanes for v from a frontier mask fm: OR them into M_{d+1}[v]; true when
+ * this call was the first to give v any lane at depth d+1. */
+__device__ __forceinline__ bool relax(uint64_t fm, int32_t v, const uint64_t* vis, const uint64_t* Mc, uint64_t* Mn) {
+  const uint64_t add = fm & ~(vis[v] | Mc[v]);
+  if (!add) return false;
+  if ((Mn[v] & add) == add) return false;
+  return atomic_or(&Mn[v], add) == 0ull;
+}
+
+/* A wave's walk over the (entry, lane) pairs of 64 entry masks (lane i holds
+ * entry i's mask): an entry per step (lane = bit) when the masks are dense,
+ * 64 pairs per step when they are sparse -- whichever takes fewer steps. */
+struct PairWalk {
+  uint64_t words;
+  uint32_t incl, pc, total, steps;
+  bool flat;
+};
+__device__ __forceinline__ PairWalk pair_walk(uint64_t mine) {
+  PairWalk w;
+  w.words = ballot(mine != 0);
+  w.pc = (uint32_t)__popcll(mine);
+  w.incl = wave_inclusive_scan(w.pc);
+  w.total = __shfl(w.incl, 63, 64);
+  const uint32_t flat_steps = (w.total + 63) / 64, word_steps = (uint32_t)__popcll(w.words);
+  w.flat = flat_steps < word_steps;
+  w.steps = w.flat ? flat_steps : word_steps;
+  return w;
+}
+/* this lane's pair at `step`: owner entry (lane index) and source lane */
+__device__ __forceinline__ bool pair_item(PairWalk& w, uint64_t mine, uint32_t step, int& owner, int& bit) {
+  const int lane = lane_id();
+  if (w.flat) {
+    const uint32_t t = step * 64 + lane;
+    owner = scan_owner(w.incl, t);
+    const uint32_t before = __shfl(w.incl - w.pc, owner, 64);
+    const uint64_t word = shfl_u64(mine, owner);
+    bit = t < w.total ? (int)select_bit(word, t - before) : 0;
+    return t < w.total;
+  }
+  owner = __ffsll((unsigned long long)w.words) - 1;
+  w.words &= w.words - 1;
+  const uint64_t word = shfl_u64(mine, owner);
+  bit = lane;
+  return (word >> lane) & 1ull;
+}
+
+/* running sum S * 2^E of terms a * 2^e (exponents aligned as they come) */
+__device__ __forceinline__ void xadd(float& S, int& E, float a, int e) {
+  if (e > E) {
+    const int sh = E - e;
+    S = __builtin_amdgcn_ldexpf(S, sh < -200 ? -200 : sh) + a;
+    E = e;
+  } else {
+    const int sh = e - E;
+    S += __builtin_amdgcn_ldexpf(a, sh < -200 ? -200 : sh);
+  }
+}
+constexpr int kNoExp = -(1 << 30);
+
+/* store S * 2^E as mantissa in [0.5, 1) and int16 exponent */
+template<typename OffT>
+__device__ __forceinline__ void store_norm(const MsArgs<OffT>& a, int64_t at, float S, int E) {
+  int k = 0;
+  float m = 0.f;
+  if (S > 0.f) {
+    m = __builtin_amdgcn_frexp_mantf(S);
+    k = __builtin_amdgcn_frexp_expf(S);
+  }
+  const int ex = S > 0.f ? E + k : 0;
+  if (ex > kExpMax || ex < -kExpMax) raise_flag(a.ctl, kRange);
+  a.sm[at] = m;
+  a.se[at] = (int16_t)(ex > kExpMax ? kExpMax : (ex < -kExpMax ? -kExpMax : ex));
+}
+
+/* ------------------------------------------------------------ level phase --- */
+/* Push expansion of L_d: lane per arc over groups of 64 entries (long lists
+ * are left to their chunks), four arcs per lane in flight. */
+template<typename OffT>
+__device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, int64_t base, int64_t cnt,
+                                            const uint64_t* Mc, uint64_t* Mn, LevelInfo* nx, const Bases& nb,
+                                            int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
+    const int64_t i = g * 64 + lane;
+    uint64_t fm = 0;
+    int64_t st = 0;
+    uint32_t len = 0;
+    if (i < cnt) {
+      const int32_t u = a.lv[base + i];
+      st = (int64_t)a.off[u];
+      const int64_t dg = (int64_t)a.off[u + 1] - st;
+      if (dg <= kHub) {
+        len = (uint32_t)dg;
+        fm = Mc[u];
+      }
+    }
+    const uint32_t incl = wave_inclusive_scan(len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - len;
+    for (uint32_t t0 = 0; t0 < total; t0 += 4 * 64) {
+      int32_t v[4];
+      uint64_t f[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t t = t0 + 64 * k + lane;
+        const int owner = scan_owner(incl, t);
+        const uint32_t before = __shfl(excl, owner, 64);
+        const int64_t s0 = shfl_i64(st, owner);
+        f[k] = shfl_u64(fm, owner);
+        v[k] = t < total ? a.cols[s0 + (t - before)] : -1;
+      }
+      bool first[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(f[k], v[k], a.vis, Mc, Mn);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) append(a, nx, nb, first[k], v[k]);
+    }
+  }
+}
+
+/* Push expansion of the chunks of L_d's long lists: a wave per chunk. */
+template<typename OffT>
+__device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC* hc, int64_t nc, const uint64_t* Mc,
+                                              uint64_t* Mn, LevelInfo* nx, const Bases& nb, int64_t gwave,
+                                              int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t c = gwave; c < nc; c += nwaves) {
+    const HubC h = hc[c];
+    const uint64_t fm = Mc[h.v];
+    const int64_t s = (int64_t)a.off[h.v] + (int64_t)h.c * kHub;
+    const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.off[h.v + 1]);
+    for (int64_t j0 = s; j0 < e; j0 += 4 * 64) {
+      int32_t v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t j = j0 + 64 * k + lane;
+        v[k] = j < e ? a.cols[j] : -1;
+      }
+      bool first[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(fm, v[k], a.vis, Mc, Mn);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) append(a, nx, nb, first[k], v[k]);
+    }
+  }
+}
+
+/* Pull expansion: every vertex still missing lanes ORs the level masks of its
+ * in-neighbours, stopping once the missing lanes are covered; lists longer
+ * than 64 arcs are ORed by the whole wave. */
+template<typename OffT>
+__device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_t* Mc, uint64_t* Mn, LevelInfo* nx,
+                                            const Bases& nb, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t base = gwave * 64; base < a.n; base += nwaves * 64) {
+    const int64_t v = base + lane;
+    uint64_t want = 0, acc = 0;
+    int64_t b = 0, e = 0;
+    if (v < a.n) {
+      want = a.full & ~(a.vis[v] | Mc[v]);
+      if (want) {
+        b = (int64_t)a.ioff[v];
+        e = (int64_t)a.ioff[v + 1];
+      }
+    }
+    const bool coop = e - b > 64;
+    if (!coop)
+      for (int64_t j = b; j < e; ++j) {
+        acc |= Mc[a.icols[j]];
+        if ((acc & want) == want) break;
+      }
+    uint64_t heavy = ballot(coop);
+    while (heavy) {
+      const int k = __ffsll((unsigned long long)heavy) - 1;
+      heavy &= heavy - 1;
+      const int64_t kb = shfl_i64(b, k), ke = shfl_i64(e, k);
+      const uint64_t kwant = shfl_u64(want, k);
+      uint64_t part = 0;
+      for (int64_t j0 = kb; j0 < ke; j0 += 64) {
+        if (j0 + lane < ke) part |= Mc[a.icols[j0 + lane]];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) part |= shfl_u64(part, lane ^ o);
+        if ((part & kwant) == kwant) break;
+      }
+      if (lane == k) acc = part;
+    }
+    const uint64_t add = acc & want;
+    const bool first = add != 0;
+    if (first) Mn[v] = add;
+    append(a, nx, nb, first, (int32_t)v);
+  }
+}
+
+/* Visit L_d: vis |= M_d, depth rows (MS-BFS) or the kept masks (BC). */
+template<typename OffT>
+__device__ __forceinline__ void visit_level(const MsArgs<OffT>& a, int32_t d, int64_t base, int64_t cnt,
+                                            const uint64_t* Mc, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
+    const int64_t i = g * 64 + lane;
+    int32_t u = 0;
+    uint64_t m = 0;
+    if (i < cnt) {
+      u = a.lv[base + i];
+      m = Mc[u];
+      a.vis[u] |= m;
+      if (a.bc) a.lm[base + i] = m;
+    }
+    if (!a.bc) {
+      uint64_t w = m;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) w |= shfl_u64(w, lane ^ o);
+      while (w) { /* one store per source row for the wave's entries */
+        const int s = __ffsll((unsigned long long)w) - 1;
+        w &= w - 1;
+        if ((m >> s) & 1ull) a.depth[(int64_t)s * a.n + u] = d;
+      }
+    }
+  }
+}
+
+/* Clear the level masks of L_{d-2}, so that their array serves as M_{d+2}. */
+template<typename OffT>
+__device__ __forceinline__ void clear_level(const MsArgs<OffT>& a, int64_t base, int64_t cnt, uint64_t* Mold,
+                                            int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) Mold[a.lv[base + i]] = 0ull;
+}
+
+/* Path counts of the pairs of L_d whose in-lists are short: each pair sums
+ * sigma over its in-arcs from depth d-1 (M_{d-1}). */
+template<typename OffT>
+__device__ __forceinline__ void sigma_pairs(const MsArgs<OffT>& a, int64_t base, int64_t cnt, const uint64_t* Mc,
+                                            const uint64_t* Mp, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
+    const int64_t i = g * 64 + lane;
+    int32_t u = 0;
+    uint64_t own = 0;
+    int64_t b = 0, e = 0;
+    if (i < cnt) {
+      u = a.lv[base + i];
+      b = (int64_t)a.ioff[u];
+      e = (int64_t)a.ioff[u + 1];
+      if (e - b <= kHub) own = Mc[u];
+    }
+    PairWalk w = pair_walk(own);
+    for (uint32_t step = 0; step < w.steps; ++step) {
+      int owner, l;
+      const bool ok = pair_item(w, own, step, owner, l);
+      const int32_t v = __shfl(u, owner, 64);
+      const int64_t vb = shfl_i64(b, owner), ve = shfl_i64(e, owner);
+      if (!ok) continue;
+      float S = 0.f;
+      int E = kNoExp;
+      for (int64_t j = vb; j < ve; j += 4) {
+        int32_t x[4];
+        bool hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = j + k < ve ? a.icols[j + k] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((Mp[x[k]] >> l) & 1ull);
+        float sv[4];
+        int ev[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          sv[k] = hit[k] ? a.sm[(int64_t)x[k] * 64 + l] : 0.f;
+          ev[k] = hit[k] ? (int)a.se[(int64_t)x[k] * 64 + l] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (hit[k]) xadd(S, E, sv[k], ev[k]);
+      }
+      store_norm(a, (int64_t)v * 64 + l, S, E);
+    }
+  }
+}
+
+/* Partial path counts over one chunk of a long in-list: a wave per chunk,
+ * lane = source. */
+template<typename OffT>
+__device__ __forceinline__ void sigma_chunks(const MsArgs<OffT>& a, const HubC* hc, int64_t nc, const uint64_t* Mc,
+                                             const uint64_t* Mp, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t c = gwave; c < nc; c += nwaves) {
+    const HubC h = hc[c];
+    const uint64_t own = Mc[h.v];
+    const bool mine = (own >> lane) & 1ull;
+    const int64_t s = (int64_t)a.ioff[h.v] + (int64_t)h.c * kHub;
+    const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.ioff[h.v + 1]);
+    float S = 0.f;
+    int E = kNoExp;
+    for (int64_t j = s; j < e; j += 4) {
+      int32_t x[4];
+      bool hit[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[k] = j + k < e ? a.icols[j + k] : -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) hit[k] = mine && x[k] >= 0 && ((Mp[x[k]] >> lane) & 1ull);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (hit[k]) xadd(S, E, a.sm[(int64_t)x[k] * 64 + lane], (int)a.se[(int64_t)x[k] * 64 + lane]);
+    }
+    if (c < a.p_cap) {
+      a.pS[c * 64 + lane] = S;
+      a.pE[c * 64 + lane] = E;
+    } else if (lane == 0) {
+      raise_flag(a.ctl, kOvPart);
+    }
+  }
+}
+
+/* Chunked vertices: add the partial rows in chunk order. */
+template<typename OffT>
+__device__ __forceinline__ void sigma_combine(const MsArgs<OffT>& a, const HubV* hv, int64_t nv, const uint64_t* Mc,
+                                              int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t i = gwave; i < nv; i += nwaves) {
+    const HubV h = hv[i];
+    if (!((Mc[h.v] >> lane) & 1ull)) continue;
+    float S = 0.f;
+    int E = kNoExp;
+    for (int32_t c = 0; c < h.nch; ++c) {
+      const int64_t slot = (int64_t)h.first + c;
+      const float s = a.pS[slot * 64 + lane];
+      if (s > 0.f) xadd(S, E, s, a.pE[slot * 64 + lane]);
+    }
+    store_norm(a, (int64_t)h.v * 64 + lane, S, E);
+  }
+}
+
+/* ------------------------------------------------------- backward phase --- */
+/* delta of a pair from its arc sum: R = (1 + delta) / sigma replaces sigma */
+template<typename OffT>
+__device__ __forceinline__ void finish_delta(const MsArgs<OffT>& a, int64_t at, float delta) {
+  const float m = a.sm[at];
+  const int e = (int)a.se[at];
+  a.dl[at] = delta;
+  const float r = (1.0f + delta) / m;
+  store_norm(a, at, r, -e);
+}
+
+/* One arc's term sigma_l(v) / sigma_l(w) * (1 + delta_l(w)) = m_v 2^e_v R_l(w). */
+__device__ __forceinline__ float delta_term(float mv, int ev, float mr, int er) {
+  const int sh = ev + er;
+  return __builtin_amdgcn_ldexpf(mv * mr, sh < -200 ? -200 : (sh > 100 ? 100 : sh));
+}
+
+template<typename OffT>
+__device__ __forceinline__ void delta_pairs(const MsArgs<OffT>& a, int64_t base, int64_t cnt, const uint64_t* Xa,
+                                            int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
+    const int64_t i = g * 64 + lane;
+    int32_t u = 0;
+    uint64_t own = 0;
+    int64_t b = 0, e = 0;
+    if (i < cnt) {
+      u = a.lv[base + i];
+      b = (int64_t)a.off[u];
+      e = (int64_t)a.off[u + 1];
+      if (e - b <= kHub) own = a.lm[base + i];
+    }
+    PairWalk w = pair_walk(own);
+    for (uint32_t step = 0; step < w.steps; ++step) {
+      int owner, l;
+      const bool ok = pair_item(w, own, step, owner, l);
+      const int32_t v = __shfl(u, owner, 64);
+      const int64_t vb = shfl_i64(b, owner), ve = shfl_i64(e, owner);
+      if (!ok) continue;
+      const int64_t at = (int64_t)v * 64 + l;
+      const float mv = a.sm[at];
+      const int ev = (int)a.se[at];
+      float acc = 0.f;
+      for (int64_t j = vb; j < ve; j += 4) {
+        int32_t x[4];
+        bool hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = j + k < ve ? a.cols[j + k] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((Xa[x[k]] >> l) & 1ull);
+        float r[4];
+        int er[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          r[k] = hit[k] ? a.sm[(int64_t)x[k] * 64 + l] : 0.f;
+          er[k] = hit[k] ? (int)a.se[(int64_t)x[k] * 64 + l] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (hit[k]) acc += delta_term(mv, ev, r[k], er[k]);
+      }
+      finish_delta(a, at, acc);
+    }
+  }
+}
+
+template<typename OffT>
+__device__ __forceinline__ void delta_chunks(const MsArgs<OffT>& a, const HubC* hc, int64_t nc, int64_t lbase,
+                                             const uint64_t* Xa, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t c = gwave; c < nc; c += nwaves) {
+    const HubC h = hc[c];
+    const uint64_t own = a.lm[lbase + h.pos];
+    const bool mine = (own >> lane) & 1ull;
+    const int64_t at = (int64_t)h.v * 64 + lane;
+    const float mv = mine ? a.sm[at] : 0.f;
+    const int ev = mine ? (int)a.se[at] : 0;
+    const int64_t s = (int64_t)a.off[h.v] + (int64_t)h.c * kHub;
+    const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.off[h.v + 1]);
+    float acc = 0.f;
+    for (int64_t j = s; j < e; j += 4) {
+      int32_t x[4];
+      bool hit[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[k] = j + k < e ? a.cols[j + k] : -1;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) hit[k] = mine && x[k] >= 0 && ((Xa[x[k]] >> lane) & 1ull);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (hit[k])
+          acc += delta_term(mv, ev, a.sm[(int64_t)x[k] * 64 + lane], (int)a.se[(int64_t)x[k] * 64 + lane]);
+    }
+    if (c < a.p_cap) a.pS[c * 64 + lane] = acc;
+    else if (lane == 0) raise_flag(a.ctl, kOvPart);
+  }
+}
+
+template<typename OffT>
+__device__ __forceinline__ void delta_combine(const MsArgs<OffT>& a, const HubV* hv, int64_t nv, int64_t lbase,
+                                              int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t i = gwave; i < nv; i += nwaves) {
+    const HubV h = hv[i];
+    if (!((a.lm[lbase + h.pos] >> lane) & 1ull)) continue;
+    float acc = 0.f;
+    for (int32_t c = 0; c < h.nch; ++c) acc += a.pS[((int64_t)h.first + c) * 64 + lane];
+    finish_delta(a, (int64_t)h.v * 64 + lane, acc);
+  }
+}
+
+/* Level membership for the next backward step: OR L_d's masks into Xb and
+ * take L_{d+2}'s out (disjoint lanes, so the two commute). */
+template<typename OffT>
+__device__ __forceinline__ void scatter_masks(const MsArgs<OffT>& a, int64_t base, int64_t cnt, uint64_t* Xb,
+                                              bool set, int64_t gwave, int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) {
+    const int32_t v = a.lv[base + i];
+    const uint64_t m = a.lm[base + i];
+    if (set) atomic_or(&Xb[v], m);
+    else atomic_and(&Xb[v], ~m);
+  }
+}
+
+/* bc(v) += sum of delta over the lanes of v's entry of one level, in lane order */
+template<typename OffT>
+__device__ __forceinline__ void accumulate(const MsArgs<OffT>& a, int64_t base, int64_t cnt, int64_t gwave,
+                                           int64_t nwaves) {
+  const int lane = lane_id();
+  for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) {
+    const int32_t v = a.lv[base + i];
+    uint64_t m = a.lm[base + i];
+    double s = 0.0;
+    while (m) {
+      const int l = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      s += (double)a.dl[(int64_t)v * 64 + l];
+    }
+    a.bc64[v] += s;
+  }
+}
+
+/* ------------------------------------------------------------- the kernel --- */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
+  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  const int64_t n = a.n;
+  uint32_t t = 0;
+  int32_t d = 0;
+  int64_t max_chunks = 0; /* largest chunk list of a level (the backward pass's partial rows) */
+  for (;; ++d) {
+    LevelInfo* L = level(a, d);
+    const int64_t cnt = (int64_t)atomic_load(&L->count);
+    if (cnt == 0) break;
+    if (!a.ring && d + 2 >= a.li_cap) { /* every workgroup takes this exit */
+      if (blockIdx.x == 0 && threadIdx.x == 0) raise_flag(a.ctl, kOvLevels);
+      return;
+    }
+    const int64_t base = list_base(a, d);
+    uint64_t* Mc = a.M + (int64_t)(d & 3) * n;
+    uint64_t* Mn = a.M + (int64_t)((d + 1) & 3) * n;
+    const uint64_t* Mp = a.M + (int64_t)((d + 3) & 3) * n;
+    LevelInfo* nx = level(a, d + 1);
+    const Bases nb = next_bases(a, d);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      LevelInfo* z = level(a, d + 2); /* the level after next starts from zero */
+      unsigned long long* zp = reinterpret_cast<unsigned long long*>(z);
+      for (int k = 0; k < 11; ++k) atomic_store(zp + k, 0ull);
+      atomic_store(&nx->base, (unsigned long long)nb.l);
+      atomic_store(&nx->hov_base, (unsigned long long)nb.hov);
+      atomic_store(&nx->hoc_base, (unsigned long long)nb.hoc);
+      atomic_store(&nx->hiv_base, (unsigned long long)nb.hiv);
+      atomic_store(&nx->hic_base, (unsigned long long)nb.hic);
+    }
+    const int64_t hoc = (int64_t)atomic_load(&L->hoc);
+    max_chunks = hoc > max_chunks ? hoc : max_chunks;
+    const HubC* hocp = a.hoc + (a.ring ? (int64_t)(d & 3) * a.hc_cap : (int64_t)atomic_load(&L->hoc_base));
+    if (cnt * 20 > n) {
+      expand_pull(a, Mc, Mn, nx, nb, gwave, nwaves);
+    } else {
+      expand_push(a, d, base, cnt, Mc, Mn, nx, nb, gwave, nwaves);
+      expand_chunks(a, hocp, hoc, Mc, Mn, nx, nb, gwave, nwaves);
+    }
+    visit_level(a, d, base, cnt, Mc, gwave, nwaves);
+    if (d >= 2) {
+      LevelInfo* O = level(a, d - 2);
+      clear_level(a, list_base(a, d - 2), (int64_t)atomic_load(&O->count), a.M + (int64_t)((d + 2) & 3) * n, gwave,
+                  nwaves);
+    }
+    int64_t hiv = 0;
+    const HubV* hivp = nullptr;
+    if (a.bc && d >= 1) {
+      sigma_pairs(a, base, cnt, Mc, Mp, gwave, nwaves);
+      const HubC* hicp = a.directed ? a.hic + (int64_t)atomic_load(&L->hic_base) : hocp;
+      const int64_t hic = a.directed ? (int64_t)atomic_load(&L->hic) : hoc;
+      sigma_chunks(a, hicp, hic, Mc, Mp, gwave, nwaves);
+      hiv = a.directed ? (int64_t)atomic_load(&L->hiv) : (int64_t)atomic_load(&L->hov);
+      hivp = a.directed ? a.hiv + (int64_t)atomic_load(&L->hiv_base) : a.hov + (int64_t)atomic_load(&L->hov_base);
+    }
+    if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
+    if (hiv > 0) {
+      sigma_combine(a, hivp, hiv, Mc, gwave, nwaves);
+      if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->levels = d;
+  if (!a.bc) return;
+  /* the partial rows must hold every level's chunks before anything is added
+   * to bc (a batch that stops here runs again with more rows) */
+  if (max_chunks > a.p_cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) raise_flag(a.ctl, kOvPart);
+    return;
+  }
+
+  /* backward: levels d-1 .. 1; X[j] = M[j] hold the memberships */
+  const int32_t deepest = d - 1;
+  for (int64_t i = gwave * 64 + lane_id(); i < n; i += nwaves * 64) {
+    a.M[i] = 0ull;
+    a.M[n + i] = 0ull;
+  }
+  if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
+  for (int32_t dd = deepest; dd >= 1; --dd) {
+    LevelInfo* L = level(a, dd);
+    const int64_t base = (int64_t)atomic_load(&L->base), cnt = (int64_t)atomic_load(&L->count);
+    const uint64_t* Xa = a.M + (int64_t)((dd + 1) & 1) * n;
+    uint64_t* Xb = a.M + (int64_t)(dd & 1) * n;
+    const int64_t hov = (int64_t)atomic_load(&L->hov), hoc = (int64_t)atomic_load(&L->hoc);
+    const HubV* hovp = a.hov + (int64_t)atomic_load(&L->hov_base);
+    const HubC* hocp = a.hoc + (int64_t)atomic_load(&L->hoc_base);
+    delta_pairs(a, base, cnt, Xa, gwave, nwaves);
+    delta_chunks(a, hocp, hoc, base, Xa, gwave, nwaves);
+    scatter_masks(a, base, cnt, Xb, true, gwave, nwaves);
+    if (dd + 2 <= deepest) {
+      LevelInfo* L2 = level(a, dd + 2);
+      scatter_masks(a, (int64_t)atomic_load(&L2->base), (int64_t)atomic_load(&L2->count), Xb, false, gwave, nwaves);
+    }
+    if (dd + 1 <= deepest) {
+      LevelInfo* L1 = level(a, dd + 1);
+      accumulate(a, (int64_t)atomic_load(&L1->base), (int64_t)atomic_load(&L1->count), gwave, nwaves);
+    }
+    if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
+    if (hov > 0) {
+      delta_combine(a, hovp, hov, base, gwave, nwaves);
+      if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
+    }
+  }
+  if (deepest >= 1) {
+    LevelInfo* L1 = level(a, 1);
+    accumulate(a, (int64_t)atomic_load(&L1->base), (int64_t)atomic_load(&L1->count), gwave, nwaves);
+  }
+}
+
+/* ----------------------------------------------------------- batch set-up --- */
+__global__ void ms_clear_kernel(int64_t n, uint64_t* vis, uint64_t* M, LevelInfo* li, int32_t nli, Ctl* ctl) {
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i < n; i += stride) {
+    vis[i] = 0ull;
+    M[i] = 0ull;
+    M[n + i] = 0ull;
+    M[2 * n + i] = 0ull;
+    M[3 * n + i] = 0ull;
+  }
+  unsigned long long* lp = reinterpret_cast<unsigned long long*>(li);
+  for (int64_t i = tid; i < (int64_t)nli * (int64_t)(sizeof(LevelInfo) / 8); i += stride) lp[i] = 0ull;
+  uint32_t* cp = reinterpret_cast<uint32_t*>(ctl);
+  for (int64_t i = tid; i < (int64_t)(sizeof(Ctl) / 4); i += stride) cp[i] = 0u;
+}
+
+/* Level 0: the batch's distinct sources (one wave; lane i = source i). */
+template<typename OffT>
+__global__ void __launch_bounds__(64) ms_seed_kernel(MsArgs<OffT> a) {
+  const int i = threadIdx.x;
+  bool first = false;
+  int32_t s = 0;
+  if (i < a.k) {
+    s = a.src[i];
+    atomic_or(&a.vis[s], 1ull << i);
+    first = atomic_or(&a.M[s], 1ull << i) == 0ull;
+    if (a.bc) {
+      a.sm[(int64_t)s * 64 + i] = 0.5f; /* sigma = 1 = 0.5 * 2^1 */
+      a.se[(int64_t)s * 64 + i] = 1;
+    }
+  }
+  Bases b{0, 0, 0, 0, 0};
+  append(a, a.li, b, first, s);
+}
+
+__global__ void ms_fill_i32_kernel(int32_t* p, int32_t val, int64_t cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+
+__global__ void ms_zero_f64_kernel(double* p, int64_t cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void ms_add_to_float_kernel(const double* acc, float* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)((double)out[i] + acc[i]);
+}
+
+/* ------------------------------------------------------------------ host --- */
+/* Capacities of the growable lists (entries); kept with the graph under "msb_caps". */
+struct Caps {
+  int64_t lcap = 0, hv_cap = 0, hc_cap = 0, p_cap = 0, li_cap = 0;
+};
+
+template<typename OffT>
+int launch_batches(sg_graph_t g, bool bc, int32_t nsrc, const int32_t* sources, int32_t* depth, float* out) {
+  sg_device_t dev = g->dev;
+  hipStream_t s = dev->stream;
+  const int64_t n = g->n, m = g->m;
+  const bool directed = g->inv_offsets != nullptr;
+  const int64_t hubs_bound = m / kHub + 64, chunks_bound = 2 * (m / kHub) + 64;
+  Caps caps;
+  /* the growable capacities are the sizes of the buffers kept with the graph */
+  auto cap_of = [&](const char* name, int64_t dflt) -> int64_t {
+    auto it = g->buffers.find(name);
+    return it == g->buffers.end() ? dflt : (int64_t)it->second.second;
+  };
+  const int32_t kmax = std::min<int32_t>(64, nsrc);
+  if (bc) {
+    caps.lcap = std::max<int64_t>(cap_of("msb_lv", 0) / 4, std::min<int64_t>(16, kmax) * n + 64);
+    caps.hv_cap = std::max<int64_t>(cap_of("msb_hov", 0) / (int64_t)sizeof(HubV), 4 * hubs_bound);
+    caps.hc_cap = std::max<int64_t>(cap_of("msb_hoc", 0) / (int64_t)sizeof(HubC), 4 * chunks_bound);
+    caps.p_cap = std::max<int64_t>(cap_of("msb_pS", 0) / (64 * 4), chunks_bound / 4 + 64);
+    caps.li_cap = std::max<int64_t>(cap_of("msb_li", 0) / (int64_t)sizeof(LevelInfo), std::min<int64_t>(n + 3, 65536));
+  } else {
+    caps.lcap = n + 64;
+    caps.hv_cap = hubs_bound;
+    caps.hc_cap = chunks_bound;
+    caps.li_cap = kLiRing;
+  }
+  static int occ = 0; /* per offset width (one instantiation each) */
+  if (occ == 0) {
+    int nb = 0;
+    SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ms_kernel<OffT>, kBlock, 0));
+    occ = std::max(1, std::min(nb, 4));
+  }
+  const int grid = occ * dev->cus;
+  const int grid_n = sg::grid_for(dev, n, kBlock, 8);
+
+  if (!bc) {
+    ms_fill_i32_kernel<<<sg::grid_for(dev, (int64_t)nsrc * n, 256, 8), 256, 0, s>>>(depth, (int32_t)(n + 1),
+                                                                                   (int64_t)nsrc * n);
+    SG_HIP(hipGetLastError());
+  }
+  double* bc64 = nullptr;
+  if (bc) {
+    SG_TRY(sg::graph_buffer(g, "msb_bc64", 8 * (size_t)(n + 1), &bc64));
+    ms_zero_f64_kernel<<<grid_n, kBlock, 0, s>>>(bc64, n + 1);
+  }
+  Ctl* hctl = static_cast<Ctl*>(dev->host_scratch);
+  for (int32_t b0 = 0; b0 < nsrc;) {
+    const int32_t k = std::min<int32_t>(64, nsrc - b0);
+    MsArgs<OffT> a{};
+    a.n = n;
+    a.full = k == 64 ? ~0ull : ((1ull << k) - 1ull);
+    a.k = k;
+    a.bc = bc ? 1 : 0;
+    a.ring = bc ? 0 : 1;
+    a.directed = (bc && directed) ? 1 : 0;
+    a.off = (const OffT*)g->offsets;
+    a.cols = g->cols;
+    a.ioff = (const OffT*)(directed ? g->inv_offsets : g->offsets);
+    a.icols = directed ? g->inv_cols : g->cols;
+    const int64_t lslots = bc ? caps.lcap : 4 * caps.lcap;
+    const int64_t hvslots = bc ? caps.hv_cap : 4 * caps.hv_cap;
+    const int64_t hcslots = bc ? caps.hc_cap : 4 * caps.hc_cap;
+    SG_TRY(sg::graph_buffer(g, "msb_vis", 8 * (size_t)(n + 1), &a.vis));
+    SG_TRY(sg::graph_buffer(g, "msb_M", 8 * 4 * (size_t)(n + 1), &a.M));
+    SG_TRY(sg::graph_buffer(g, "msb_lv", 4 * (size_t)lslots, &a.lv));
+    SG_TRY(sg::graph_buffer(g, "msb_hov", sizeof(HubV) * (size_t)hvslots, &a.hov));
+    SG_TRY(sg::graph_buffer(g, "msb_hoc", sizeof(HubC) * (size_t)hcslots, &a.hoc));
+    SG_TRY(sg::graph_buffer(g, "msb_li", sizeof(LevelInfo) * (size_t)caps.li_cap, &a.li));
+    SG_TRY(sg::graph_buffer(g, "msb_ctl", sizeof(Ctl), &a.ctl));
+    SG_TRY(sg::graph_buffer(g, "msb_src", 4 * 64, &a.src));
+    a.lcap = caps.lcap;
+    a.hv_cap = caps.hv_cap;
+    a.hc_cap = caps.hc_cap;
+    a.li_cap = caps.li_cap;
+    if (bc) {
+      SG_TRY(sg::graph_buffer(g, "msb_lm", 8 * (size_t)lslots, &a.lm));
+      if (directed) {
+        SG_TRY(sg::graph_buffer(g, "msb_hiv", sizeof(HubV) * (size_t)hvslots, &a.hiv));
+        SG_TRY(sg::graph_buffer(g, "msb_hic", sizeof(HubC) * (size_t)hcslots, &a.hic));
+      }
+      SG_TRY(sg::graph_buffer(g, "msb_sm", 4 * 64 * (size_t)(n + 1), &a.sm));
+      SG_TRY(sg::graph_buffer(g, "msb_se", 2 * 64 * (size_t)(n + 1), &a.se));
+      SG_TRY(sg::graph_buffer(g, "msb_dl", 4 * 64 * (size_t)(n + 1), &a.dl));
+      SG_TRY(sg::graph_buffer(g, "msb_pS", 4 * 64 * (size_t)caps.p_cap, &a.pS));
+      SG_TRY(sg::graph_buffer(g, "msb_pE", 4 * 64 * (size_t)caps.p_cap, &a.pE));
+      a.p_cap = caps.p_cap;
+      a.bc64 = bc64;
+    } else {
+      a.depth = depth + (int64_t)b0 * n;
+    }
+    a.wait_ticks = dev->wait_ticks;
+    SG_HIP(hipMemcpyAsync((void*)a.src, sources + b0, 4 * (size_t)k, hipMemcpyHostToDevice, s));
+    ms_clear_kernel<<<grid_n, kBlock, 0, s>>>(n, a.vis, a.M, a.li, bc ? 2 : kLiRing, a.ctl);
+    ms_seed_kernel<OffT><<<1, 64, 0, s>>>(a);
+    SG_HIP(hipGetLastError());
+    void* args[] = {&a};
+    SG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(ms_kernel<OffT>), dim3(grid), dim3(kBlock), args,
+                                      0, s));
+    SG_HIP(hipMemcpyAsync(hctl, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    const uint32_t fl = hctl->flags;
+    SG_CHECK(!(fl & kTimeout), SG_ERR_HIP, "%s: a grid barrier waited longer than the device's wait limit",
+             bc ? "sg_bc_batched" : "sg_bfs_multi");
+    SG_CHECK(!(fl & kRange), SG_ERR_UNSUPPORTED, "sg_bc_batched: path counts beyond 2^%d", kExpMax);
+    if (fl & (kOvList | kOvHubV | kOvHubC | kOvPart | kOvLevels)) {
+      /* grow what overflowed to its bound and run the batch again (the
+       * bounds hold for any batch: each (vertex, lane) pair lies at one depth) */
+      SG_CHECK(bc, SG_ERR_HIP, "sg_bfs_multi: level list overflow");
+      bool grew = false;
+      auto grow = [&](int64_t& c, int64_t bound) {
+        if (c < bound) {
+          c = bound;
+          grew = true;
+        }
+      };
+      if (fl & kOvList) grow(caps.lcap, (int64_t)kmax * n + 64);
+      if (fl & kOvHubV) grow(caps.hv_cap, (int64_t)kmax * hubs_bound);
+      if (fl & kOvHubC) grow(caps.hc_cap, (int64_t)kmax * chunks_bound);
+      if (fl & kOvPart) grow(caps.p_cap, chunks_bound);
+      if (fl & kOvLevels) grow(caps.li_cap, std::min<int64_t>(n + 3, 8 * caps.li_cap));
+      SG_CHECK(grew, SG_ERR_HIP, "sg_bc_batched: list capacity exceeded its bound (flags %u)", fl);
+      continue; /* same batch, larger lists */
+    }
+    b0 += k;
+  }
+  if (bc) {
+    ms_add_to_float_kernel<<<grid_n, kBlock, 0, s>>>(bc64, out, n);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipStreamSynchronize(s));
+  }
+  return SG_OK;
+}
+
+int check_sources(sg_graph_t g, int32_t nsrc, const int32_t* sources, const char* who) {
+  SG_CHECK(nsrc >= 0, SG_ERR_INVALID, "%s: negative source count", who);
+  for (int32_t i = 0; i < nsrc; ++i)
+    SG_CHECK(sources[i] >= 0 && sources[i] < g->n, SG_ERR_INVALID, "%s: source %d out of range [0,%lld)", who,
+             sources[i], (long long)g->n);
+  return SG_OK;
+}
+
+} // namespace
+
+/* Multi-source BFS: 64 sources per batch, one bit-lane each. */
+extern "C" int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth) {
+  SG_CHECK(g && depth && (nsrc == 0 || sources_host), SG_ERR_INVALID, "sg_bfs_multi: null argument");
+  SG_TRY(check_sources(g, nsrc, sources_host, "sg_bfs_multi"));
+  if (nsrc == 0) return SG_OK;
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, false, nsrc, sources_host, depth, nullptr);
+  return launch_batches<uint32_t>(g, false, nsrc, sources_host, depth, nullptr);
+}
+
+/* Betweenness centrality over many sources, 64 per lane batch; the same sums
+ * as sg_bc (bc.hpp:118-200 per source) up to FP32 rounding. */
+extern "C" int sg_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float* bc) {
+  SG_CHECK(g && bc && (nsrc == 0 || sources_host), SG_ERR_INVALID, "sg_bc_batched: null argument");
+  SG_TRY(check_sources(g, nsrc, sources_host, "sg_bc_batched"));
+  if (nsrc == 0) return SG_OK;
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, true, nsrc, sources_host, nullptr, bc);
+  return launch_batches<uint32_t>(g, true, nsrc, sources_host, nullptr, bc);
+}

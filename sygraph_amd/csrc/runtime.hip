@@ -117,7 +117,20 @@ int sg_device_open(int ordinal, sg_device_t* out) {
   }
   d->host_scratch_bytes = 1 << 20;
   d->cus = d->prop.multiProcessorCount;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ordinal) != hipSuccess || khz <= 0) khz = 100000;
+  d->wall_khz = khz;
+  d->wait_ticks = (uint64_t)khz * 1000ull * 30ull; /* 30 s */
   *out = d;
+  return SG_OK;
+}
+
+int sg_device_set_wait_limit(sg_device_t dev, double seconds) {
+  SG_CHECK(dev, SG_ERR_INVALID, "sg_device_set_wait_limit: null device");
+  SG_CHECK(seconds > 0.0 && seconds <= 3600.0, SG_ERR_INVALID, "sg_device_set_wait_limit: %g s outside (0, 3600]",
+           seconds);
+  dev->wait_ticks = (uint64_t)(seconds * 1000.0 * (double)dev->wall_khz);
+  if (dev->wait_ticks == 0) dev->wait_ticks = 1;
   return SG_OK;
 }
 
