@@ -361,7 +361,11 @@ void sg_sssp_options_default(sg_sssp_options* opt);
 /* sg_sssp with options; the distances do not depend on delta. */
 int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt, void* dist, sg_sssp_stats* stats);
 /* CC::run (algorithms/cc.hpp:103-169): labels (device uint32[n]) converge to
- * the largest vertex id of each component. */
+ * the largest vertex id that reaches each vertex along out-arcs: the largest
+ * id of its component when the graph stores both directions of every arc
+ * (union-find), the reference's label propagation otherwise (directed
+ * graphs, and undirected ones whose CSR lacks some reverse arc; symmetry is
+ * checked once per graph). */
 int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);
 /* TC::run + getNumTriangles (algorithms/tc.hpp:67-138): exact count on the
  * (degree, id)-oriented DAG; adjacency lists must be sorted. per_vertex

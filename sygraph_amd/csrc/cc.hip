@@ -18,7 +18,11 @@
  *    its root.
  *  - directed graphs: label propagation along out-arcs with atomic max
  *    (labels[v] = largest id that reaches v), frontier by frontier, which is
- *    exactly the reference's fixpoint on a directed graph.
+ *    exactly the reference's fixpoint on a directed graph.  An undirected
+ *    graph whose CSR does not hold the reverse of every arc takes this path
+ *    too (the reference propagates along out-arcs whatever the graph's
+ *    flag); symmetry is checked once per graph by an order-independent hash
+ *    of the arcs against that of their reverses (sym_hash_kernel).
  * Arcs are visited through the degree-class queues of the whole vertex set
  * (thread / wave / workgroup bins), so hubs are split across the grid.
  */
@@ -94,6 +98,79 @@ __global__ void outside_kernel(const OffT* off, const uint32_t* parent, int64_t 
   }
 }
 
+/* Symmetry check: the sum over arcs (u,v) of H(u,v) - H(v,u) (mod 2^64), H a
+ * 64-bit mix of the ordered pair, is exactly zero when every arc has its
+ * reverse (the arc multiset equals its transpose) and nonzero otherwise
+ * except with probability ~2^-64.  A wave takes 64 vertices; lists longer
+ * than 64 arcs are summed by the whole wave. */
+__device__ __forceinline__ uint64_t pair_mix(uint32_t a, uint32_t b) {
+  uint64_t x = ((uint64_t)a << 32) | b; /* splitmix64 finaliser */
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) sym_hash_kernel(const OffT* off, const int32_t* cols, int64_t n,
+                                                          unsigned long long* acc) {
+  const int lane = lane_id();
+  const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+  uint64_t h = 0;
+  for (int64_t base = gw * 64; base < n; base += nw * 64) {
+    const int64_t u = base + lane;
+    int64_t b = 0, e = 0;
+    if (u < n) {
+      b = (int64_t)off[u];
+      e = (int64_t)off[u + 1];
+    }
+    const bool coop = e - b > 64;
+    if (!coop)
+      for (int64_t j = b; j < e; ++j) {
+        const uint32_t v = (uint32_t)cols[j];
+        h += pair_mix((uint32_t)u, v) - pair_mix(v, (uint32_t)u);
+      }
+    uint64_t heavy = ballot(coop);
+    while (heavy) {
+      const int k = __ffsll((unsigned long long)heavy) - 1;
+      heavy &= heavy - 1;
+      const int64_t kb = shfl_i64(b, k), ke = shfl_i64(e, k);
+      const uint32_t ku = (uint32_t)(base + k);
+      for (int64_t j = kb + lane; j < ke; j += 64) {
+        const uint32_t v = (uint32_t)cols[j];
+        h += pair_mix(ku, v) - pair_mix(v, ku);
+      }
+    }
+  }
+  h = wave_sum_u64(h);
+  if (lane == 0 && h) atomic_add(acc, (unsigned long long)h);
+}
+
+__global__ void zero_acc_kernel(unsigned long long* acc) {
+  if (threadIdx.x == 0) *acc = 0ull;
+}
+
+/* Whether an undirected graph stores both directions of every arc, checked
+ * once per graph (acc: one device word to sum into); cached in g. */
+template<typename OffT>
+int graph_symmetric(sg_graph_t g, unsigned long long* acc, bool* sym) {
+  if (g->symmetric < 0) {
+    sg_device_t dev = g->dev;
+    hipStream_t s = dev->stream;
+    auto* host = static_cast<unsigned long long*>(dev->host_scratch);
+    zero_acc_kernel<<<1, 64, 0, s>>>(acc);
+    sym_hash_kernel<OffT><<<sg::grid_for(dev, (g->n + 63) / 64 * 64, kBlock, 8), kBlock, 0, s>>>(
+        (const OffT*)g->offsets, g->cols, g->n, acc);
+    SG_HIP(hipGetLastError());
+    SG_HIP(hipMemcpyAsync(host, acc, 8, hipMemcpyDeviceToHost, s));
+    SG_HIP(hipStreamSynchronize(s));
+    g->symmetric = *host == 0ull ? 1 : 0;
+  }
+  *sym = g->symmetric == 1;
+  return SG_OK;
+}
+
 __global__ void iota_kernel(uint32_t* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = (uint32_t)i;
@@ -155,9 +232,11 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
   const OffT* off = (const OffT*)g->offsets;
   const int grid = dev->cus * 8;
   int32_t rounds = 0;
+  bool sym = false;
+  if (!(g->flags & SG_GRAPH_DIRECTED)) SG_TRY(graph_symmetric<OffT>(g, reinterpret_cast<unsigned long long*>(roots), &sym));
   iota_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(labels, n);
   fill_words_kernel<<<sg::grid_for(dev, w0, 256, 4), 256, 0, s>>>(bm[0], w0, n);
-  if (!(g->flags & SG_GRAPH_DIRECTED)) {
+  if (sym) {
     const int grid_n = sg::grid_for(dev, n, 256, 4);
     sample_link_kernel<OffT><<<grid_n, 256, 0, s>>>(off, g->cols, n, labels);
     flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
@@ -230,3 +309,4 @@ extern "C" int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds) {
   if (g->offset_bytes == 8) return run_cc<int64_t>(g, labels, rounds);
   return run_cc<uint32_t>(g, labels, rounds);
 }
+

@@ -123,6 +123,7 @@ struct sg_graph_s {
   uint32_t sssp_split_bits = 0;  /* the width's bit pattern */
   bool sssp_split_valid = false;
   int64_t total_degree = 0;
+  int symmetric = -1;           /* every arc (u,v) has its reverse (v,u): 1 / 0, -1 = not checked yet */
   void* bfs_ws = nullptr;       /* traversal workspaces, reused across calls */
   void* sssp_ws = nullptr;
   void* aux_queues = nullptr;   /* degree-class queue set shared by CC, multi-source BFS and batched BC */

@@ -386,3 +386,83 @@ def test_cc_sampling_shapes(sg, device, oracle_mod):
         c.init()
         c.run()
         assert np.array_equal(c.get_labels(), oracle_mod.cc(g))
+
+
+def test_tc_long_dag_lists_clique(sg, device, oracle_mod):
+    """A 1,100-vertex clique joined to a sparse RMAT graph: in the (degree,
+    id) orientation the clique's lowest vertices keep more than 1,024
+    higher-ranked neighbours, the lists that are binary-searched in global
+    memory instead of hashed in LDS.  The clique alone has C(1100, 3)
+    triangles, and vertex u of an isolated clique (all degrees equal, so rank
+    = id) is the lowest corner of C(1099 - u, 2) of them; joined to the RMAT
+    graph, total and per-vertex counts are checked against the brute-force
+    lowest-corner count."""
+    import scipy.sparse as sp
+    from math import comb
+    k = 1100
+    iu, iv = np.triu_indices(k, 1)
+    clique = oracle_mod.csr_from_pairs(k, iu, iv)
+    t = sg.TC(sg.Graph.from_csr(device, clique.offsets, clique.cols))
+    assert t.run(per_vertex=True) == comb(k, 3)
+    want = np.array([comb(k - 1 - u, 2) for u in range(k)], dtype=np.uint64)
+    assert np.array_equal(t.per_vertex.download(np.uint64)[:k], want)
+    r = oracle_mod.rmat(12, 8, seed=31)
+    n = k + r.n
+    src = np.repeat(np.arange(r.n), np.diff(r.offsets)) + k
+    keep = src < r.cols + k
+    rng = np.random.default_rng(31)
+    bridge_u = rng.integers(0, k, 3000)  # clique vertices wired into the sparse part
+    bridge_v = rng.integers(k, n, 3000)
+    g = oracle_mod.csr_from_pairs(n, np.concatenate([iu, src[keep], bridge_u]),
+                                  np.concatenate([iv, r.cols[keep].astype(np.int64) + k, bridge_v]))
+    deg = np.diff(g.offsets).astype(np.int64)
+    s = np.repeat(np.arange(g.n), deg)
+    d = g.cols.astype(np.int64)
+    up = (deg[s] < deg[d]) | ((deg[s] == deg[d]) & (s < d))
+    assert np.bincount(s[up], minlength=g.n).max() > 1024
+    D = sp.csr_matrix((np.ones(int(up.sum())), (s[up], d[up])), shape=(g.n, g.n))
+    want = np.asarray((D @ D).multiply(D).sum(axis=1)).ravel().astype(np.uint64)
+    assert int(want.sum()) == oracle_mod.tc(g)
+    t = sg.TC(sg.Graph.from_csr(device, g.offsets, g.cols))
+    assert t.run(per_vertex=True) == int(want.sum())
+    assert np.array_equal(t.per_vertex.download(np.uint64)[:g.n], want)
+
+
+def _max_reaching(n, src, dst):
+    """Fixpoint of the reference's propagation (cc.hpp:103-169) by numpy
+    sweeps: labels[v] = largest id with a directed path to v."""
+    lab = np.arange(n, dtype=np.int64)
+    while True:
+        new = lab.copy()
+        np.maximum.at(new, dst, lab[src])
+        if np.array_equal(new, lab):
+            return lab.astype(np.uint32)
+        lab = new
+
+
+def test_cc_asymmetric_csr_without_directed_flag(sg, device, oracle_mod):
+    """A CSR that lacks some reverse arcs but is not flagged directed gets the
+    reference's labels (largest id reaching each vertex along out-arcs), not
+    its weak components."""
+    rng = np.random.default_rng(77)
+    n, m = 4000, 9000
+    u, v = rng.integers(0, n, m), rng.integers(0, n, m)
+    keep = u != v
+    pairs = np.unique(np.stack([u[keep], v[keep]], 1), axis=0)
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(np.bincount(pairs[:, 0], minlength=n), out=off[1:])
+    cols = np.ascontiguousarray(pairs[:, 1], dtype=np.int32)
+    want = _max_reaching(n, pairs[:, 0], pairs[:, 1])
+    weak = oracle_mod.cc(oracle_mod.CSRGraph(n, off, cols))
+    assert not np.array_equal(want, weak)  # the two definitions differ on this graph
+    G = sg.Graph.from_csr(device, off, cols)  # directed=False
+    c = sg.CC(G)
+    c.init()
+    c.run()
+    assert np.array_equal(c.get_labels(), want)
+    # its symmetrised form still takes the union-find path: weak components
+    g = oracle_mod.csr_from_pairs(n, pairs[:, 0], pairs[:, 1])
+    c = sg.CC(sg.Graph.from_csr(device, g.offsets, g.cols))
+    c.init()
+    c.run()
+    assert np.array_equal(c.get_labels(), oracle_mod.cc(g))
