@@ -367,6 +367,14 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt, void* d
  * graphs, and undirected ones whose CSR lacks some reverse arc; symmetry is
  * checked once per graph). */
 int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds);
+/* sg_cc without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same labels as sg_cc;
+ * every buffer besides labels comes from a static array of the library
+ * (32 MiB per device, held until the process ends), so graphs of at most
+ * 2^27 vertices are accepted (SG_ERR_INVALID beyond).  Calls that overlap on
+ * one device share that array and race; calls one after another are safe. */
+int sg_cc_noalloc(sg_graph_t g, uint32_t* labels, int32_t* rounds);
 /* TC::run + getNumTriangles (algorithms/tc.hpp:67-138): exact count on the
  * (degree, id)-oriented DAG; adjacency lists must be sorted. per_vertex
  * (device uint64[n] or NULL) gets the triangles attributed to each DAG source. */

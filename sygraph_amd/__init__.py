@@ -165,6 +165,7 @@ _SIGS = {
     "sg_bfs": (_I, [_P, C.c_int32, C.POINTER(BfsOptions), _P, C.POINTER(BfsStats)]),
     "sg_sssp": (_I, [_P, C.c_int32, C.c_int32, _P, C.POINTER(SsspStats)]),
     "sg_cc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
+    "sg_cc_noalloc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
     "sg_tc": (_I, [_P, C.POINTER(C.c_uint64), _P]),
     "sg_bc": (_I, [_P, C.c_int32, _P, _P]),
     "sg_bc_batched": (_I, [_P, C.c_int32, _P, _P]),

@@ -290,9 +290,215 @@ int run_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
   return st;
 }
 
+/* ------------------------------------------------------------ no-alloc CC ---
+ * sg_cc_noalloc computes the labels of sg_cc with the same kernels for the
+ * hooks and flattens, but takes every buffer beyond the caller's labels from
+ * one file-scope array, so it never allocates device memory:
+ *   [0, kNaWords)            bitmap A: vertices outside the giant component
+ *                            (undirected) or the current frontier (directed)
+ *   [kNaWords, 2 kNaWords)   bitmap B: the next frontier (directed)
+ *   then kRootSamples roots of the sampled vertices (uint32), the giant root
+ *   and two frontier counters.
+ * The array holds bitmaps of up to kNaMaxN vertices (32 MiB in all); every
+ * device keeps its own copy at its own address, taken once per device when
+ * the device is opened (cc_scratch_init).  Calls on one device share it, so
+ * calls that overlap on one device race. */
+constexpr int64_t kNaMaxN = int64_t(1) << 27;
+constexpr int64_t kNaWords = kNaMaxN / 64;
+constexpr int64_t kNaRoots = 2 * kNaWords;                  /* kRootSamples uint32 */
+constexpr int64_t kNaMisc = kNaRoots + kRootSamples / 2;    /* giant, counter 0, counter 1 */
+constexpr int64_t kNaTotalWords = kNaMisc + 32;
+
+static __device__ __align__(256) uint64_t scratch[kNaTotalWords];
+
+/* The most frequent root of the sample (the smallest of equally frequent
+ * ones, as the host-side choice of sg_cc): one workgroup of kRootSamples
+ * threads sorts the sample in LDS and keeps the longest run. */
+__global__ void __launch_bounds__(kRootSamples) giant_kernel(const uint32_t* roots, uint64_t* giant) {
+  __shared__ uint32_t key[kRootSamples];
+  __shared__ unsigned long long best;
+  const int i = threadIdx.x;
+  key[i] = roots[i];
+  if (i == 0) best = 0ull;
+  __syncthreads();
+  for (int k = 2; k <= kRootSamples; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = i ^ j;
+      if (p > i) {
+        const bool up = (i & k) == 0;
+        const uint32_t a = key[i], b = key[p];
+        if ((a > b) == up) {
+          key[i] = b;
+          key[p] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (i == 0 || key[i] != key[i - 1]) {
+    int len = 1;
+    while (i + len < kRootSamples && key[i + len] == key[i]) ++len;
+    /* longest run first, then the smallest root */
+    const unsigned long long score = ((unsigned long long)len << 32) | (0xffffffffull - key[i]);
+    atomicMax(&best, score);
+  }
+  __syncthreads();
+  if (i == 0) *giant = 0xffffffffull - (best & 0xffffffffull);
+}
+
+/* outside_kernel with the giant root read from device memory */
+template<typename OffT>
+__global__ void outside_na_kernel(const OffT* off, const uint32_t* parent, int64_t n, int64_t w0,
+                                  const uint64_t* giant, uint64_t* bm) {
+  const int lane = lane_id();
+  const uint32_t gr = (uint32_t)*giant;
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = gw; w < w0; w += nw) {
+    const int64_t v = w * 64 + lane;
+    const bool keep = v < n && parent[v] != gr && (int64_t)(off[v + 1] - off[v]) > kSampleArcs;
+    const uint64_t m = ballot(keep);
+    if (lane == 0) bm[w] = m;
+  }
+}
+
+/* Visit the arcs [b, e) of the vertices whose bit is set in the wave's word:
+ * lists up to 64 arcs by their own lane, longer ones by the whole wave. */
+template<typename OffT, typename F>
+__device__ __forceinline__ void wave_word_arcs(const OffT* off, const int32_t* cols, int64_t w, uint64_t word,
+                                               int64_t skip, F&& f) {
+  const int lane = lane_id();
+  const int64_t u = w * 64 + lane;
+  int64_t b = 0, e = 0;
+  if ((word >> lane) & 1ull) {
+    b = (int64_t)off[u] + skip;
+    e = (int64_t)off[u + 1];
+    if (b > e) b = e;
+  }
+  const bool coop = e - b > 64;
+  if (!coop)
+    for (int64_t j = b; j < e; ++j) f((uint32_t)u, (uint32_t)cols[j]);
+  uint64_t heavy = ballot(coop);
+  while (heavy) {
+    const int k = __ffsll((unsigned long long)heavy) - 1;
+    heavy &= heavy - 1;
+    const int64_t kb = shfl_i64(b, k), ke = shfl_i64(e, k);
+    const uint32_t ku = (uint32_t)(w * 64 + k);
+    for (int64_t j = kb + lane; j < ke; j += 64) f(ku, (uint32_t)cols[j]);
+  }
+}
+
+/* vertices outside the giant component link their arcs after the sampled ones */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) hook_na_kernel(const OffT* off, const int32_t* cols, const uint64_t* bm,
+                                                         int64_t w0, uint32_t* parent) {
+  const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (int64_t w = gw; w < w0; w += nw) {
+    const uint64_t word = bm[w];
+    if (word) wave_word_arcs(off, cols, w, word, kSampleArcs, [&](uint32_t u, uint32_t v) { hook(parent, u, v); });
+  }
+}
+
+/* Directed rounds: every vertex of the frontier pushes its label along its
+ * out-arcs with atomic max; the vertices whose label grew form the next
+ * frontier, counted in ctr[round & 1] (ctr[(round + 1) & 1] is cleared for the
+ * next round); the frontier's words are cleared as they are read. */
+template<typename OffT>
+__global__ void __launch_bounds__(kBlock) maxprop_na_kernel(const OffT* off, const int32_t* cols, uint32_t* labels,
+                                                            uint64_t* cur, uint64_t* next, int64_t w0,
+                                                            unsigned long long* ctr, int32_t round) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomic_store(&ctr[(round + 1) & 1], 0ull);
+  const int64_t gw = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * kBlock) >> 6;
+  for (int64_t w = gw; w < w0; w += nw) {
+    const uint64_t word = cur[w];
+    if (!word) continue;
+    if (lane_id() == 0) cur[w] = 0ull;
+    uint32_t joined = 0;
+    wave_word_arcs(off, cols, w, word, 0, [&](uint32_t u, uint32_t v) {
+      const uint32_t lu = atomic_load(&labels[u]);
+      if (lu > atomic_load(&labels[v]) && lu > atomic_max(&labels[v], lu)) {
+        const uint64_t bit = 1ull << (v & 63);
+        if (!(atomic_load(&next[v >> 6]) & bit) && !(atomic_or(&next[v >> 6], bit) & bit)) ++joined;
+      }
+    });
+    const uint64_t tot = wave_sum_u64(joined);
+    if (lane_id() == 0 && tot) atomic_add(&ctr[round & 1], (unsigned long long)tot);
+  }
+}
+
+__global__ void zero_words_kernel(uint64_t* p, int64_t cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+template<typename OffT>
+int run_cc_noalloc(sg_graph_t g, uint32_t* labels, int32_t* rounds_out) {
+  sg_device_t dev = g->dev;
+  hipStream_t s = dev->stream;
+  const int64_t n = g->n, w0 = (n + 63) / 64;
+  uint64_t* base = static_cast<uint64_t*>(dev->cc_scratch);
+  uint64_t* bmA = base;
+  uint64_t* bmB = base + kNaWords;
+  uint32_t* roots = reinterpret_cast<uint32_t*>(base + kNaRoots);
+  uint64_t* giant = base + kNaMisc;
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(base + kNaMisc + 1);
+  const OffT* off = (const OffT*)g->offsets;
+  const int grid = dev->cus * 8;
+  const int grid_n = sg::grid_for(dev, n, 256, 4);
+  const int grid_w = sg::grid_for(dev, w0 * 64, kBlock, 8);
+  int32_t rounds = 0;
+  bool sym = false;
+  if (!(g->flags & SG_GRAPH_DIRECTED)) SG_TRY(graph_symmetric<OffT>(g, ctr, &sym));
+  iota_kernel<<<grid_n, 256, 0, s>>>(labels, n);
+  if (sym) {
+    sample_link_kernel<OffT><<<grid_n, 256, 0, s>>>(off, g->cols, n, labels);
+    flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
+    sample_roots_kernel<<<kRootSamples / 256, 256, 0, s>>>(labels, n, roots);
+    giant_kernel<<<1, kRootSamples, 0, s>>>(roots, giant);
+    outside_na_kernel<OffT><<<grid_w, kBlock, 0, s>>>(off, labels, n, w0, giant, bmA);
+    hook_na_kernel<OffT><<<grid, kBlock, 0, s>>>(off, g->cols, bmA, w0, labels);
+    flatten_kernel<<<grid_n, 256, 0, s>>>(labels, n);
+    SG_HIP(hipGetLastError());
+    rounds = 1;
+  } else {
+    /* every vertex starts active (the all-vertex pass of cc.hpp:135-145) */
+    auto* host = static_cast<unsigned long long*>(dev->host_scratch);
+    fill_words_kernel<<<sg::grid_for(dev, w0, 256, 4), 256, 0, s>>>(bmA, w0, n);
+    zero_words_kernel<<<sg::grid_for(dev, w0, 256, 4), 256, 0, s>>>(bmB, w0);
+    zero_words_kernel<<<1, 64, 0, s>>>(reinterpret_cast<uint64_t*>(ctr), 2);
+    uint64_t* cur = bmA;
+    uint64_t* nxt = bmB;
+    for (;;) {
+      maxprop_na_kernel<OffT><<<grid, kBlock, 0, s>>>(off, g->cols, labels, cur, nxt, w0, ctr, rounds);
+      SG_HIP(hipGetLastError());
+      SG_HIP(hipMemcpyAsync(host, ctr + (rounds & 1), 8, hipMemcpyDeviceToHost, s));
+      SG_HIP(hipStreamSynchronize(s));
+      ++rounds;
+      if (*host == 0ull) break;
+      uint64_t* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+  }
+  SG_HIP(hipStreamSynchronize(s));
+  if (rounds_out) *rounds_out = rounds;
+  return SG_OK;
+}
+
 } // namespace
 
 namespace sg {
+/* The device address of this device's copy of the no-alloc scratch array,
+ * taken once when the device is opened (the device is current). */
+void cc_scratch_init(sg_device_t dev) {
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  dev->cc_scratch = e == hipSuccess ? p : nullptr;
+  dev->cc_scratch_error = e;
+}
+
 void aux_queues_release(sg_graph_t g) {
   if (g && g->aux_queues) {
     auto* qs = static_cast<sg::trav::QueueSet*>(g->aux_queues);
@@ -310,3 +516,16 @@ extern "C" int sg_cc(sg_graph_t g, uint32_t* labels, int32_t* rounds) {
   return run_cc<uint32_t>(g, labels, rounds);
 }
 
+/* CC::run (algorithms/cc.hpp:103-169) with no device allocation: the labels
+ * of sg_cc, with every other buffer in the library's static scratch array. */
+extern "C" int sg_cc_noalloc(sg_graph_t g, uint32_t* labels, int32_t* rounds) {
+  SG_CHECK(g && labels, SG_ERR_INVALID, "sg_cc_noalloc: null argument");
+  SG_CHECK(g->n <= kNaMaxN, SG_ERR_INVALID,
+           "sg_cc_noalloc: %lld vertices exceed the static scratch's maximum of %lld (use sg_cc)",
+           (long long)g->n, (long long)kNaMaxN);
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  SG_CHECK(g->dev->cc_scratch, SG_ERR_HIP, "sg_cc_noalloc: address of the static scratch unavailable: %s",
+           hipGetErrorString(g->dev->cc_scratch_error));
+  if (g->offset_bytes == 8) return run_cc_noalloc<int64_t>(g, labels, rounds);
+  return run_cc_noalloc<uint32_t>(g, labels, rounds);
+}

@@ -59,6 +59,9 @@ struct sg_device_s {
    * with an error instead of waiting on */
   uint64_t wait_ticks = 0;
   int64_t wall_khz = 0;
+  /* this device's copy of sg_cc_noalloc's static scratch array (cc.hip) */
+  void* cc_scratch = nullptr;
+  hipError_t cc_scratch_error = hipSuccess;
   /* sg_device_close and every graph, frontier and event on the device hold
    * one reference; the stream and scratch go with the last one, so handles
    * may be released in any order */
@@ -67,6 +70,7 @@ struct sg_device_s {
 
 namespace sg {
 void device_release(sg_device_t dev);
+void cc_scratch_init(sg_device_t dev);
 
 /* Owning reference to a device, held by graphs, frontiers and events. */
 class DevRef {

@@ -121,6 +121,7 @@ int sg_device_open(int ordinal, sg_device_t* out) {
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ordinal) != hipSuccess || khz <= 0) khz = 100000;
   d->wall_khz = khz;
   d->wait_ticks = (uint64_t)khz * 1000ull * 30ull; /* 30 s */
+  sg::cc_scratch_init(d);
   *out = d;
   return SG_OK;
 }
