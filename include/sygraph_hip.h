@@ -150,6 +150,12 @@ typedef struct sg_graph_view {
 } sg_graph_view;
 /* GraphCSR::getDeviceGraph / getInverseDeviceGraph (graph_csr.hpp:206-208). */
 int sg_graph_get_view(sg_graph_t g, sg_graph_view* view);
+/* Build the per-graph helpers that BFS otherwise builds on its first call:
+ * the bitmap of vertices without in-arcs and the pull hint (per vertex, its
+ * in-neighbour of largest out-degree).  ms (may be NULL) receives their
+ * device time, 0 when already built.  No counterpart in the reference (its
+ * pull reads every in-list from the start). */
+int sg_graph_prepare(sg_graph_t g, float* ms);
 
 /* Synthetic graphs, generated and assembled on the device (symmetrise, drop
  * self loops, sort, deduplicate; int64 offsets).  No reference counterpart:
@@ -262,6 +268,11 @@ typedef struct sg_bfs_options {
   int32_t burst_arcs;      /* push levels whose frontier has at most this many arcs run back to back
                               inside one single-workgroup launch (default 4096, at most 65536;
                               negative: never) */
+  int32_t grid_burst_arcs; /* push levels whose frontier has more arcs than burst_arcs but at most this
+                              many run back to back inside one cooperative launch of the whole grid,
+                              one grid barrier per level, once two steps in a row have left such a
+                              level (default 262144, at most 2^24; negative: never) */
+  int32_t grid_burst_wgs_per_cu; /* workgroups per compute unit of that launch (default 1, at most 4) */
 } sg_bfs_options;
 
 typedef struct sg_level_stat {
@@ -400,6 +411,23 @@ int sg_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, float
  * launch with a grid barrier per level.  No counterpart in the reference
  * (its BFS takes one source). */
 int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth);
+/* Direction rule of each level of a multi-source batch.  policy: -1 pull when
+ * the level holds more than n/20 vertices (the default); SG_POLICY_PUSH /
+ * SG_POLICY_PULL always; SG_POLICY_HYBRID the reference's alpha / beta rule
+ * (algorithms/bfs.hpp:341-350) on the batch's totals (frontier arcs against
+ * the arcs of vertices some lane has not reached yet); SG_POLICY_LEARNED the
+ * learned policy's direction on the same totals (its work-mapping part is not
+ * used: a batch has one push and one pull mapping).  The depths do not
+ * depend on it. */
+typedef struct sg_msbfs_options {
+  int32_t policy;
+  float alpha, beta;
+  sg_policy_t learned;
+} sg_msbfs_options;
+void sg_msbfs_options_default(sg_msbfs_options* opt);
+/* sg_bfs_multi with a direction rule (opt may be NULL: the default). */
+int sg_bfs_multi_ex(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, const sg_msbfs_options* opt,
+                    int32_t* depth);
 
 #ifdef __cplusplus
 }

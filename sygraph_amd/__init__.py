@@ -69,7 +69,11 @@ class BfsOptions(C.Structure):
     _fields_ = [("policy", C.c_int32), ("load_balancer", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
                 ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p),
                 ("t_medium", C.c_int32), ("t_large", C.c_int32), ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("burst_arcs", C.c_int32)]
+                ("burst_arcs", C.c_int32), ("grid_burst_arcs", C.c_int32), ("grid_burst_wgs_per_cu", C.c_int32)]
+
+
+class MsbfsOptions(C.Structure):
+    _fields_ = [("policy", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float), ("learned", C.c_void_p)]
 
 
 class LevelStat(C.Structure):
@@ -164,12 +168,15 @@ _SIGS = {
     "sg_bfs_options_default": (None, [C.POINTER(BfsOptions)]),
     "sg_bfs": (_I, [_P, C.c_int32, C.POINTER(BfsOptions), _P, C.POINTER(BfsStats)]),
     "sg_sssp": (_I, [_P, C.c_int32, C.c_int32, _P, C.POINTER(SsspStats)]),
+    "sg_graph_prepare": (_I, [_P, C.POINTER(C.c_float)]),
     "sg_cc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
     "sg_cc_noalloc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
     "sg_tc": (_I, [_P, C.POINTER(C.c_uint64), _P]),
     "sg_bc": (_I, [_P, C.c_int32, _P, _P]),
     "sg_bc_batched": (_I, [_P, C.c_int32, _P, _P]),
     "sg_bfs_multi": (_I, [_P, C.c_int32, _P, _P]),
+    "sg_msbfs_options_default": (None, [C.POINTER(MsbfsOptions)]),
+    "sg_bfs_multi_ex": (_I, [_P, C.c_int32, _P, C.POINTER(MsbfsOptions), _P]),
 }
 
 
@@ -393,6 +400,12 @@ class Graph:
                                       w.ctypes.data_as(C.c_void_p) if w is not None else None))
         return off, col, w
 
+    def prepare(self) -> float:
+        """Build the BFS helpers (in-degree-0 bitmap, pull hint); returns their device time in ms."""
+        ms = C.c_float()
+        check(lib().sg_graph_prepare(self.h, C.byref(ms)))
+        return float(ms.value)
+
     def digest(self) -> tuple[int, int]:
         a, b = C.c_uint64(), C.c_uint64()
         check(lib().sg_graph_digest(self.h, C.byref(a), C.byref(b)))
@@ -550,7 +563,8 @@ class BFS:
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
             load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
             learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsStats:
-        """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu, burst_arcs."""
+        """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu, burst_arcs, grid_burst_arcs,
+        grid_burst_wgs_per_cu."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
         o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,
@@ -695,12 +709,31 @@ class BC:
         return self.values.download(np.float32, self.g.n)
 
 
-def bfs_multi(g: Graph, sources, on_device: bool = False):
+MSBFS_AUTO = -1  # pull when a level holds more than n / 20 vertices
+
+
+def msbfs_options(policy: int = MSBFS_AUTO, alpha: float = 15.0, beta: float = 24.0,
+                  learned: Optional["Policy"] = None) -> MsbfsOptions:
+    """Direction rule of a multi-source batch: MSBFS_AUTO (default), POLICY_PUSH / POLICY_PULL
+    (always), POLICY_HYBRID (alpha / beta on the batch's totals) or POLICY_LEARNED."""
+    o = MsbfsOptions()
+    lib().sg_msbfs_options_default(C.byref(o))
+    o.policy = policy
+    o.alpha = alpha
+    o.beta = beta
+    o.learned = learned.h if learned is not None else None
+    return o
+
+
+def bfs_multi(g: Graph, sources, on_device: bool = False, policy: int = MSBFS_AUTO,
+              learned: Optional["Policy"] = None):
     """Depths from several sources, row i = sources[i] (unreached = n + 1).
-    on_device: return the device buffer (int32[len(sources)][n]) instead."""
+    on_device: return the device buffer (int32[len(sources)][n]) instead.
+    policy / learned: the per-level direction rule (msbfs_options)."""
     s = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
     out = DeviceBuffer(g.dev, 4 * max(len(s) * g.n, 1), dtype=np.int32)
-    check(lib().sg_bfs_multi(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr)))
+    o = msbfs_options(policy, learned=learned)
+    check(lib().sg_bfs_multi_ex(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.byref(o), C.c_void_p(out.ptr)))
     if on_device:
         return out
     return out.download(np.int32, len(s) * g.n).reshape(len(s), g.n)

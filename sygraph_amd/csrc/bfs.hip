@@ -239,7 +239,10 @@ struct alignas(64) LevelLog {
  * step's completion event. */
 struct alignas(64) StepLog {
   int32_t step, last_level, done, n_push, n_pull;
+  int32_t next_dir; /* direction of the level after last_level */
+  int32_t fail;     /* a grid barrier timed out */
   int64_t reached_v, reached_e;
+  int64_t next_f_edges; /* its frontier's arcs */
 };
 
 constexpr int kRing = 8; /* steps in the log / event rings (> lookahead + 1) */
@@ -420,14 +423,16 @@ __device__ __forceinline__ void finish_level(const FinishArgs& f, BfsCtl* ctl, c
   ctl->burst = burst;
   ctl->skip = 0;
   if (!burst) {
-    StepLog sl;
+    StepLog sl{};
     sl.step = f.step;
     sl.last_level = level;
     sl.done = done;
     sl.n_push = n_push;
     sl.n_pull = n_pull;
+    sl.next_dir = next;
     sl.reached_v = (int64_t)reached;
     sl.reached_e = (int64_t)vdeg;
+    sl.next_f_edges = (int64_t)disc_deg;
     f.steplog[f.step % kRing] = sl;
   }
   __threadfence_system();
@@ -768,7 +773,7 @@ __device__ __forceinline__ void burst_body(const BurstArgs<OffT>& a, BfsCtl* ctl
     ctl->reached = reached;
     ctl->f_vertices = f_vertices;
     ctl->f_edges = f_edges;
-    StepLog sl;
+    StepLog sl{};
     sl.step = step;
     sl.last_level = level - 1;
     sl.done = done;
@@ -776,6 +781,7 @@ __device__ __forceinline__ void burst_body(const BurstArgs<OffT>& a, BfsCtl* ctl
     sl.n_pull = ctl->n_pull;
     sl.reached_v = (int64_t)reached;
     sl.reached_e = (int64_t)vdeg;
+    sl.next_f_edges = f_edges; /* the level handed back, a push level */
     steplog[step % kRing] = sl;
     __threadfence_system();
   }
@@ -809,6 +815,356 @@ __global__ void __launch_bounds__(kBurstThreads) write_dyn_kernel(CompactArgs<Of
   burst_body(ba, ctl, p, step, levlog, steplog);
 }
 
+/* ------------------------------------------------------- grid bursts --- */
+/* Push levels too large for one workgroup but far too small to fill the chip
+ * (a 4096 x 4096 grid has ~8,000 levels of a few thousand arcs each) still
+ * pay the ~25 us of a step's three launches.  A grid burst runs such levels
+ * in one cooperative launch, with one grid barrier per level instead: every
+ * workgroup of the launch is co-resident, the frontier lives in two global
+ * lists, and the totals that decide the next level are summed with atomics
+ * and read by every workgroup after the barrier, so all of them take the
+ * same decision.  It starts from the queues the previous step wrote (like
+ * level_kernel), keeps going while the policy pushes and the frontier has at
+ * most gb_arcs arcs, and hands the level at which it stops back exactly like
+ * a burst: traversed, described by the control block as a push level, its
+ * discoveries in visited and depth but not in prev, so that the same step's
+ * level_kernel skips it and its count pass counts and decides it.
+ *
+ * Lists: short adjacency lists (< kBurstBig arcs) as one entry each, laid
+ * end to end 64 at a time by a wave (merge path); longer ones as pieces of
+ * kGbPiece arcs, one wave per piece, so a hub is spread over many waves. */
+constexpr int kGbThreads = 512;
+constexpr int kGbWaves = kGbThreads / 64;
+constexpr int kGbPiece = 256;
+constexpr int64_t kGbDefaultArcs = 1 << 18;
+constexpr int64_t kGbMaxArcs = 1 << 24;
+
+/* per level: entries of the next short list and pieces of the next long
+ * list (three rotating copies, so the copy a level adds into was cleared two
+ * levels earlier and read one level earlier) */
+struct alignas(128) GbSums {
+  unsigned long long nsmall, npieces;
+  unsigned long long pad[14];
+};
+
+/* per workgroup and level: its discoveries, their arcs and their largest
+ * degree, written by the workgroup and read by every workgroup after the
+ * level's barrier (two rotating copies: a workgroup writes a copy only after
+ * the barrier that follows every read of its previous contents).  One slot
+ * per workgroup, not a shared atomic counter: hundreds of workgroups adding
+ * into one line serialize for tens of microseconds per level. */
+constexpr int kGbMaxWgs = 1024;
+struct alignas(32) GbSlot {
+  unsigned long long disc, deg, mx, pad;
+};
+
+template<typename OffT>
+struct GbArgs {
+  const OffT* off;
+  const int32_t* cols;
+  uint64_t* visited;
+  uint64_t* prev;
+  int32_t* depth;
+  QEntry* q[3]; /* the queues of the level about to run */
+  const QueueHdr* hdr;
+  QEntry* small[2];
+  QEntry* pieces[2];
+  int64_t cap_small, cap_pieces;
+  GbSums* sums;  /* [3] */
+  GbSlot* slots; /* [2][kGbMaxWgs] */
+  GridBar* bar;
+  int64_t gb_arcs;
+  uint64_t wait_ticks;
+};
+
+/* Claim the (up to) four arcs y[k] of each lane; new vertices get depth
+ * level + 1 and join the short list or, cut into pieces, the long list. */
+template<typename OffT>
+__device__ __forceinline__ void gb_claim4(const GbArgs<OffT>& a, const uint32_t (&y)[4], const bool (&ok)[4],
+                                          int32_t level, QEntry* out_small, QEntry* out_pieces, GbSums* sm,
+                                          BurstAcc& acc) {
+  const int lane = lane_id();
+  uint64_t old[4];
+  int64_t vs[4], ve[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    old[k] = ~0ull;
+    vs[k] = ve[k] = 0;
+    if (ok[k]) {
+      const uint64_t bit = 1ull << (y[k] & 63);
+      const uint64_t w = a.visited[y[k] >> 6];
+      if (!(w & bit)) {
+        old[k] = atomic_or(&a.visited[y[k] >> 6], bit);
+        vs[k] = (int64_t)a.off[y[k]];
+        ve[k] = (int64_t)a.off[y[k] + 1];
+      }
+    }
+  }
+  bool nw[4];
+  uint32_t ns = 0, np = 0, npc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    nw[k] = ((old[k] >> (y[k] & 63)) & 1ull) == 0ull;
+    const int64_t len = ve[k] - vs[k];
+    npc[k] = (nw[k] && len >= kBurstBig) ? (uint32_t)((len + kGbPiece - 1) / kGbPiece) : 0u;
+    ns += (nw[k] && len < kBurstBig) ? 1u : 0u;
+    np += npc[k];
+  }
+  const uint32_t is = wave_inclusive_scan(ns), ip = wave_inclusive_scan(np);
+  const uint32_t ts = __shfl(is, 63, 64), tp = __shfl(ip, 63, 64);
+  if ((ts | tp) == 0) return;
+  unsigned long long bs = 0, bp = 0;
+  if (lane == 63) {
+    if (ts) bs = atomic_add(&sm->nsmall, (unsigned long long)ts);
+    if (tp) bp = atomic_add(&sm->npieces, (unsigned long long)tp);
+  }
+  bs = (unsigned long long)shfl_i64((int64_t)bs, 63) + (is - ns);
+  bp = (unsigned long long)shfl_i64((int64_t)bp, 63) + (ip - np);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!nw[k]) continue;
+    const uint32_t v = y[k];
+    const int64_t len = ve[k] - vs[k];
+    a.depth[v] = level + 1;
+    ++acc.disc;
+    acc.deg += (unsigned long long)len;
+    acc.mx = (unsigned long long)len > acc.mx ? (unsigned long long)len : acc.mx;
+    if (npc[k] == 0) {
+      if ((int64_t)bs < a.cap_small) store_entry_coherent(out_small + bs, QEntry{vs[k], (int32_t)v, (int32_t)len});
+      ++bs;
+    } else {
+      for (uint32_t c = 0; c < npc[k]; ++c, ++bp) {
+        const int64_t o = (int64_t)c * kGbPiece;
+        const int64_t l = len - o < kGbPiece ? len - o : kGbPiece;
+        if ((int64_t)bp < a.cap_pieces) store_entry_coherent(out_pieces + bp, QEntry{vs[k] + o, (int32_t)v, (int32_t)l});
+      }
+    }
+  }
+}
+
+/* short lists, 64 entries per wave laid end to end, four arcs per lane per round */
+template<typename OffT>
+__device__ __forceinline__ void gb_visit_small(const GbArgs<OffT>& a, const QEntry* ents, int64_t cnt, int32_t level,
+                                               QEntry* out_small, QEntry* out_pieces, GbSums* sm, BurstAcc& acc,
+                                               int64_t gwave, int64_t nwaves, bool coherent) {
+  const int lane = lane_id();
+  for (int64_t base = gwave * 64; base < cnt; base += nwaves * 64) {
+    QEntry qe{0, 0, 0};
+    if (base + lane < cnt) qe = coherent ? load_entry_coherent(ents + base + lane) : ents[base + lane];
+    const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
+    const uint32_t total = __shfl(incl, 63, 64);
+    const uint32_t excl = incl - (uint32_t)qe.len;
+    for (uint32_t t0 = 0; t0 < total; t0 += 4 * 64) {
+      uint32_t y[4];
+      bool ok[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const uint32_t t = t0 + 64 * k + lane;
+        const int owner = scan_owner(incl, t);
+        const uint32_t before = __shfl(excl, owner, 64);
+        const int64_t st = shfl_i64(qe.start, owner);
+        ok[k] = t < total;
+        y[k] = ok[k] ? (uint32_t)a.cols[st + (t - before)] : 0u;
+      }
+      gb_claim4(a, y, ok, level, out_small, out_pieces, sm, acc);
+    }
+  }
+}
+
+/* long lists: a wave per entry (a piece, or a class 1 / class 2 queue entry),
+ * 256 arcs per round */
+template<typename OffT>
+__device__ __forceinline__ void gb_visit_pieces(const GbArgs<OffT>& a, const QEntry* ents, int64_t cnt, int32_t level,
+                                                QEntry* out_small, QEntry* out_pieces, GbSums* sm, BurstAcc& acc,
+                                                int64_t gwave, int64_t nwaves, bool coherent) {
+  const int lane = lane_id();
+  for (int64_t i = gwave; i < cnt; i += nwaves) {
+    const QEntry qe = coherent ? load_entry_coherent(ents + i) : ents[i];
+    for (int64_t o0 = 0; o0 < qe.len; o0 += 4 * 64) {
+      uint32_t y[4];
+      bool ok[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t o = o0 + 64 * k + lane;
+        ok[k] = o < qe.len;
+        y[k] = ok[k] ? (uint32_t)a.cols[qe.start + o] : 0u;
+      }
+      gb_claim4(a, y, ok, level, out_small, out_pieces, sm, acc);
+    }
+  }
+}
+
+template<typename OffT>
+__global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsCtl* ctl, DecideParams p, int32_t step,
+                                                            LevelLog* levlog, StepLog* steplog) {
+  __shared__ sg::policy::Staged s_pol;
+  __shared__ unsigned long long s_red[kGbWaves][3];
+  __shared__ unsigned long long s_tot[3];
+  if (ctl->done || ctl->skip || ctl->dir != 0 || ctl->f_edges > a.gb_arcs) return; /* the same for every workgroup */
+  stage_policy(p, &s_pol);
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const int64_t gwave = ((int64_t)blockIdx.x * kGbThreads + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * kGbThreads) >> 6;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  int32_t level = ctl->level, lb = ctl->lb;
+  int64_t f_vertices = ctl->f_vertices, f_edges = ctl->f_edges;
+  unsigned long long vdeg = ctl->visited_deg, reached = ctl->reached;
+  int32_t n_push = ctl->n_push, done = 0, act = 0;
+  const int32_t n_pull = ctl->n_pull;
+  const QueueHdr h0 = *a.hdr;
+  uint32_t bt = 0;
+  int64_t n_small = 0, n_pieces = 0;
+  __syncthreads();
+  for (int it = 0;; ++it) {
+    const int cur = it & 1;
+    GbSums* sm = a.sums + (it % 3);
+    if (lead) { /* the copy of the level after next: last read before the previous barrier */
+      unsigned long long* z = reinterpret_cast<unsigned long long*>(a.sums + ((it + 1) % 3));
+      for (int k = 0; k < 2; ++k) atomic_store(z + k, 0ull);
+    }
+    BurstAcc acc{0, 0, 0, 0, 0};
+    QEntry* out_small = a.small[cur ^ 1];
+    QEntry* out_pieces = a.pieces[cur ^ 1];
+    if (it == 0) {
+      gb_visit_small(a, a.q[0], h0.count[0], level, out_small, out_pieces, sm, acc, gwave, nwaves, false);
+      gb_visit_pieces(a, a.q[1], h0.count[1], level, out_small, out_pieces, sm, acc, gwave, nwaves, false);
+      gb_visit_pieces(a, a.q[2], h0.count[2], level, out_small, out_pieces, sm, acc, gwave, nwaves, false);
+    } else {
+      /* the previous level's discoveries join prev (nothing here reads prev) */
+      for (int64_t i = gwave * 64 + lane; i < n_small + n_pieces; i += nwaves * 64) {
+        const uint32_t v = (uint32_t)(i < n_small ? load_entry_coherent(a.small[cur] + i)
+                                                  : load_entry_coherent(a.pieces[cur] + i - n_small)).v;
+        atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
+      }
+      gb_visit_small(a, a.small[cur], n_small, level, out_small, out_pieces, sm, acc, gwave, nwaves, true);
+      gb_visit_pieces(a, a.pieces[cur], n_pieces, level, out_small, out_pieces, sm, acc, gwave, nwaves, true);
+    }
+    acc.disc = wave_sum_u64(acc.disc);
+    acc.deg = wave_sum_u64(acc.deg);
+    acc.mx = wave_max_u64(acc.mx);
+    if (lane == 0) {
+      s_red[wv][0] = acc.disc;
+      s_red[wv][1] = acc.deg;
+      s_red[wv][2] = acc.mx;
+    }
+    __syncthreads();
+    GbSlot* slots = a.slots + (int64_t)(it & 1) * kGbMaxWgs;
+    if (threadIdx.x == 0) {
+      unsigned long long d = 0, g = 0, x = 0;
+      for (int w = 0; w < kGbWaves; ++w) {
+        d += s_red[w][0];
+        g += s_red[w][1];
+        x = s_red[w][2] > x ? s_red[w][2] : x;
+      }
+      atomic_store(&slots[blockIdx.x].disc, d);
+      atomic_store(&slots[blockIdx.x].deg, g);
+      atomic_store(&slots[blockIdx.x].mx, x);
+    }
+    /* no fences: the lists, sums and slots are written and read coherently */
+    if (!grid_barrier<false>(a.bar, bt, a.wait_ticks, 0u)) break; /* timed out: reported through the bar flags */
+    if (threadIdx.x < 64) { /* wave 0 totals the slots of every workgroup */
+      unsigned long long d = 0, g = 0, x = 0;
+      for (uint32_t i = threadIdx.x; i < gridDim.x; i += 64) {
+        d += atomic_load(&slots[i].disc);
+        g += atomic_load(&slots[i].deg);
+        const unsigned long long y = atomic_load(&slots[i].mx);
+        x = y > x ? y : x;
+      }
+      d = wave_sum_u64(d);
+      g = wave_sum_u64(g);
+      x = wave_max_u64(x);
+      if (threadIdx.x == 0) {
+        s_tot[0] = d;
+        s_tot[1] = g;
+        s_tot[2] = x;
+      }
+    }
+    __syncthreads();
+    const unsigned long long disc = s_tot[0], disc_deg = s_tot[1], max_deg = s_tot[2];
+    const int64_t ns = (int64_t)atomic_load(&sm->nsmall), np = (int64_t)atomic_load(&sm->npieces);
+    const unsigned long long unexplored = p.total_degree - (vdeg + disc_deg);
+    act = decide_action(p, &s_pol, 0, disc, disc_deg, unexplored, level + 1, max_deg);
+    done = disc == 0 ? 1 : 0;
+    const bool more = !done && act / 3 == 0 && (int64_t)disc_deg <= a.gb_arcs && ns <= a.cap_small &&
+                      np <= a.cap_pieces;
+    if (!more && !done) break; /* the count pass of this step counts and decides this level */
+    if (lead && level < SG_MAX_LEVELS_RECORDED) {
+      LevelLog e;
+      e.level = level;
+      e.dir = 0;
+      e.done = done;
+      e.next_dir = act / 3;
+      e.action = lb;
+      e.burst = 2;
+      e.f_vertices = f_vertices;
+      e.f_edges = f_edges;
+      e.disc = (int64_t)disc;
+      e.disc_deg = (int64_t)disc_deg;
+      e.edges = f_edges;
+      e.scanned = f_vertices;
+      e.misses = 0;
+      e.max_deg = (int64_t)max_deg;
+      e.unexplored = (int64_t)unexplored;
+      levlog[level] = e;
+    }
+    vdeg += disc_deg;
+    reached += disc;
+    ++n_push;
+    f_vertices = (int64_t)disc;
+    f_edges = (int64_t)disc_deg;
+    lb = act % 3;
+    ++level;
+    n_small = ns;
+    n_pieces = np;
+    if (!more) break; /* done */
+  }
+  if (atomic_load(&a.bar->flags) & kBarTimeout) {
+    /* a barrier timed out: stop the traversal and report it in the step record */
+    if (threadIdx.x == 0) {
+      ctl->done = 1;
+      StepLog sl{};
+      sl.step = step;
+      sl.last_level = level;
+      sl.done = 1;
+      sl.fail = 1;
+      steplog[step % kRing] = sl;
+      __threadfence_system();
+    }
+    return;
+  }
+  if (!lead) return;
+  ctl->level = level;
+  ctl->dir = 0;
+  ctl->lb = lb;
+  ctl->done = done;
+  ctl->write = 0;
+  ctl->burst = 0;
+  ctl->skip = done ? 0 : 1;
+  ctl->n_push = n_push;
+  ctl->visited_deg = vdeg;
+  ctl->reached = reached;
+  ctl->f_vertices = f_vertices;
+  ctl->f_edges = f_edges;
+  StepLog sl{};
+  sl.step = step;
+  sl.last_level = level - 1;
+  sl.done = done;
+  sl.n_push = n_push;
+  sl.n_pull = n_pull;
+  sl.reached_v = (int64_t)reached;
+  sl.reached_e = (int64_t)vdeg;
+  sl.next_f_edges = f_edges;
+  steplog[step % kRing] = sl;
+  __threadfence_system();
+}
+
+__global__ void gb_reset_kernel(GridBar* bar, GbSums* sums) {
+  uint32_t* b = reinterpret_cast<uint32_t*>(bar);
+  for (int i = threadIdx.x; i < (int)(sizeof(GridBar) / 4); i += blockDim.x) b[i] = 0u;
+  unsigned long long* z = reinterpret_cast<unsigned long long*>(sums);
+  for (int i = threadIdx.x; i < (int)(3 * sizeof(GbSums) / 8); i += blockDim.x) z[i] = 0ull;
+}
+
 __global__ void copy_words_kernel(uint64_t* dst, uint64_t* dst2, const uint64_t* src, int64_t nw) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t x = src[i];
@@ -832,6 +1188,13 @@ struct BfsWorkspace {
   BfsCtl* ctl = nullptr;
   QEntry* burst_list[4] = {nullptr, nullptr, nullptr, nullptr}; /* kBurstMaxArcs entries each */
   uint32_t* arrive = nullptr; /* arrival counters of the count and write passes (kArriveWords each), zero between launches */
+  /* grid bursts, allocated on first use for gb_arcs arcs */
+  int64_t gb_arcs = 0;
+  GridBar* gb_bar = nullptr;
+  GbSums* gb_sums = nullptr;
+  GbSlot* gb_slots = nullptr;
+  QEntry* gb_small[2] = {nullptr, nullptr};
+  QEntry* gb_pieces[2] = {nullptr, nullptr};
   LevelLog* levlog = nullptr; /* SG_MAX_LEVELS_RECORDED entries, pinned coherent host memory */
   StepLog* steplog = nullptr; /* kRing entries, pinned coherent host memory */
   /* per step: after its traversal kernel (record_levels only) and at its end */
@@ -850,6 +1213,13 @@ void ws_free(BfsWorkspace* w) {
   (void)hipFree(w->ctl);
   for (QEntry* l : w->burst_list) (void)hipFree(l);
   (void)hipFree(w->arrive);
+  (void)hipFree(w->gb_bar);
+  (void)hipFree(w->gb_sums);
+  (void)hipFree(w->gb_slots);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(w->gb_small[k]);
+    (void)hipFree(w->gb_pieces[k]);
+  }
   if (w->levlog) (void)hipHostFree(w->levlog);
   if (w->steplog) (void)hipHostFree(w->steplog);
   for (int r = 0; r < kRing; ++r)
@@ -858,6 +1228,40 @@ void ws_free(BfsWorkspace* w) {
   for (hipEvent_t e : {w->ev_begin, w->ev_end, w->ev_first})
     if (e) (void)hipEventDestroy(e);
   delete w;
+}
+
+/* list capacities of a grid burst of at most gb_arcs frontier arcs: a level
+ * discovers at most its frontier's arcs, and while the discoveries' arcs stay
+ * within gb_arcs their pieces number at most gb_arcs / 256 + gb_arcs / 64
+ * (a larger count ends the burst, whose lists are then not needed) */
+inline int64_t gb_cap_small(int64_t arcs) { return arcs + 64; }
+inline int64_t gb_cap_pieces(int64_t arcs) { return arcs / 32 + 64; }
+
+int gb_buffers(sg_graph_t g, BfsWorkspace* w, int64_t arcs) {
+  if (w->gb_arcs >= arcs) return SG_OK;
+  sg_device_t dev = g->dev;
+  SG_HIP(hipStreamSynchronize(dev->stream));
+  (void)hipFree(w->gb_bar);
+  (void)hipFree(w->gb_sums);
+  (void)hipFree(w->gb_slots);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(w->gb_small[k]);
+    (void)hipFree(w->gb_pieces[k]);
+    w->gb_small[k] = w->gb_pieces[k] = nullptr;
+  }
+  w->gb_bar = nullptr;
+  w->gb_sums = nullptr;
+  w->gb_slots = nullptr;
+  w->gb_arcs = 0;
+  SG_TRY(sg::alloc_device(dev, (void**)&w->gb_bar, sizeof(GridBar)));
+  SG_TRY(sg::alloc_device(dev, (void**)&w->gb_sums, 3 * sizeof(GbSums)));
+  SG_TRY(sg::alloc_device(dev, (void**)&w->gb_slots, 2 * kGbMaxWgs * sizeof(GbSlot)));
+  for (int k = 0; k < 2; ++k) {
+    SG_TRY(sg::alloc_device(dev, (void**)&w->gb_small[k], sizeof(QEntry) * gb_cap_small(arcs)));
+    SG_TRY(sg::alloc_device(dev, (void**)&w->gb_pieces[k], sizeof(QEntry) * gb_cap_pieces(arcs)));
+  }
+  w->gb_arcs = arcs;
+  return SG_OK;
 }
 
 int workspace_for(sg_graph_t g, BfsWorkspace** out) {
@@ -967,6 +1371,19 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   /* work-mapping variant under the fixed policies (see sg_bfs_options) */
   const int32_t lb_fixed = opt.load_balancer == SG_LB_WORKGROUP ? 1 : (opt.load_balancer == SG_LB_SUBGROUP ? 2 : 0);
   const int64_t burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
+  const int64_t gb_arcs = opt.grid_burst_arcs < 0 ? 0 : (opt.grid_burst_arcs == 0 ? kGbDefaultArcs : opt.grid_burst_arcs);
+  int gb_grid = 0;
+  if (gb_arcs > 0 && opt.policy != SG_POLICY_PULL) {
+    SG_TRY(gb_buffers(g, ws, gb_arcs));
+    static int occ = 0; /* per offset width (one instantiation each) */
+    if (occ == 0) {
+      int nb = 0;
+      SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gburst_kernel<OffT>, kGbThreads, 0));
+      occ = std::max(1, nb);
+    }
+    gb_grid = dev->cus * std::min(occ, std::max(1, opt.grid_burst_wgs_per_cu > 0 ? opt.grid_burst_wgs_per_cu : 1));
+    gb_grid = std::min(gb_grid, kGbMaxWgs);
+  }
   init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
                                             opt.policy == SG_POLICY_PULL ? 1 : 0,
                                             opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed, burst_arcs);
@@ -1002,7 +1419,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     ba.tune = tune;
     return ba;
   };
-  auto enqueue = [&](int32_t step) -> int {
+  auto enqueue = [&](int32_t step, bool gb) -> int {
     BfsArgs<OffT> la = a;
     const int cur = step & 1, nxt = (step + 1) & 1, r = step % kRing;
     la.stats = ws->stats + kSlots * cur;
@@ -1017,6 +1434,31 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     /* timing: the traversal kernel runs from the previous step's end (or
      * ev_first) to ev_mid; each event is a stream barrier, so only these */
     if (timed && step == 0) SG_HIP(hipEventRecord(ws->ev_first, s));
+    if (gb) { /* a grid burst ahead of the traversal kernel (returns at once unless the level qualifies) */
+      GbArgs<OffT> gba;
+      gba.off = off;
+      gba.cols = g->cols;
+      gba.visited = ws->visited;
+      gba.prev = ws->prev;
+      gba.depth = depth;
+      for (int k = 0; k < 3; ++k) gba.q[k] = Q.q[cur][k];
+      gba.hdr = Q.hdr + cur;
+      for (int k = 0; k < 2; ++k) {
+        gba.small[k] = ws->gb_small[k];
+        gba.pieces[k] = ws->gb_pieces[k];
+      }
+      gba.cap_small = gb_cap_small(gb_arcs);
+      gba.cap_pieces = gb_cap_pieces(gb_arcs);
+      gba.sums = ws->gb_sums;
+      gba.slots = ws->gb_slots;
+      gba.bar = ws->gb_bar;
+      gba.gb_arcs = gb_arcs;
+      gba.wait_ticks = dev->wait_ticks;
+      gb_reset_kernel<<<1, 256, 0, s>>>(ws->gb_bar, ws->gb_sums);
+      int32_t stp = step;
+      void* args[] = {&gba, &ws->ctl, &P, &stp, &ws->levlog, &ws->steplog};
+      SG_HIP(launch_resident(reinterpret_cast<const void*>(gburst_kernel<OffT>), gb_grid, kGbThreads, args, s));
+    }
     level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
     FinishArgs fa{Q.blocks, Q.nblocks, Q.hdr + nxt, step, ws->levlog, ws->steplog, ws->arrive};
@@ -1033,14 +1475,23 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
    * traversal; the kernel returns at once unless init set ctl->burst */
   if (burst_arcs > 0 && opt.policy != SG_POLICY_PULL)
     burst_kernel<OffT><<<1, kBurstThreads, 0, s>>>(burst_args(0), ws->ctl, P, 0, ws->levlog, ws->steplog);
-  SG_TRY(enqueue(0));
-  for (int k = 1; k <= lookahead; ++k) SG_TRY(enqueue(k));
+  SG_TRY(enqueue(0, false));
+  for (int k = 1; k <= lookahead; ++k) SG_TRY(enqueue(k, false));
   StepLog last{};
   int32_t first_level = 0;
+  /* steps in a row that left a push level of at most gb_arcs arcs (and more
+   * than a single-workgroup burst takes) for the next step; from two on, the
+   * steps enqueued start with a grid burst */
+  int32_t small_push = 0;
   for (int32_t step = 0;; ++step) {
     const int r = step % kRing;
     SG_HIP(hipEventSynchronize(ws->ev_done[r]));
     std::memcpy(&last, (const void*)&ws->steplog[r], sizeof(last));
+    if (last.fail) {
+      SG_HIP(hipStreamSynchronize(s));
+      sg::set_error("sg_bfs: a grid barrier waited longer than the device's wait limit (step %d)", step);
+      return SG_ERR_HIP;
+    }
     if (last.step != step || last.last_level < first_level) {
       SG_HIP(hipStreamSynchronize(s));
       sg::set_error("sg_bfs: step record %d out of order (found step %d, level %d)", step, last.step,
@@ -1084,7 +1535,10 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       return SG_ERR_STATE;
     }
     first_level = last.last_level + 1;
-    SG_TRY(enqueue(step + 1 + lookahead));
+    const bool qualifies = gb_grid > 0 && last.next_dir == 0 && last.next_f_edges > burst_arcs &&
+                           last.next_f_edges <= gb_arcs;
+    small_push = qualifies ? small_push + 1 : 0;
+    SG_TRY(enqueue(step + 1 + lookahead, small_push >= 2));
   }
   if (want_total) SG_HIP(hipEventRecord(ws->ev_end, s));
   SG_HIP(hipStreamSynchronize(s));
@@ -1135,6 +1589,9 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
            opt.t_medium, opt.t_large, opt.chunk, opt.blocks_per_cu);
   SG_CHECK(opt.burst_arcs <= kBurstMaxArcs, SG_ERR_INVALID, "sg_bfs: burst_arcs %d above %d", opt.burst_arcs,
            kBurstMaxArcs);
+  SG_CHECK(opt.grid_burst_arcs <= kGbMaxArcs && opt.grid_burst_wgs_per_cu >= 0 && opt.grid_burst_wgs_per_cu <= 4,
+           SG_ERR_INVALID, "sg_bfs: grid_burst_arcs %d above %lld or grid_burst_wgs_per_cu %d outside [0, 4]",
+           opt.grid_burst_arcs, (long long)kGbMaxArcs, opt.grid_burst_wgs_per_cu);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
   if (stats) {

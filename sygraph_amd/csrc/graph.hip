@@ -614,6 +614,31 @@ int sg_graph_destroy(sg_graph_t g) {
   return SG_OK;
 }
 
+int sg_graph_prepare(sg_graph_t g, float* ms) {
+  SG_CHECK(g, SG_ERR_INVALID, "sg_graph_prepare: null graph");
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  hipStream_t s = g->dev->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  SG_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    sg::set_error("sg_graph_prepare: event creation failed");
+    return SG_ERR_HIP;
+  }
+  int st = hipEventRecord(e0, s) == hipSuccess ? SG_OK : SG_ERR_HIP;
+  if (st == SG_OK) st = sg::graph_ensure_isolated(g);
+  if (st == SG_OK) st = sg::graph_ensure_pull_hint(g);
+  if (st == SG_OK && hipEventRecord(e1, s) != hipSuccess) st = SG_ERR_HIP;
+  if (st == SG_OK && hipEventSynchronize(e1) != hipSuccess) st = SG_ERR_HIP;
+  float t = 0.f;
+  if (st == SG_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess) st = SG_ERR_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st == SG_ERR_HIP) sg::set_error("sg_graph_prepare: event timing failed");
+  if (st == SG_OK && ms) *ms = t;
+  return st;
+}
+
 int sg_graph_get_view(sg_graph_t g, sg_graph_view* v) {
   SG_CHECK(g && v, SG_ERR_INVALID, "sg_graph_get_view: null argument");
   v->n = g->n;

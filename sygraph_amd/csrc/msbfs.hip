@@ -52,6 +52,7 @@
 #include <string>
 #include <vector>
 
+#include "policy.hpp"
 #include "traverse.hpp"
 
 using namespace sg::trav;
@@ -74,8 +75,10 @@ struct alignas(128) LevelInfo {
   unsigned long long count, arcs, base;
   unsigned long long hov, hov_base, hoc, hoc_base; /* out-hubs: vertices, chunks */
   unsigned long long hiv, hiv_base, hic, hic_base; /* in-hubs (directed BC) */
-  unsigned long long pad[5];
+  unsigned long long mx;                           /* largest out-degree among the level's vertices */
+  unsigned long long pad[4];
 };
+constexpr int kLevelInfoZero = 12; /* leading words of a LevelInfo cleared before the level fills */
 
 /* a vertex with a long list: its chunks are [first, first + nch) of the level's chunk list */
 struct HubV {
@@ -90,9 +93,12 @@ struct Ctl {
   uint32_t top[kBarStride];
   uint32_t gen[kBarStride];
   uint32_t flags;
-  int32_t levels; /* levels of the batch (depths 0 .. levels-1) */
-  uint32_t pad[30];
+  int32_t levels;                /* levels of the batch (depths 0 .. levels-1) */
+  unsigned long long full_deg;   /* out-degrees of the vertices every lane has reached */
+  int32_t n_pull;                /* levels expanded by pulling */
+  uint32_t pad[25];
 };
+static_assert(sizeof(Ctl) % 8 == 0, "Ctl is cleared in 4-byte words");
 
 template<typename OffT>
 struct MsArgs {
@@ -129,6 +135,15 @@ struct MsArgs {
   const int32_t* src;
   Ctl* ctl;
   uint64_t wait_ticks;
+  /* direction of each level's expansion: -1 pull when the level holds more
+   * than n/20 vertices; SG_POLICY_PUSH / _PULL always; SG_POLICY_HYBRID the
+   * reference's alpha / beta rule (algorithms/bfs.hpp:341-350) on the batch's
+   * totals; SG_POLICY_LEARNED the learned policy on the same totals */
+  int32_t dir_policy;
+  float alpha, beta;
+  const float* mlp;
+  float f6, f7;
+  unsigned long long total_degree;
 };
 
 /* --------------------------------------------------------- device helpers --- */
@@ -221,27 +236,23 @@ __device__ __forceinline__ Bases next_bases(const MsArgs<OffT>& a, int32_t d) {
   return b;
 }
 
-/* Append the vertices this wave touched first (first == true) to level
- * nx's list, with their out-degrees and, for long lists, their chunks.
- * All lanes call. */
+/* Write the entries whose list positions `rel` (relative to the level's
+ * base) are known: the vertex id and, for lists longer than kHub arcs, the
+ * hub and chunk records.  All lanes call; lanes with ok == false only take
+ * part in the wave's reservations. */
 template<typename OffT>
-__device__ __forceinline__ void append(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, bool first,
-                                       int32_t v) {
-  const uint64_t any = ballot(first);
-  if (!any) return;
+__device__ __forceinline__ void place(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, bool ok, int32_t v,
+                                     uint64_t rel) {
   const int64_t cap_l = a.ring ? a.lcap : a.lcap - b.l;
   const int64_t cap_hv = a.ring ? a.hv_cap : a.hv_cap - b.hov;
   const int64_t cap_hc = a.ring ? a.hc_cap : a.hc_cap - b.hoc;
-  const uint64_t rel = wave_reserve(&nx->count, first ? 1u : 0u);
   int64_t deg = 0;
-  if (first) {
+  if (ok) {
     if ((int64_t)rel < cap_l) a.lv[b.l + (int64_t)rel] = v;
     else raise_flag(a.ctl, kOvList);
     deg = (int64_t)(a.off[v + 1] - a.off[v]);
   }
-  const uint64_t arcs = wave_sum_u64((uint64_t)deg);
-  if (lane_id() == 0 && arcs) atomic_add(&nx->arcs, (unsigned long long)arcs);
-  uint32_t nch = (first && deg > kHub) ? (uint32_t)((deg + kHub - 1) / kHub) : 0u;
+  uint32_t nch = (ok && deg > kHub) ? (uint32_t)((deg + kHub - 1) / kHub) : 0u;
   if (ballot(nch != 0)) {
     const uint64_t hv = wave_reserve(&nx->hov, nch ? 1u : 0u);
     const uint64_t hc = wave_reserve(&nx->hoc, nch);
@@ -255,7 +266,7 @@ __device__ __forceinline__ void append(const MsArgs<OffT>& a, LevelInfo* nx, con
     }
   }
   if (a.bc && a.directed) { /* in-arc chunks for the path counts */
-    const int64_t ideg = first ? (int64_t)(a.ioff[v + 1] - a.ioff[v]) : 0;
+    const int64_t ideg = ok ? (int64_t)(a.ioff[v + 1] - a.ioff[v]) : 0;
     nch = ideg > kHub ? (uint32_t)((ideg + kHub - 1) / kHub) : 0u;
     if (ballot(nch != 0)) {
       const int64_t cap_iv = a.hv_cap - b.hiv, cap_ic = a.hc_cap - b.hic;
@@ -273,6 +284,95 @@ __device__ __forceinline__ void append(const MsArgs<OffT>& a, LevelInfo* nx, con
   }
 }
 
+/* Append the vertices this wave touched first (first == true) to level
+ * nx's list, with their out-degrees and, for long lists, their chunks.
+ * All lanes call.  One atomic per wave and call: the level-0 seeds only; the
+ * expansions stage their discoveries (Stage below). */
+template<typename OffT>
+__device__ __forceinline__ void append(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, bool first,
+                                       int32_t v) {
+  const uint64_t any = ballot(first);
+  if (!any) return;
+  const uint64_t rel = wave_reserve(&nx->count, first ? 1u : 0u);
+  place(a, nx, b, first, v, rel);
+  const uint64_t deg = first ? (uint64_t)(a.off[v + 1] - a.off[v]) : 0ull;
+  const uint64_t arcs = wave_sum_u64(deg), mx = wave_max_u64(deg);
+  if (lane_id() == 0 && arcs) {
+    atomic_add(&nx->arcs, (unsigned long long)arcs);
+    atomic_max(&nx->mx, (unsigned long long)mx);
+  }
+}
+
+/* A wave's discoveries of one phase, staged in LDS and written to the level
+ * list kStage at a time with one atomic, so that a dense level (millions of
+ * discoveries) does not serialize on its list counter; the level's arc count
+ * and largest degree are summed per workgroup and added once. */
+constexpr uint32_t kStage = 448;
+struct Stage {
+  int32_t* buf; /* this wave's LDS region, kStage ids */
+  uint32_t n;   /* staged ids (the same in every lane) */
+  uint64_t arcs, mx; /* per lane */
+};
+
+template<typename OffT>
+__device__ __forceinline__ void stage_flush(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st) {
+  if (st.n == 0) return;
+  const int lane = lane_id();
+  unsigned long long base = 0;
+  if (lane == 0) base = atomic_add(&nx->count, (unsigned long long)st.n);
+  base = (unsigned long long)shfl_i64((int64_t)base, 0);
+  for (uint32_t i0 = 0; i0 < st.n; i0 += 64) {
+    const uint32_t i = i0 + lane;
+    const bool ok = i < st.n;
+    const int32_t v = ok ? st.buf[i] : 0;
+    if (ok) {
+      const uint64_t deg = (uint64_t)(a.off[v + 1] - a.off[v]);
+      st.arcs += deg;
+      st.mx = deg > st.mx ? deg : st.mx;
+    }
+    place(a, nx, b, ok, v, base + i);
+  }
+  st.n = 0;
+}
+
+/* stage the vertices this wave touched first; all lanes call */
+template<typename OffT>
+__device__ __forceinline__ void stage_add(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st, bool first,
+                                          int32_t v) {
+  const uint64_t m = ballot(first);
+  if (!m) return;
+  const uint32_t c = (uint32_t)__popcll(m);
+  if (st.n + c > kStage) stage_flush(a, nx, b, st);
+  if (first) st.buf[st.n + lanes_below(m)] = v;
+  st.n += c;
+}
+
+/* End of a phase: flush every wave, then one add (arcs) and one max (largest
+ * degree) per workgroup.  Every thread of the workgroup calls. */
+template<typename OffT>
+__device__ __forceinline__ void stage_finish(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st,
+                                             unsigned long long* s_red) {
+  stage_flush(a, nx, b, st);
+  const uint64_t arcs = wave_sum_u64(st.arcs), mx = wave_max_u64(st.mx);
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    s_red[2 * wv] = arcs;
+    s_red[2 * wv + 1] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0, x = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t += s_red[2 * w];
+      x = s_red[2 * w + 1] > x ? s_red[2 * w + 1] : x;
+    }
+    if (t) atomic_add(&nx->arcs, t);
+    if (x) atomic_max(&nx->mx, x);
+  }
+  __syncthreads();
+  st.arcs = st.mx = 0;
+}
+
 /* New lanes for v from a frontier mask fm: OR them into M_{d+1}[v]; true when
  * this call was the first to give v any lane at depth d+1. */
 __device__ __forceinline__ bool relax(uint64_t fm, int32_t v, const uint64_t* vis, const uint64_t* Mc, uint64_t* Mn) {
@@ -283,21 +383,27 @@ __device__ __forceinline__ bool relax(uint64_t fm, int32_t v, const uint64_t* vi
 }
 
 /* A wave's walk over the (entry, lane) pairs of 64 entry masks (lane i holds
- * entry i's mask): an entry per step (lane = bit) when the masks are dense,
- * 64 pairs per step when they are sparse -- whichever takes fewer steps. */
+ * entry i's mask and its list length len): an entry per step (lane = bit,
+ * every lane walking the same list) or 64 pairs per step from several
+ * entries (each lane its own list, the step as long as its longest list) --
+ * whichever takes fewer arc steps: the flat walk suits sparse masks over
+ * lists of similar length (grids), the entry walk dense masks and lists of
+ * very different lengths (power-law graphs). */
 struct PairWalk {
   uint64_t words;
   uint32_t incl, pc, total, steps;
   bool flat;
 };
-__device__ __forceinline__ PairWalk pair_walk(uint64_t mine) {
+__device__ __forceinline__ PairWalk pair_walk(uint64_t mine, int64_t len) {
   PairWalk w;
   w.words = ballot(mine != 0);
   w.pc = (uint32_t)__popcll(mine);
   w.incl = wave_inclusive_scan(w.pc);
   w.total = __shfl(w.incl, 63, 64);
   const uint32_t flat_steps = (w.total + 63) / 64, word_steps = (uint32_t)__popcll(w.words);
-  w.flat = flat_steps < word_steps;
+  const uint64_t l = mine ? (uint64_t)len + 1 : 0ull;
+  const uint64_t word_cost = wave_sum_u64(l), flat_cost = (uint64_t)flat_steps * wave_max_u64(l);
+  w.flat = flat_cost < word_cost;
   w.steps = w.flat ? flat_steps : word_steps;
   return w;
 }
@@ -353,7 +459,7 @@ __device__ __forceinline__ void store_norm(const MsArgs<OffT>& a, int64_t at, fl
 template<typename OffT>
 __device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, int64_t base, int64_t cnt,
                                             const uint64_t* Mc, uint64_t* Mn, LevelInfo* nx, const Bases& nb,
-                                            int64_t gwave, int64_t nwaves) {
+                                            Stage& stg, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
     const int64_t i = g * 64 + lane;
@@ -388,7 +494,7 @@ __device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, in
 #pragma unroll
       for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(f[k], v[k], a.vis, Mc, Mn);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) append(a, nx, nb, first[k], v[k]);
+      for (int k = 0; k < 4; ++k) stage_add(a, nx, nb, stg, first[k], v[k]);
     }
   }
 }
@@ -396,7 +502,7 @@ __device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, in
 /* Push expansion of the chunks of L_d's long lists: a wave per chunk. */
 template<typename OffT>
 __device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC* hc, int64_t nc, const uint64_t* Mc,
-                                              uint64_t* Mn, LevelInfo* nx, const Bases& nb, int64_t gwave,
+                                              uint64_t* Mn, LevelInfo* nx, const Bases& nb, Stage& st, int64_t gwave,
                                               int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t c = gwave; c < nc; c += nwaves) {
@@ -415,7 +521,7 @@ __device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC*
 #pragma unroll
       for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(fm, v[k], a.vis, Mc, Mn);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) append(a, nx, nb, first[k], v[k]);
+      for (int k = 0; k < 4; ++k) stage_add(a, nx, nb, st, first[k], v[k]);
     }
   }
 }
@@ -425,7 +531,7 @@ __device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC*
  * than 64 arcs are ORed by the whole wave. */
 template<typename OffT>
 __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_t* Mc, uint64_t* Mn, LevelInfo* nx,
-                                            const Bases& nb, int64_t gwave, int64_t nwaves) {
+                                            const Bases& nb, Stage& st, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t base = gwave * 64; base < a.n; base += nwaves * 64) {
     const int64_t v = base + lane;
@@ -462,7 +568,7 @@ __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_
     const uint64_t add = acc & want;
     const bool first = add != 0;
     if (first) Mn[v] = add;
-    append(a, nx, nb, first, (int32_t)v);
+    stage_add(a, nx, nb, st, first, (int32_t)v);
   }
 }
 
@@ -475,11 +581,19 @@ __device__ __forceinline__ void visit_level(const MsArgs<OffT>& a, int32_t d, in
     const int64_t i = g * 64 + lane;
     int32_t u = 0;
     uint64_t m = 0;
+    uint64_t fdeg = 0;
     if (i < cnt) {
       u = a.lv[base + i];
       m = Mc[u];
-      a.vis[u] |= m;
+      const uint64_t old = a.vis[u];
+      a.vis[u] = old | m;
       if (a.bc) a.lm[base + i] = m;
+      if (a.dir_policy >= SG_POLICY_HYBRID && old != a.full && (old | m) == a.full)
+        fdeg = (uint64_t)(a.off[u + 1] - a.off[u]);
+    }
+    if (a.dir_policy >= SG_POLICY_HYBRID) {
+      fdeg = wave_sum_u64(fdeg);
+      if (lane == 0 && fdeg) atomic_add(&a.ctl->full_deg, (unsigned long long)fdeg);
     }
     if (!a.bc) {
       uint64_t w = m;
@@ -519,7 +633,7 @@ __device__ __forceinline__ void sigma_pairs(const MsArgs<OffT>& a, int64_t base,
       e = (int64_t)a.ioff[u + 1];
       if (e - b <= kHub) own = Mc[u];
     }
-    PairWalk w = pair_walk(own);
+    PairWalk w = pair_walk(own, e - b);
     for (uint32_t step = 0; step < w.steps; ++step) {
       int owner, l;
       const bool ok = pair_item(w, own, step, owner, l);
@@ -636,7 +750,7 @@ __device__ __forceinline__ void delta_pairs(const MsArgs<OffT>& a, int64_t base,
       e = (int64_t)a.off[u + 1];
       if (e - b <= kHub) own = a.lm[base + i];
     }
-    PairWalk w = pair_walk(own);
+    PairWalk w = pair_walk(own, e - b);
     for (uint32_t step = 0; step < w.steps; ++step) {
       int owner, l;
       const bool ok = pair_item(w, own, step, owner, l);
@@ -747,13 +861,44 @@ __device__ __forceinline__ void accumulate(const MsArgs<OffT>& a, int64_t base, 
 }
 
 /* ------------------------------------------------------------- the kernel --- */
+/* Direction of level d's expansion (the same in every workgroup: the inputs
+ * are the totals every workgroup reads after the barrier).  dir: the
+ * previous level's direction. */
+template<typename OffT>
+__device__ __forceinline__ int32_t ms_direction(const MsArgs<OffT>& a, const sg::policy::Staged* pol, int32_t d,
+                                                int32_t dir, const LevelInfo* L, int64_t cnt) {
+  switch (a.dir_policy) {
+  case SG_POLICY_PUSH: return 0;
+  case SG_POLICY_PULL: return 1;
+  case SG_POLICY_HYBRID:
+  case SG_POLICY_LEARNED: {
+    if (d == 0) return 0; /* the sources push, as a single-source traversal starts */
+    const unsigned long long fe = atomic_load(&L->arcs), mx = atomic_load(&L->mx);
+    const unsigned long long unexplored = a.total_degree - atomic_load(&a.ctl->full_deg);
+    if (a.dir_policy == SG_POLICY_LEARNED)
+      return sg::policy::infer_staged(pol, (double)cnt, (double)fe, (double)unexplored, d, (double)mx, a.f6, a.f7) / 3;
+    if (dir == 0) return ((float)(uint32_t)fe > (float)(uint32_t)unexplored / a.alpha) ? 1 : 0;
+    return ((float)cnt < (float)a.n / a.beta) ? 0 : 1;
+  }
+  default: return cnt * 20 > a.n ? 1 : 0;
+  }
+}
+
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
+  __shared__ sg::policy::Staged s_pol;
+  __shared__ int32_t s_stage[kBlock / 64][kStage];
+  __shared__ unsigned long long s_red[2 * (kBlock / 64)];
+  Stage st{s_stage[threadIdx.x >> 6], 0u, 0ull, 0ull};
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
   const int64_t n = a.n;
   uint32_t t = 0;
-  int32_t d = 0;
+  int32_t d = 0, dir = 0, n_pull = 0;
+  if (a.dir_policy == SG_POLICY_LEARNED) {
+    sg::policy::stage(&s_pol, a.mlp, (int)threadIdx.x, kBlock);
+    __syncthreads();
+  }
   int64_t max_chunks = 0; /* largest chunk list of a level (the backward pass's partial rows) */
   for (;; ++d) {
     LevelInfo* L = level(a, d);
@@ -772,7 +917,7 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       LevelInfo* z = level(a, d + 2); /* the level after next starts from zero */
       unsigned long long* zp = reinterpret_cast<unsigned long long*>(z);
-      for (int k = 0; k < 11; ++k) atomic_store(zp + k, 0ull);
+      for (int k = 0; k < kLevelInfoZero; ++k) atomic_store(zp + k, 0ull);
       atomic_store(&nx->base, (unsigned long long)nb.l);
       atomic_store(&nx->hov_base, (unsigned long long)nb.hov);
       atomic_store(&nx->hoc_base, (unsigned long long)nb.hoc);
@@ -782,12 +927,15 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
     const int64_t hoc = (int64_t)atomic_load(&L->hoc);
     max_chunks = hoc > max_chunks ? hoc : max_chunks;
     const HubC* hocp = a.hoc + (a.ring ? (int64_t)(d & 3) * a.hc_cap : (int64_t)atomic_load(&L->hoc_base));
-    if (cnt * 20 > n) {
-      expand_pull(a, Mc, Mn, nx, nb, gwave, nwaves);
+    dir = ms_direction(a, &s_pol, d, dir, L, cnt);
+    n_pull += dir;
+    if (dir) {
+      expand_pull(a, Mc, Mn, nx, nb, st, gwave, nwaves);
     } else {
-      expand_push(a, d, base, cnt, Mc, Mn, nx, nb, gwave, nwaves);
-      expand_chunks(a, hocp, hoc, Mc, Mn, nx, nb, gwave, nwaves);
+      expand_push(a, d, base, cnt, Mc, Mn, nx, nb, st, gwave, nwaves);
+      expand_chunks(a, hocp, hoc, Mc, Mn, nx, nb, st, gwave, nwaves);
     }
+    stage_finish(a, nx, nb, st, s_red);
     visit_level(a, d, base, cnt, Mc, gwave, nwaves);
     if (d >= 2) {
       LevelInfo* O = level(a, d - 2);
@@ -810,7 +958,10 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
       if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl->levels = d;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctl->levels = d;
+    a.ctl->n_pull = n_pull;
+  }
   if (!a.bc) return;
   /* the partial rows must hold every level's chunks before anything is added
    * to bc (a batch that stops here runs again with more rows) */
@@ -914,7 +1065,8 @@ struct Caps {
 };
 
 template<typename OffT>
-int launch_batches(sg_graph_t g, bool bc, int32_t nsrc, const int32_t* sources, int32_t* depth, float* out) {
+int launch_batches(sg_graph_t g, bool bc, int32_t nsrc, const int32_t* sources, int32_t* depth, float* out,
+                   const sg_msbfs_options& opt) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   const int64_t n = g->n, m = g->m;
@@ -1004,13 +1156,19 @@ int launch_batches(sg_graph_t g, bool bc, int32_t nsrc, const int32_t* sources, 
       a.depth = depth + (int64_t)b0 * n;
     }
     a.wait_ticks = dev->wait_ticks;
+    a.dir_policy = opt.policy;
+    a.alpha = opt.alpha;
+    a.beta = opt.beta;
+    a.mlp = opt.policy == SG_POLICY_LEARNED ? sg::policy_device_params(opt.learned) : nullptr;
+    a.f6 = sg::policy::feature(6, 0, 0, 0, 0, 0, (double)n, (double)m);
+    a.f7 = sg::policy::feature(7, 0, 0, 0, 0, 0, (double)n, (double)m);
+    a.total_degree = (unsigned long long)g->total_degree;
     SG_HIP(hipMemcpyAsync((void*)a.src, sources + b0, 4 * (size_t)k, hipMemcpyHostToDevice, s));
     ms_clear_kernel<<<grid_n, kBlock, 0, s>>>(n, a.vis, a.M, a.li, bc ? 2 : kLiRing, a.ctl);
     ms_seed_kernel<OffT><<<1, 64, 0, s>>>(a);
     SG_HIP(hipGetLastError());
     void* args[] = {&a};
-    SG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(ms_kernel<OffT>), dim3(grid), dim3(kBlock), args,
-                                      0, s));
+    SG_HIP(launch_resident(reinterpret_cast<const void*>(ms_kernel<OffT>), grid, kBlock, args, s));
     SG_HIP(hipMemcpyAsync(hctl, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     SG_HIP(hipStreamSynchronize(s));
     const uint32_t fl = hctl->flags;
@@ -1056,14 +1214,38 @@ int check_sources(sg_graph_t g, int32_t nsrc, const int32_t* sources, const char
 
 } // namespace
 
-/* Multi-source BFS: 64 sources per batch, one bit-lane each. */
-extern "C" int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth) {
+extern "C" void sg_msbfs_options_default(sg_msbfs_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->policy = -1;
+  opt->alpha = 15.0f;
+  opt->beta = 24.0f;
+}
+
+/* Multi-source BFS: 64 sources per batch, one bit-lane each, with a choice
+ * of the per-level direction rule. */
+extern "C" int sg_bfs_multi_ex(sg_graph_t g, int32_t nsrc, const int32_t* sources_host,
+                               const sg_msbfs_options* opt_in, int32_t* depth) {
   SG_CHECK(g && depth && (nsrc == 0 || sources_host), SG_ERR_INVALID, "sg_bfs_multi: null argument");
   SG_TRY(check_sources(g, nsrc, sources_host, "sg_bfs_multi"));
+  sg_msbfs_options opt;
+  if (opt_in) opt = *opt_in;
+  else sg_msbfs_options_default(&opt);
+  SG_CHECK(opt.policy >= -1 && opt.policy <= SG_POLICY_LEARNED, SG_ERR_INVALID, "sg_bfs_multi: policy %d out of range",
+           opt.policy);
+  SG_CHECK(opt.policy != SG_POLICY_LEARNED || opt.learned, SG_ERR_INVALID,
+           "sg_bfs_multi: the learned policy needs a trained sg_policy_t in options.learned");
+  SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
+           "sg_bfs_multi: alpha and beta must be positive");
   if (nsrc == 0) return SG_OK;
   SG_HIP(hipSetDevice(g->dev->ordinal));
-  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, false, nsrc, sources_host, depth, nullptr);
-  return launch_batches<uint32_t>(g, false, nsrc, sources_host, depth, nullptr);
+  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, false, nsrc, sources_host, depth, nullptr, opt);
+  return launch_batches<uint32_t>(g, false, nsrc, sources_host, depth, nullptr, opt);
+}
+
+/* Multi-source BFS: 64 sources per batch, one bit-lane each. */
+extern "C" int sg_bfs_multi(sg_graph_t g, int32_t nsrc, const int32_t* sources_host, int32_t* depth) {
+  return sg_bfs_multi_ex(g, nsrc, sources_host, nullptr, depth);
 }
 
 /* Betweenness centrality over many sources, 64 per lane batch; the same sums
@@ -1073,6 +1255,8 @@ extern "C" int sg_bc_batched(sg_graph_t g, int32_t nsrc, const int32_t* sources_
   SG_TRY(check_sources(g, nsrc, sources_host, "sg_bc_batched"));
   if (nsrc == 0) return SG_OK;
   SG_HIP(hipSetDevice(g->dev->ordinal));
-  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, true, nsrc, sources_host, nullptr, bc);
-  return launch_batches<uint32_t>(g, true, nsrc, sources_host, nullptr, bc);
+  sg_msbfs_options opt;
+  sg_msbfs_options_default(&opt);
+  if (g->offset_bytes == 8) return launch_batches<int64_t>(g, true, nsrc, sources_host, nullptr, bc, opt);
+  return launch_batches<uint32_t>(g, true, nsrc, sources_host, nullptr, bc, opt);
 }

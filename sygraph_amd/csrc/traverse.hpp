@@ -856,4 +856,123 @@ inline int compact_write_only(hipStream_t s, QueueSet& qs, int dst, const OffT* 
   return SG_OK;
 }
 
+/* ------------------------------------------------- grid barrier --- */
+/* Barrier over all workgroups of a cooperative launch (every workgroup
+ * co-resident).  Arrival counters only grow -- barrier t completes at t
+ * arrivals per member -- so nothing is reset between barriers; the block is
+ * zeroed before each launch.  Workgroups are grouped by blockIdx % 8 (the
+ * XCD they are dealt to): a group's last arrival bumps the top counter, whose
+ * last arrival publishes generation t.  The wait is bounded: past wait_ticks
+ * of the constant-rate clock kBarTimeout is raised and every workgroup
+ * leaves.  Returns false when the launch must stop (a flag of stop_mask is
+ * set); the caller's stores before the barrier are visible to every
+ * workgroup after it (agent-scope release / acquire). */
+/* Launch a kernel whose workgroups meet at grid barriers.  The caller sizes
+ * the grid from the kernel's occupancy so that every workgroup is resident at
+ * once on the device.  The launch is cooperative where the runtime grants it
+ * (the runtime then also checks that the grid fits); where it refuses for
+ * another reason than the grid's size -- a process whose device another
+ * process is already running cooperative launches on gets "the operation
+ * cannot be performed in the present state" -- the same grid is launched as
+ * an ordinary kernel, and the barriers' bounded wait still ends it should its
+ * workgroups not all become resident.  The refusal is remembered for the
+ * process. */
+inline hipError_t launch_resident(const void* fn, int grid, int block, void** args, hipStream_t s) {
+  static std::atomic<int> coop_refused{0};
+  if (!coop_refused.load(std::memory_order_relaxed)) {
+    const hipError_t e = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, s);
+    if (e == hipSuccess || e == hipErrorCooperativeLaunchTooLarge) return e;
+    (void)hipGetLastError();
+    coop_refused.store(1, std::memory_order_relaxed);
+  }
+  return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
+}
+
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 32; /* counters 128 bytes apart */
+constexpr uint32_t kBarTimeout = 1u << 31;
+
+struct GridBar {
+  uint32_t grp[kBarGroups * kBarStride];
+  uint32_t top[kBarStride];
+  uint32_t gen[kBarStride];
+  uint32_t flags;
+  uint32_t pad[31];
+};
+
+__device__ __forceinline__ void bar_raise(GridBar* b, uint32_t f) { atomic_or(&b->flags, f); }
+
+/* kFences: the caller exchanges data through ordinary loads and stores, so
+ * the barrier releases (an L2 write-back of every workgroup's XCD) and
+ * acquires (an L2 invalidate) at agent scope.  Without: everything the
+ * workgroups exchange is written with agent-coherent stores and read with
+ * agent-coherent loads (relaxed atomics at agent scope, which go to the
+ * point of coherence every XCD shares), and each wave only waits for its own
+ * stores to complete before its workgroup arrives -- no L2 is written back or
+ * invalidated, which costs tens of microseconds per barrier once hundreds of
+ * workgroups each do it. */
+template<bool kFences = true>
+__device__ __noinline__ bool grid_barrier(GridBar* c, uint32_t& t, uint64_t wait_ticks, uint32_t stop_mask) {
+  __shared__ uint32_t s_stop;
+  if (kFences) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  else wait_stores_complete();
+  __syncthreads();
+  ++t;
+  if (threadIdx.x == 0) {
+    const uint32_t grid = gridDim.x, g = blockIdx.x % kBarGroups;
+    const uint32_t ngroups = grid < (uint32_t)kBarGroups ? grid : (uint32_t)kBarGroups;
+    const uint32_t in_group = (grid - g + kBarGroups - 1) / kBarGroups;
+    bool released = false;
+    if (atomic_add(&c->grp[g * kBarStride], 1u) == t * in_group - 1) {
+      if (kFences) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (atomic_add(&c->top[0], 1u) == t * ngroups - 1) {
+        if (kFences) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+          __hip_atomic_store(&c->gen[0], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          atomic_store(&c->gen[0], t);
+        }
+        released = true;
+      }
+    }
+    if (!released) {
+      const uint64_t t0 = wall_clock64();
+      while ((kFences ? __hip_atomic_load(&c->gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                      : atomic_load(&c->gen[0])) < t) {
+        if (atomic_load(&c->flags) & (stop_mask | kBarTimeout)) break;
+        if ((uint64_t)(wall_clock64() - t0) > wait_ticks) {
+          bar_raise(c, kBarTimeout);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    s_stop = (atomic_load(&c->flags) & (stop_mask | kBarTimeout)) ? 1u : 0u;
+  }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __syncthreads();
+  if (kFences) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_stop == 0;
+}
+
+/* A queue entry written with agent-coherent stores / read with
+ * agent-coherent loads (entries one workgroup hands to another within a
+ * launch whose barriers do not fence). */
+__device__ __forceinline__ void store_entry_coherent(QEntry* p, const QEntry& e) {
+  uint64_t w[2];
+  __builtin_memcpy(w, &e, sizeof(w));
+  uint64_t* q = reinterpret_cast<uint64_t*>(p);
+  __hip_atomic_store(q, w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, w[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ QEntry load_entry_coherent(const QEntry* p) {
+  uint64_t w[2];
+  const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+  w[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  w[1] = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  QEntry e;
+  __builtin_memcpy(&e, w, sizeof(w));
+  return e;
+}
+
 } // namespace sg::trav

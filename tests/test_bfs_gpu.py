@@ -257,7 +257,7 @@ def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph,
         for burst in (-1, 64, 0, 65536):
             b = sg.BFS(G)
             b.init(src)
-            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst})
+            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": -1})
             assert np.array_equal(b.get_distances(), ref), (src, burst)
             assert st.levels == levels
             lv = [(st.level[i].direction, st.level[i].frontier_vertices, st.level[i].frontier_edges,
@@ -272,6 +272,61 @@ def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph,
             assert runs[0][2] >= levels // 2  # most levels of a high-diameter graph run inside bursts
     with pytest.raises(sg.SygraphError):
         b.run(pol, 15.0, 24.0, tuning={"burst_arcs": 65537})
+
+
+@pytest.mark.parametrize("graph", ["grid", "path", "rmat", "rmat_directed", "star_chain"])
+@pytest.mark.parametrize("policy", ["push", "hybrid"])
+def test_grid_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy):
+    """Push levels between the single-workgroup burst's limit and
+    grid_burst_arcs run back to back in one cooperative launch with a grid
+    barrier per level.  Every combination of thresholds gives the oracle's
+    depths and the per-level records of running each level on its own; on the
+    high-diameter graphs most levels run inside grid bursts when the
+    single-workgroup bursts are off.  star_chain: hubs of 3,000 arcs on a long
+    path, so grid bursts cut long lists into pieces."""
+    def star_chain():
+        hubs, leaves = 40, 3000
+        u, v = [], []
+        for h in range(hubs):
+            c = h * (leaves + 1)
+            if h:
+                u.append(c - leaves - 1)
+                v.append(c)
+            u.extend([c] * leaves)
+            v.extend(range(c + 1, c + 1 + leaves))
+        return oracle_mod.csr_from_pairs(hubs * (leaves + 1), np.array(u), np.array(v))
+    g = {"grid": lambda: oracle_mod.grid(120, 90, 0.15, seed=8),
+         "path": lambda: oracle_mod.grid(1, 3000),
+         "rmat": lambda: oracle_mod.rmat(13, 8, seed=19),
+         "rmat_directed": lambda: _directed_rmat(oracle_mod, 12, 8, 21),
+         "star_chain": star_chain}[graph]()
+    directed = graph == "rmat_directed"
+    G = sg.Graph.from_csr(device, g.offsets, g.cols, directed=directed)
+    pol = sg.POLICY_PUSH if policy == "push" else sg.POLICY_HYBRID
+    srcs = [int(np.argmax(g.degrees())), int(np.nonzero(g.degrees() > 0)[0][-1])]
+    configs = [(-1, -1), (-1, 0), (64, 0), (-1, 128), (0, 0), (-1, 1 << 24)]
+    for src in srcs:
+        ref, levels = oracle_mod.bfs(g, src)
+        runs = {}
+        for burst, grid in configs:
+            b = sg.BFS(G)
+            b.init(src)
+            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": grid})
+            assert np.array_equal(b.get_distances(), ref), (src, burst, grid)
+            assert st.levels == levels
+            lv = [(st.level[i].direction, st.level[i].frontier_vertices, st.level[i].frontier_edges,
+                   st.level[i].discovered, st.level[i].unexplored_edges, st.level[i].max_degree,
+                   st.level[i].action) for i in range(min(st.levels, sg.MAX_LEVELS_RECORDED))]
+            runs[(burst, grid)] = ((st.n_push, st.n_pull, st.reached_vertices, st.reached_edges), lv,
+                                   sum(1 for i in range(min(st.levels, sg.MAX_LEVELS_RECORDED))
+                                       if st.level[i].burst == 2))
+        assert runs[(-1, -1)][2] == 0
+        for c in configs[1:]:
+            assert runs[c][:2] == runs[(-1, -1)][:2], (src, c)
+        if graph in ("grid", "path") and policy == "push":
+            assert runs[(-1, 0)][2] >= min(levels, sg.MAX_LEVELS_RECORDED) // 2
+    with pytest.raises(sg.SygraphError):
+        b.run(pol, 15.0, 24.0, tuning={"grid_burst_arcs": (1 << 24) + 1})
 
 
 def test_bursts_with_fine_degree_classes(sg, device, oracle_mod):
