@@ -82,6 +82,24 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
     return edges_total, wall, kernel_ms_total, per_kernel, per_trav
 
 
+def run_policy_lb(sg, G, sources, policy, lb):
+    """One BFS per source under a fixed direction and work mapping (push:
+    bucketing bins / edge-balanced / wave-split large lists; pull: lane-serial
+    head of 16 / 4 / 64 in-arcs), after one untimed pass."""
+    b = sg.BFS(G)
+    for s in sources[:2]:
+        b.init(s)
+        b.run(policy, 15.0, 24.0, load_balancer=lb)
+    G.dev.synchronize()
+    t0 = time.perf_counter()
+    e = 0
+    for s in sources:
+        b.init(s)
+        e += b.run(policy, 15.0, 24.0, load_balancer=lb).reached_edges
+    G.dev.synchronize()
+    return e, time.perf_counter() - t0
+
+
 def copy_ceiling(sg, dev, mib: int = 1024, reps: int = 5):
     """Read+write bandwidth of a plain 16-byte streaming copy (sg_copy_device)
     of `mib` MiB, best of `reps`, timed with hipEvents on the library stream."""
@@ -105,11 +123,11 @@ def copy_ceiling(sg, dev, mib: int = 1024, reps: int = 5):
     return 2.0 * nbytes / (best / 1e3) / 1e9
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+def measured_traffic(kernel: str, summary: str = "bfs_hybrid_rmat24_traffic.json"):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter read on
     the calibration copy kernel), or None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "bfs_hybrid_rmat24_traffic.json")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -163,7 +181,8 @@ def secondary(sg, dev):
     """CC, TC and BC on the other configurations of the path (best of 3,
     graphs generated on the device): CC on the 4096 x 4096 grid with 10 % of
     the edges deleted and on RMAT scale 20 edge factor 32; BFS on that grid
-    with and without bursts of small levels; TC on that RMAT; BC from 64
+    with bursts of small levels (single-workgroup and grid-wide), with the
+    single-workgroup ones only, and with none; TC on that RMAT; BC from 64
     sources on RMAT scale 20 edge factor 48, a = 0.55, one source at a time
     and in a 64-lane batch, and 256 sources in four batches."""
     out = {}
@@ -190,10 +209,11 @@ def secondary(sg, dev):
     off, _, _ = G.download()
     src = int(np.nonzero(np.diff(off) > 0)[0][0])
     b = sg.BFS(G)
-    for key, burst in (("bfs_grid4096_ms", 0), ("bfs_grid4096_no_burst_ms", -1)):
+    for key, burst, grid in (("bfs_grid4096_ms", 0, 0), ("bfs_grid4096_no_grid_burst_ms", 0, -1),
+                             ("bfs_grid4096_no_burst_ms", -1, -1)):
         def one():
             b.init(src)
-            b.run(sg.POLICY_HYBRID, 15.0, 24.0, tuning={"burst_arcs": burst})
+            b.run(sg.POLICY_HYBRID, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": grid})
         out[key] = round(best(one, reps=2) * 1e3, 3)
     out["bfs_grid4096_levels"] = int(b.stats.levels)
     G.destroy()
@@ -202,10 +222,18 @@ def secondary(sg, dev):
     cc.init()
     t = best(cc.run)
     out["cc_rmat20_ms"] = round(t * 1e3, 3)
+    off, cols, _ = G.download()
+    b_cc = cc_bytes(off, cc.get_labels())
+    out["cc_rmat20_roofline"] = {"alg_bytes": round(b_cc), "achieved_gbs": round(b_cc / t / 1e9, 1),
+                                 "frac": round(b_cc / t / 1e9 / HBM_PEAK_GBS, 4), "time": "wall, whole call"}
     tc = sg.TC(G)
     t = best(tc.run)
     out["tc_rmat20_ms"] = round(t * 1e3, 3)
     out["tc_rmat20_triangles"] = tc.count
+    b_tc = tc_bytes(off, cols)
+    out["tc_rmat20_roofline"] = {"alg_bytes": round(b_tc), "achieved_gbs": round(b_tc / t / 1e9, 1),
+                                 "frac": round(b_tc / t / 1e9 / HBM_PEAK_GBS, 4), "time": "wall, whole call"}
+    del off, cols
     G.destroy()
     G = sg.Graph.rmat(dev, 20, 48, 0.55, 0.17, 0.17, seed=6)
     off, _, _ = G.download()
@@ -223,22 +251,113 @@ def secondary(sg, dev):
     t = best(lambda: bc.run(batched=True), reps=1)
     out["bc_rmat20_256src_batched_ms"] = round(t * 1e3, 3)
     G.destroy()
+    out.update(c4(sg, dev, best))
     return out
 
 
-def run_sssp(sg, G, sources, delta=0.0):
+def c4(sg, dev, best):
+    """The two batched-BC graphs with the committed fixture's sources
+    (tests/golden/c4_bc.json): RMAT scale 20 edge factor 56, a/b/c/d =
+    .55/.20/.20/.05, seed 6 (1,048,576 vertices, average degree 101.5), all
+    256 sources in four 64-lane batches; the 4096 x 4096 grid, the first
+    64-lane batch (one batch takes seconds there).  Batched BC and
+    multi-source BFS, best of the timed calls after a first call that sizes
+    the buffers."""
+    import ctypes as C
+    path = os.path.join(ROOT, "tests", "golden", "c4_bc.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        fx = json.load(f)
+    out = {}
+    for key, nsrc in (("rmat", 256), ("grid", 64)):
+        rec = fx[key]
+        p = rec["params"]
+        if p["kind"] == "rmat":
+            G = sg.Graph.rmat(dev, p["scale"], p["edge_factor"], p["a"], p["b"], p["c"], seed=p["seed"])
+        else:
+            G = sg.Graph.grid(dev, p["rows"], p["cols"], p["p_delete"], seed=p["seed"])
+        srcs = rec["sources"][:nsrc]
+        bc = sg.BC(G)
+        bc.init(srcs)
+        bc.run(batched=True)
+        t = best(lambda: bc.run(batched=True), reps=1 if key == "grid" else 2)
+        tag = "c4a_rmat20_ef56" if key == "rmat" else "c4b_grid4096"
+        out[f"{tag}_bc_{nsrc}src_batched_ms"] = round(t * 1e3, 3)
+        buf = dev.alloc(4 * nsrc * G.n)
+        sa = np.ascontiguousarray(np.asarray(srcs, dtype=np.int32))
+        run = lambda: sg.check(sg.lib().sg_bfs_multi(G.h, nsrc, sa.ctypes.data_as(C.c_void_p), C.c_void_p(buf.ptr)))
+        run()
+        t = best(run, reps=1 if key == "grid" else 2)
+        out[f"{tag}_msbfs_{nsrc}src_ms"] = round(t * 1e3, 3)
+        buf.free()
+        G.destroy()
+    return out
+
+
+def sssp_bytes(sg, st, n: int, off_bytes: int = 8) -> float:
+    """Algorithmic bytes of one SSSP run: 4 B per vertex (distance init) and
+    per relaxing round three bitmap passes, 2*w_o per frontier vertex and 12 B
+    per relaxed arc (4 B id + 4 B weight + 4 B distance atomic-min), from the
+    run's per-round counters."""
+    w0 = ((n + 63) // 64) * 8
+    b = 4.0 * n
+    for r in range(min(st.rounds, sg.MAX_LEVELS_RECORDED)):
+        b += 3 * w0 + 2 * off_bytes * st.round_vertices[r] + 12 * st.round_edges[r]
+    return b
+
+
+def run_sssp(sg, G, sources, delta=0.0, model=None):
+    """Wall time of one SSSP per source; `model` (a dict) receives the summed
+    algorithmic bytes and device time of the runs."""
     s = sg.SSSP(G)
     s.init(sources[0])
     s.run(delta=delta)  # workspace and mean weight outside the timed loop
     edges = 0
+    byts, dev_ms = 0.0, 0.0
     G.dev.synchronize()
     t0 = time.perf_counter()
     for src in sources:
         s.init(src)
         st = s.run(delta=delta)
         edges += st.reached_edges
+        if model is not None:
+            byts += sssp_bytes(sg, st, G.n)
+            dev_ms += st.kernel_ms
     G.dev.synchronize()
+    if model is not None:
+        model["bytes"], model["ms"] = byts, dev_ms
     return edges, time.perf_counter() - t0
+
+
+def tc_bytes(off: np.ndarray, cols: np.ndarray, off_bytes: int = 8) -> float:
+    """Algorithmic bytes of TC: orientation (offsets, ids and the heads'
+    offsets read, DAG ids written) plus the intersections, sum over u of
+    (2*w_o + 4*d+(u)) and over DAG arcs (u,v) of (2*w_o + 4*d+(v))."""
+    deg = np.diff(off).astype(np.int64)
+    n, m = len(deg), len(cols)
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    dst = cols.astype(np.int64)
+    up = (deg[src] < deg[dst]) | ((deg[src] == deg[dst]) & (src < dst))
+    dplus = np.bincount(src[up], minlength=n).astype(np.int64)
+    m_dag = int(up.sum())
+    orient = 2 * off_bytes * n + 4 * m + 2 * off_bytes * m + 4 * m_dag
+    inter = 2 * off_bytes * n + 4 * m_dag + 2 * off_bytes * m_dag + 4 * int(dplus[dst[up]].sum())
+    return float(orient + inter)
+
+
+def cc_bytes(off: np.ndarray, labels: np.ndarray, off_bytes: int = 8) -> float:
+    """Algorithmic bytes of undirected CC (sampling union-find): every vertex
+    links its first two arcs (offsets, ids, two parent reads each), two
+    flattens (parent read and written), the outside pass (offsets, parent),
+    and the vertices outside the largest component link their other arcs."""
+    deg = np.diff(off).astype(np.int64)
+    n = len(deg)
+    vals, counts = np.unique(labels, return_counts=True)
+    outside = labels != vals[np.argmax(counts)]
+    rest = int(np.maximum(deg[outside] - 2, 0).sum())
+    return float(2 * off_bytes * n + 12 * int(np.minimum(deg, 2).sum()) + 16 * n + (2 * off_bytes + 4) * n
+                 + 12 * rest)
 
 
 def cpu_baseline(oracle, G, sources, budget_s: float):
@@ -293,6 +412,8 @@ def main():
     t0 = time.perf_counter()
     G = sg.Graph.rmat(dev, a.scale, 16, 0.57, 0.19, 0.19, seed=2, weight_seed=3)
     log(f"graph: RMAT s{a.scale} n={G.n} m={G.m} generated on device in {time.perf_counter() - t0:.2f}s")
+    prep_ms = G.prepare()  # in-degree-0 bitmap and pull hint: per-graph preprocessing, outside the timed steps
+    log(f"graph preprocessing (in-degree-0 bitmap, pull hint): {prep_ms:.2f} ms device time")
     golden = os.path.join(ROOT, "tests", "golden", "rmat24_seed2.json")
     if a.scale == 24 and os.path.exists(golden):
         with open(golden) as f:
@@ -326,11 +447,17 @@ def main():
     ms = per_kernel[0][1] + per_kernel[1][1]
     launches = per_kernel[0][2] + per_kernel[1][2]
     achieved = byts / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    traffic = measured_traffic("level_kernel")
+    avg_s = ms / max(launches, 1) / 1e3
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("level_kernel"),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "basis": "achieved and frac: algorithmic bytes per launch / average launch time; measured_*: the "
+                     "committed PMC traffic per launch / the same time",
             "kernel": "level_kernel",
-            "launches": launches, "avg_launch_us": round(ms / max(launches, 1) * 1e3, 2),
-            "alg_bytes_per_launch": round(byts / max(launches, 1))}
+            "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
+            "alg_bytes_per_launch": round(byts / max(launches, 1)),
+            "measured_achieved": round(traffic / avg_s / 1e9, 1) if traffic and avg_s > 0 else None,
+            "measured_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic and avg_s > 0 else None}
     for k, name in {0: "level_kernel, push levels", 1: "level_kernel, pull levels"}.items():
         if per_kernel[k][2]:
             b_, m_, n_ = per_kernel[k]
@@ -350,9 +477,28 @@ def main():
         extra["bfs_64_sources_gteps"] = bp
         log(f"64 sources: batched {bp['batched']:.1f}, plain push {bp['plain_push']:.1f}, "
             f"hybrid {bp['hybrid']:.1f} GTEPS")
-        e2, w2 = run_sssp(sg, G, sources)
+        model = {}
+        e2, w2 = run_sssp(sg, G, sources, model=model)
         extra["gteps_sssp"] = round(e2 / w2 / 1e9, 3)
         log(f"sssp: {e2 / w2 / 1e9:.2f} GTEPS wall (16 sources, u32 weights U[1,255] seed 3, automatic delta)")
+        s_ach = model["bytes"] / (model["ms"] / 1e3) / 1e9 if model["ms"] > 0 else 0.0
+        s_traffic = measured_traffic("relax_kernel", "sssp_rmat24_traffic.json")
+        extra["roofline_sssp"] = {
+            "bound": "hbm", "achieved": round(s_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(s_ach / HBM_PEAK_GBS, 4), "alg_bytes_per_run": round(model["bytes"] / len(sources)),
+            "device_ms_per_run": round(model["ms"] / len(sources), 3),
+            "basis": "algorithmic bytes of a whole run (per-round counters, 12 B per relaxed arc) / the run's "
+                     "device time (every launch of the run, selection and queue writes included)",
+            "relax_kernel_traffic_per_launch": s_traffic}
+        log(f"sssp roofline: {s_ach:.0f} GB/s algorithmic over the whole run ({s_ach / HBM_PEAK_GBS:.3f} of peak)")
+        by_lb = {}
+        for pname, p in (("push", sg.POLICY_PUSH), ("pull", sg.POLICY_PULL)):
+            for lname, lbv in (("bucketing", sg.LB_BUCKETING), ("workgroup_mapped", sg.LB_WORKGROUP),
+                               ("subgroup_mapped", sg.LB_SUBGROUP)):
+                e3, w3 = run_policy_lb(sg, G, sources, p, lbv)
+                by_lb[f"{pname}_{lname}"] = round(e3 / w3 / 1e9, 3)
+        extra["gteps_by_load_balancer"] = by_lb
+        log(f"by load balancer (16 sources, one step each): {by_lb}")
         sweep = {}
         for d in [-1.0, 16.0, 32.0, 64.0, 128.0, 256.0]:
             e3, w3 = run_sssp(sg, G, sources[:4], d)
@@ -392,6 +538,7 @@ def main():
         "config": {"workload": f"BFS {a.policy} (alpha=15, beta=24) from 16 sources, RMAT scale {a.scale} edge factor "
                                f"16 a/b/c/d=.57/.19/.19/.05 seed 2, int64 offsets, int32 ids",
                    "vertices": G.n, "arcs": G.m, "sources": len(sources), "policy": a.policy,
+                   "graph_preprocessing_ms": round(prep_ms, 3),
                    "gteps_device_time": round(gteps_kernel, 3) if gteps_kernel else None,
                    # per traversal, device time (Graph500 style: harmonic mean over the sources)
                    "gteps_per_source": {
