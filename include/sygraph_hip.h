@@ -272,7 +272,11 @@ typedef struct sg_bfs_options {
                               many run back to back inside one cooperative launch of the whole grid,
                               one grid barrier per level, once two steps in a row have left such a
                               level (default 262144, at most 2^24; negative: never) */
-  int32_t grid_burst_wgs_per_cu; /* workgroups per compute unit of that launch (default 1, at most 4) */
+  int32_t grid_burst_workgroups; /* 512-thread workgroups of that launch (default: one per compute unit;
+                                     at most 1024, and never more than fit on the device at once) */
+  int32_t learned_gate;    /* SG_POLICY_LEARNED: a level whose frontier has fewer arcs than the graph's
+                              arcs / learned_gate pushes with the bucketing mapping without asking the
+                              network (default 1024; negative: the network decides every level) */
 } sg_bfs_options;
 
 typedef struct sg_level_stat {
@@ -345,6 +349,9 @@ typedef struct sg_sssp_stats {
   int64_t round_vertices[SG_MAX_LEVELS_RECORDED]; /* |F| per round */
   int64_t round_edges[SG_MAX_LEVELS_RECORDED];    /* arcs relaxed per round: sum deg over F (Bellman-Ford),
                                                      or over F of the light / heavy arcs (buckets) */
+  float relax_ms;            /* sg_sssp_options.record_relax: device time of the relax launches of the
+                                relaxing rounds (hipEvents around each), else 0 */
+  int32_t relax_launches;    /* those launches */
 } sg_sssp_stats;
 /* SSSP::run (algorithms/sssp.hpp:152-210): frontier Bellman-Ford with atomic
  * min; dist (device, 4-byte, same type as the weights) unreached = n+1.
@@ -367,6 +374,8 @@ typedef struct sg_sssp_options {
   int32_t burst_arcs;    /* rounds of the reference's schedule (no buckets) with at most this many
                             arcs run back to back inside one workgroup while they stay that small:
                             0 = default (4,096), negative = never, at most 8,192 */
+  int32_t record_relax;  /* 1: time every relax launch on its own (two events per round; stats must
+                            not be NULL): stats->relax_ms / relax_launches */
 } sg_sssp_options;
 void sg_sssp_options_default(sg_sssp_options* opt);
 /* sg_sssp with options; the distances do not depend on delta. */

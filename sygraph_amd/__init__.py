@@ -69,7 +69,7 @@ class BfsOptions(C.Structure):
     _fields_ = [("policy", C.c_int32), ("load_balancer", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float),
                 ("record_levels", C.c_int32), ("sync_per_level", C.c_int32), ("learned", C.c_void_p),
                 ("t_medium", C.c_int32), ("t_large", C.c_int32), ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("burst_arcs", C.c_int32), ("grid_burst_arcs", C.c_int32), ("grid_burst_wgs_per_cu", C.c_int32)]
+                ("burst_arcs", C.c_int32), ("grid_burst_arcs", C.c_int32), ("grid_burst_workgroups", C.c_int32), ("learned_gate", C.c_int32)]
 
 
 class MsbfsOptions(C.Structure):
@@ -87,12 +87,13 @@ class LevelStat(C.Structure):
 class SsspStats(C.Structure):
     _fields_ = [("rounds", C.c_int32), ("reached_vertices", C.c_int64), ("reached_edges", C.c_int64),
                 ("relaxations", C.c_int64), ("kernel_ms", C.c_float),
-                ("round_vertices", C.c_int64 * MAX_LEVELS_RECORDED), ("round_edges", C.c_int64 * MAX_LEVELS_RECORDED)]
+                ("round_vertices", C.c_int64 * MAX_LEVELS_RECORDED), ("round_edges", C.c_int64 * MAX_LEVELS_RECORDED),
+                ("relax_ms", C.c_float), ("relax_launches", C.c_int32)]
 
 
 class SsspOptions(C.Structure):
     _fields_ = [("load_balancer", C.c_int32), ("delta", C.c_double), ("heavy_pull", C.c_int32),
-                ("burst_arcs", C.c_int32)]
+                ("burst_arcs", C.c_int32), ("record_relax", C.c_int32)]
 
 
 class BfsStats(C.Structure):
@@ -564,7 +565,7 @@ class BFS:
             load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
             learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsStats:
         """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu, burst_arcs, grid_burst_arcs,
-        grid_burst_wgs_per_cu."""
+        grid_burst_workgroups, learned_gate."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
         o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,
@@ -604,13 +605,14 @@ class SSSP:
         self.source = None
 
     def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0,
-            burst_arcs: int = 0) -> SsspStats:
+            burst_arcs: int = 0, record_relax: bool = False) -> SsspStats:
         """delta: bucket width of the near/far split (0 automatic: none below
         2^25 arcs, half the mean arc weight above; negative or inf for the
         reference's frontier Bellman-Ford).  heavy_pull: direction of a
         bucket's heavy round (0 automatic, 1 pull, -1 push).  burst_arcs:
         small rounds of the reference's schedule run inside one workgroup (0
-        default 4096, negative never)."""
+        default 4096, negative never).  record_relax: time each relax launch on its own
+        (stats.relax_ms, stats.relax_launches)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "SSSP instance not initialized")
         o = SsspOptions()
@@ -619,6 +621,7 @@ class SSSP:
         o.delta = float(delta)
         o.heavy_pull = int(heavy_pull)
         o.burst_arcs = int(burst_arcs)
+        o.record_relax = 1 if record_relax else 0
         check(lib().sg_sssp_ex(self.g.h, self.source, C.byref(o), C.c_void_p(self.dist.ptr), C.byref(self.stats)))
         return self.stats
 

@@ -47,7 +47,9 @@ struct BfsArgs {
   const int32_t* cols;
   const OffT* in_off; /* inverse CSR (pull); aliases off when undirected */
   const int32_t* in_cols;
-  const int2* hint; /* per vertex: {in-neighbour of largest out-degree, out-degree} */
+  const int2* hint; /* per vertex: {in-neighbour of largest out-degree, out-degree}, or with hint_offsets
+                       {that in-neighbour, low 32 bits of the vertex's list start}, entry n = {-1, m} */
+  int32_t hint_offsets;
   int32_t directed;
   int64_t n;
   int64_t w0;
@@ -133,13 +135,25 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, 
          * vertex's out-degree from one 8-byte entry; the offsets and the
          * adjacency list are read only on a miss */
         const int2 hd = a.hint[v];
-        odeg = hd.y;
+        uint32_t rb = 0, re = 0;
+        if (a.hint_offsets) { /* the list's bounds come with the hint, usually from the same line */
+          rb = (uint32_t)hd.y;
+          re = (uint32_t)a.hint[v + 1].y;
+          odeg = (int32_t)(re - rb);
+        } else {
+          odeg = hd.y;
+        }
         if (test_bit(a.cur_bm, (uint32_t)hd.x)) {
           found = true;
         } else {
           miss = true;
-          b = a.in_off[v];
-          e = a.in_off[v + 1];
+          if (a.hint_offsets) {
+            b = (OffT)rb;
+            e = (OffT)re;
+          } else {
+            b = a.in_off[v];
+            e = a.in_off[v + 1];
+          }
           j = b;
           kk = (int64_t)(e - b);
         }
@@ -256,7 +270,19 @@ struct DecideParams {
   const float* mlp;     /* learned policy parameters (SG_POLICY_LEARNED) */
   float f6, f7;         /* the graph's constant features (average degree, vertex count) */
   int64_t burst_arcs;   /* frontier arcs up to which push levels run in a burst (0: never) */
+  unsigned long long learned_gate; /* learned policy: a level whose frontier has fewer arcs than the
+                                      graph's arcs / learned_gate pushes (variant 0) without asking the
+                                      network; 0: the network decides every level */
 };
+
+/* Default of sg_bfs_options.learned_gate.  A frontier with fewer arcs than
+ * 1/1024 of the graph's arcs is cheaper to push than any pull, which scans
+ * every word of the visited bitmap and the lists of the vertices not yet
+ * reached (the reference's rule too pushes such frontiers until the arcs left
+ * unexplored fall below 15 times the frontier's); on long traversals such
+ * levels are nearly all the levels, and inferring for them costs ~1.3 us
+ * each, 5-10 % of a grid level. */
+constexpr int64_t kLearnedGateDefault = 1024;
 
 /* evaluateHeuristic (algorithms/bfs.hpp:341-350): push -> pull when the new
  * frontier's arcs exceed the unexplored arcs / alpha (both cast to uint32 and
@@ -278,9 +304,11 @@ __device__ __forceinline__ int32_t decide_action(const DecideParams& p, const sg
                                                  unsigned long long disc, unsigned long long disc_deg,
                                                  unsigned long long unexplored, int32_t next_level,
                                                  unsigned long long max_deg) {
-  if (p.policy == SG_POLICY_LEARNED)
+  if (p.policy == SG_POLICY_LEARNED) {
+    if (p.learned_gate > 0 && disc_deg * p.learned_gate < p.total_degree) return 0;
     return sg::policy::infer_staged(pol, (double)disc, (double)disc_deg, (double)unexplored, next_level,
                                     (double)max_deg, p.f6, p.f7);
+  }
   return next_direction(p, dir, disc, disc_deg, unexplored) * 3 + p.lb_fixed;
 }
 
@@ -950,7 +978,12 @@ __device__ __forceinline__ void gb_visit_small(const GbArgs<OffT>& a, const QEnt
   const int lane = lane_id();
   for (int64_t base = gwave * 64; base < cnt; base += nwaves * 64) {
     QEntry qe{0, 0, 0};
-    if (base + lane < cnt) qe = coherent ? load_entry_coherent(ents + base + lane) : ents[base + lane];
+    if (base + lane < cnt) {
+      qe = coherent ? load_entry_coherent(ents + base + lane) : ents[base + lane];
+      /* a list entry of an earlier level of the burst: its vertex joins prev
+       * (fire and forget; nothing in the burst reads prev) */
+      if (coherent) atomic_or(&a.prev[(uint32_t)qe.v >> 6], 1ull << (qe.v & 63));
+    }
     const uint32_t incl = wave_inclusive_scan((uint32_t)qe.len);
     const uint32_t total = __shfl(incl, 63, 64);
     const uint32_t excl = incl - (uint32_t)qe.len;
@@ -980,6 +1013,8 @@ __device__ __forceinline__ void gb_visit_pieces(const GbArgs<OffT>& a, const QEn
   const int lane = lane_id();
   for (int64_t i = gwave; i < cnt; i += nwaves) {
     const QEntry qe = coherent ? load_entry_coherent(ents + i) : ents[i];
+    if (coherent && lane == 0 && qe.start == (int64_t)a.off[qe.v]) /* the vertex's first piece */
+      atomic_or(&a.prev[(uint32_t)qe.v >> 6], 1ull << (qe.v & 63));
     for (int64_t o0 = 0; o0 < qe.len; o0 += 4 * 64) {
       uint32_t y[4];
       bool ok[4];
@@ -999,7 +1034,7 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
                                                             LevelLog* levlog, StepLog* steplog) {
   __shared__ sg::policy::Staged s_pol;
   __shared__ unsigned long long s_red[kGbWaves][3];
-  __shared__ unsigned long long s_tot[3];
+  __shared__ unsigned long long s_tot[5];
   if (ctl->done || ctl->skip || ctl->dir != 0 || ctl->f_edges > a.gb_arcs) return; /* the same for every workgroup */
   stage_policy(p, &s_pol);
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -1030,12 +1065,7 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
       gb_visit_pieces(a, a.q[1], h0.count[1], level, out_small, out_pieces, sm, acc, gwave, nwaves, false);
       gb_visit_pieces(a, a.q[2], h0.count[2], level, out_small, out_pieces, sm, acc, gwave, nwaves, false);
     } else {
-      /* the previous level's discoveries join prev (nothing here reads prev) */
-      for (int64_t i = gwave * 64 + lane; i < n_small + n_pieces; i += nwaves * 64) {
-        const uint32_t v = (uint32_t)(i < n_small ? load_entry_coherent(a.small[cur] + i)
-                                                  : load_entry_coherent(a.pieces[cur] + i - n_small)).v;
-        atomic_or(&a.prev[v >> 6], 1ull << (v & 63));
-      }
+      /* the previous level's discoveries join prev as their entries are read */
       gb_visit_small(a, a.small[cur], n_small, level, out_small, out_pieces, sm, acc, gwave, nwaves, true);
       gb_visit_pieces(a, a.pieces[cur], n_pieces, level, out_small, out_pieces, sm, acc, gwave, nwaves, true);
     }
@@ -1062,8 +1092,9 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
     }
     /* no fences: the lists, sums and slots are written and read coherently */
     if (!grid_barrier<false>(a.bar, bt, a.wait_ticks, 0u)) break; /* timed out: reported through the bar flags */
-    if (threadIdx.x < 64) { /* wave 0 totals the slots of every workgroup */
-      unsigned long long d = 0, g = 0, x = 0;
+    if (threadIdx.x < 64) { /* wave 0 totals the slots of every workgroup and reads the list sizes */
+      unsigned long long d = 0, g = 0, x = 0, lists = 0;
+      if (threadIdx.x < 2) lists = atomic_load(threadIdx.x == 0 ? &sm->nsmall : &sm->npieces);
       for (uint32_t i = threadIdx.x; i < gridDim.x; i += 64) {
         d += atomic_load(&slots[i].disc);
         g += atomic_load(&slots[i].deg);
@@ -1077,11 +1108,13 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
         s_tot[0] = d;
         s_tot[1] = g;
         s_tot[2] = x;
+        s_tot[3] = lists;
       }
+      if (threadIdx.x == 1) s_tot[4] = lists;
     }
     __syncthreads();
     const unsigned long long disc = s_tot[0], disc_deg = s_tot[1], max_deg = s_tot[2];
-    const int64_t ns = (int64_t)atomic_load(&sm->nsmall), np = (int64_t)atomic_load(&sm->npieces);
+    const int64_t ns = (int64_t)s_tot[3], np = (int64_t)s_tot[4];
     const unsigned long long unexplored = p.total_degree - (vdeg + disc_deg);
     act = decide_action(p, &s_pol, 0, disc, disc_deg, unexplored, level + 1, max_deg);
     done = disc == 0 ? 1 : 0;
@@ -1357,6 +1390,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   a.in_off = (const OffT*)g->inv_offsets;
   a.in_cols = g->inv_cols;
   a.hint = g->pull_hint;
+  a.hint_offsets = g->pull_hint_offsets ? 1 : 0;
   a.directed = (g->flags & SG_GRAPH_DIRECTED) ? 1 : 0;
   a.n = n;
   a.w0 = w0;
@@ -1381,7 +1415,8 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gburst_kernel<OffT>, kGbThreads, 0));
       occ = std::max(1, nb);
     }
-    gb_grid = dev->cus * std::min(occ, std::max(1, opt.grid_burst_wgs_per_cu > 0 ? opt.grid_burst_wgs_per_cu : 1));
+    /* at most what is resident at once (the barrier needs every workgroup) */
+    gb_grid = std::min(opt.grid_burst_workgroups > 0 ? opt.grid_burst_workgroups : dev->cus, dev->cus * occ);
     gb_grid = std::min(gb_grid, kGbMaxWgs);
   }
   init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
@@ -1403,6 +1438,8 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   P.f6 = sg::policy::feature(6, 0, 0, 0, 0, 0, (double)n, (double)g->m);
   P.f7 = sg::policy::feature(7, 0, 0, 0, 0, 0, (double)n, (double)g->m);
   P.burst_arcs = burst_arcs;
+  P.learned_gate = opt.learned_gate < 0 ? 0ull
+                                        : (unsigned long long)(opt.learned_gate == 0 ? kLearnedGateDefault : opt.learned_gate);
   if (want_total) SG_HIP(hipEventRecord(ws->ev_begin, s));
 
   /* the burst of a step reads the queue set its count pass wrote */
@@ -1589,9 +1626,10 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
            opt.t_medium, opt.t_large, opt.chunk, opt.blocks_per_cu);
   SG_CHECK(opt.burst_arcs <= kBurstMaxArcs, SG_ERR_INVALID, "sg_bfs: burst_arcs %d above %d", opt.burst_arcs,
            kBurstMaxArcs);
-  SG_CHECK(opt.grid_burst_arcs <= kGbMaxArcs && opt.grid_burst_wgs_per_cu >= 0 && opt.grid_burst_wgs_per_cu <= 4,
-           SG_ERR_INVALID, "sg_bfs: grid_burst_arcs %d above %lld or grid_burst_wgs_per_cu %d outside [0, 4]",
-           opt.grid_burst_arcs, (long long)kGbMaxArcs, opt.grid_burst_wgs_per_cu);
+  SG_CHECK(opt.grid_burst_arcs <= kGbMaxArcs && opt.grid_burst_workgroups >= 0 &&
+               opt.grid_burst_workgroups <= kGbMaxWgs,
+           SG_ERR_INVALID, "sg_bfs: grid_burst_arcs %d above %lld or grid_burst_workgroups %d outside [0, %d]",
+           opt.grid_burst_arcs, (long long)kGbMaxArcs, opt.grid_burst_workgroups, kGbMaxWgs);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
   if (stats) {

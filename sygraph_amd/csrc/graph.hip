@@ -252,7 +252,7 @@ __global__ void isolated_kernel(const OffT* offsets, int64_t n, int64_t w0, uint
  * (degree << 32) | ~id, so the maximum key is the answer. */
 template<typename OffT>
 __global__ void __launch_bounds__(256) pull_hint_kernel(const OffT* in_off, const int32_t* in_cols, const OffT* out_off,
-                                                        int64_t n, int2* hint) {
+                                                        int64_t n, int2* hint, int32_t with_offsets) {
   const int lane = lane_id();
   const int64_t gwave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(256) pull_hint_kernel(const OffT* in_off, cons
     }
     if (v < n)
       hint[v] = make_int2(e > b ? (int32_t)(0xFFFFFFFFu - (uint32_t)(best & 0xFFFFFFFFull)) : -1,
-                          (int32_t)(out_off[v + 1] - out_off[v]));
+                          with_offsets ? (int32_t)(uint32_t)b : (int32_t)(out_off[v + 1] - out_off[v]));
+    if (with_offsets && v == n - 1) hint[n] = make_int2(-1, (int32_t)(uint32_t)e); /* the end of the last list */
   }
 }
 
@@ -496,12 +497,17 @@ int graph_ensure_pull_hint(sg_graph_t g) {
   const int grid = grid_for(g->dev, (g->n + 3) / 4, 256);
   const void* in_off = g->inv_offsets ? g->inv_offsets : g->offsets;
   const int32_t* in_cols = g->inv_cols ? g->inv_cols : g->cols;
+  /* an undirected graph with fewer than 2^32 arcs keeps the low 32 bits of
+   * each in-list's start next to its hint (in- and out-lists are the same
+   * lists, so the out-degree is the difference of two starts) */
+  g->pull_hint_offsets = !(g->flags & SG_GRAPH_DIRECTED) && g->m < ((int64_t)1 << 32) && g->n > 0;
+  const int32_t wo = g->pull_hint_offsets ? 1 : 0;
   if (g->offset_bytes == 8)
     pull_hint_kernel<int64_t><<<grid, 256, 0, g->dev->stream>>>((const int64_t*)in_off, in_cols,
-                                                                (const int64_t*)g->offsets, g->n, g->pull_hint);
+                                                                (const int64_t*)g->offsets, g->n, g->pull_hint, wo);
   else
     pull_hint_kernel<uint32_t><<<grid, 256, 0, g->dev->stream>>>((const uint32_t*)in_off, in_cols,
-                                                                 (const uint32_t*)g->offsets, g->n, g->pull_hint);
+                                                                 (const uint32_t*)g->offsets, g->n, g->pull_hint, wo);
   SG_HIP(hipGetLastError());
   return SG_OK;
 }

@@ -117,7 +117,9 @@ struct sg_graph_s {
   /* lazily built helpers */
   uint64_t* isolated = nullptr; /* bit set for vertices with in-degree 0 */
   int2* pull_hint = nullptr;     /* per vertex: {in-neighbour of largest out-degree (pull probes it first),
-                                    out-degree of the vertex} */
+                                    out-degree of the vertex}, or with pull_hint_offsets {that in-neighbour,
+                                    low 32 bits of the start of the vertex's list} and entry n = {-1, m} */
+  bool pull_hint_offsets = false;
   double mean_weight = -1.0;     /* mean arc weight, computed on first use (automatic SSSP delta) */
   /* SSSP's light-first copy of the adjacency for one bucket width: per vertex
    * the arcs lighter than the width first, then the others */

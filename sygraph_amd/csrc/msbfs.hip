@@ -59,6 +59,39 @@ using namespace sg::trav;
 
 namespace {
 
+/* Data one workgroup writes and another reads later in the same launch (the
+ * masks, level lists, chunk records, path-count and dependency rows, partial
+ * rows, bc sums) goes to and comes from the point of coherence that every
+ * XCD shares -- agent-scope relaxed atomic loads and stores -- so the grid
+ * barrier between phases neither writes back nor invalidates any L2 (with
+ * fences, a 4,096-level batch on a grid spent ~100 us per barrier). */
+template<typename T>
+__device__ __forceinline__ T cld(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template<typename T, typename U>
+__device__ __forceinline__ void cst(T* p, U v) {
+  __hip_atomic_store(p, static_cast<T>(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template<typename S>
+__device__ __forceinline__ void cst16(S* p, const S& v) {
+  static_assert(sizeof(S) == 16, "16-byte records");
+  uint64_t w[2];
+  __builtin_memcpy(w, &v, 16);
+  cst(reinterpret_cast<uint64_t*>(p), w[0]);
+  cst(reinterpret_cast<uint64_t*>(p) + 1, w[1]);
+}
+template<typename S>
+__device__ __forceinline__ S cld16(const S* p) {
+  static_assert(sizeof(S) == 16, "16-byte records");
+  uint64_t w[2];
+  w[0] = cld(reinterpret_cast<const uint64_t*>(p));
+  w[1] = cld(reinterpret_cast<const uint64_t*>(p) + 1);
+  S v;
+  __builtin_memcpy(&v, w, 16);
+  return v;
+}
+
 constexpr int kHub = 256;     /* adjacency lists longer than this are chunked */
 constexpr int kLiRing = 8;    /* level records of the MS-BFS ring */
 constexpr int kBarGroups = 8; /* arrival counters, one per XCD */
@@ -92,11 +125,13 @@ struct Ctl {
   uint32_t grp[kBarGroups * kBarStride];
   uint32_t top[kBarStride];
   uint32_t gen[kBarStride];
-  uint32_t flags;
+  uint32_t flags;                /* polled by every waiting workgroup */
   int32_t levels;                /* levels of the batch (depths 0 .. levels-1) */
-  unsigned long long full_deg;   /* out-degrees of the vertices every lane has reached */
   int32_t n_pull;                /* levels expanded by pulling */
-  uint32_t pad[25];
+  uint32_t pad0[kBarStride - 3];
+  /* on a line of its own: atomics on the polled flags line queue behind the polls */
+  unsigned long long full_deg;   /* out-degrees of the vertices every lane has reached */
+  uint32_t pad1[kBarStride - 2];
 };
 static_assert(sizeof(Ctl) % 8 == 0, "Ctl is cleared in 4-byte words");
 
@@ -150,14 +185,17 @@ struct MsArgs {
 __device__ __forceinline__ uint32_t load_flags(const Ctl* c) { return atomic_load(&c->flags); }
 __device__ __forceinline__ void raise_flag(Ctl* c, uint32_t f) { atomic_or(&c->flags, f); }
 
-/* Grid barrier over a cooperative launch.  Arrival counters only grow (barrier
- * t completes at t arrivals per member), so nothing is reset between
- * barriers.  The wait is bounded: past wait_ticks of the constant-rate clock
- * the kTimeout flag is raised and every workgroup leaves.  Returns false when
- * the run must stop (a flag is set). */
+/* Grid barrier over a launch whose workgroups are all resident.  Arrival
+ * counters only grow (barrier t completes at t arrivals per member), so
+ * nothing is reset between barriers.  No fences: everything the workgroups
+ * exchange goes through agent-coherent loads and stores (cld / cst and the
+ * atomics), so a wave only waits for its own stores to complete before its
+ * workgroup arrives.  The wait is bounded: past wait_ticks of the
+ * constant-rate clock the kTimeout flag is raised and every workgroup leaves.
+ * Returns false when the run must stop (a flag is set). */
 __device__ __noinline__ bool grid_sync(Ctl* c, uint32_t& t, uint64_t wait_ticks) {
   __shared__ uint32_t s_stop;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  wait_stores_complete();
   __syncthreads();
   ++t;
   if (threadIdx.x == 0) {
@@ -166,28 +204,26 @@ __device__ __noinline__ bool grid_sync(Ctl* c, uint32_t& t, uint64_t wait_ticks)
     const uint32_t in_group = (grid - g + kBarGroups - 1) / kBarGroups;
     bool released = false;
     if (atomic_add(&c->grp[g * kBarStride], 1u) == t * in_group - 1) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       if (atomic_add(&c->top[0], 1u) == t * ngroups - 1) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-        __hip_atomic_store(&c->gen[0], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        atomic_store(&c->gen[0], t);
         released = true;
       }
     }
     if (!released) {
       const uint64_t t0 = wall_clock64();
-      while (__hip_atomic_load(&c->gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < t) {
+      while (atomic_load(&c->gen[0]) < t) {
         if (load_flags(c) & kStop) break;
         if ((uint64_t)(wall_clock64() - t0) > wait_ticks) {
           raise_flag(c, kTimeout);
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
       }
     }
     s_stop = (load_flags(c) & kStop) ? 1u : 0u;
   }
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_stop == 0;
 }
 
@@ -248,7 +284,7 @@ __device__ __forceinline__ void place(const MsArgs<OffT>& a, LevelInfo* nx, cons
   const int64_t cap_hc = a.ring ? a.hc_cap : a.hc_cap - b.hoc;
   int64_t deg = 0;
   if (ok) {
-    if ((int64_t)rel < cap_l) a.lv[b.l + (int64_t)rel] = v;
+    if ((int64_t)rel < cap_l) cst(&a.lv[b.l + (int64_t)rel], v);
     else raise_flag(a.ctl, kOvList);
     deg = (int64_t)(a.off[v + 1] - a.off[v]);
   }
@@ -258,8 +294,8 @@ __device__ __forceinline__ void place(const MsArgs<OffT>& a, LevelInfo* nx, cons
     const uint64_t hc = wave_reserve(&nx->hoc, nch);
     if (nch) {
       if ((int64_t)hv < cap_hv && (int64_t)(hc + nch) <= cap_hc) {
-        a.hov[b.hov + (int64_t)hv] = HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel};
-        for (uint32_t c = 0; c < nch; ++c) a.hoc[b.hoc + (int64_t)hc + c] = HubC{v, (int32_t)c, (int32_t)rel, 0};
+        cst16(&a.hov[b.hov + (int64_t)hv], HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel});
+        for (uint32_t c = 0; c < nch; ++c) cst16(&a.hoc[b.hoc + (int64_t)hc + c], HubC{v, (int32_t)c, (int32_t)rel, 0});
       } else {
         raise_flag(a.ctl, (int64_t)hv < cap_hv ? kOvHubC : kOvHubV);
       }
@@ -274,8 +310,8 @@ __device__ __forceinline__ void place(const MsArgs<OffT>& a, LevelInfo* nx, cons
       const uint64_t hc = wave_reserve(&nx->hic, nch);
       if (nch) {
         if ((int64_t)hv < cap_iv && (int64_t)(hc + nch) <= cap_ic) {
-          a.hiv[b.hiv + (int64_t)hv] = HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel};
-          for (uint32_t c = 0; c < nch; ++c) a.hic[b.hic + (int64_t)hc + c] = HubC{v, (int32_t)c, (int32_t)rel, 0};
+          cst16(&a.hiv[b.hiv + (int64_t)hv], HubV{v, (int32_t)nch, (int32_t)hc, (int32_t)rel});
+          for (uint32_t c = 0; c < nch; ++c) cst16(&a.hic[b.hic + (int64_t)hc + c], HubC{v, (int32_t)c, (int32_t)rel, 0});
         } else {
           raise_flag(a.ctl, (int64_t)hv < cap_iv ? kOvHubC : kOvHubV);
         }
@@ -376,9 +412,9 @@ __device__ __forceinline__ void stage_finish(const MsArgs<OffT>& a, LevelInfo* n
 /* New lanes for v from a frontier mask fm: OR them into M_{d+1}[v]; true when
  * this call was the first to give v any lane at depth d+1. */
 __device__ __forceinline__ bool relax(uint64_t fm, int32_t v, const uint64_t* vis, const uint64_t* Mc, uint64_t* Mn) {
-  const uint64_t add = fm & ~(vis[v] | Mc[v]);
+  const uint64_t add = fm & ~(cld(&vis[v]) | cld(&Mc[v]));
   if (!add) return false;
-  if ((Mn[v] & add) == add) return false;
+  if ((cld(&Mn[v]) & add) == add) return false;
   return atomic_or(&Mn[v], add) == 0ull;
 }
 
@@ -449,8 +485,8 @@ __device__ __forceinline__ void store_norm(const MsArgs<OffT>& a, int64_t at, fl
   }
   const int ex = S > 0.f ? E + k : 0;
   if (ex > kExpMax || ex < -kExpMax) raise_flag(a.ctl, kRange);
-  a.sm[at] = m;
-  a.se[at] = (int16_t)(ex > kExpMax ? kExpMax : (ex < -kExpMax ? -kExpMax : ex));
+  cst(&a.sm[at], m);
+  cst(&a.se[at], (int16_t)(ex > kExpMax ? kExpMax : (ex < -kExpMax ? -kExpMax : ex)));
 }
 
 /* ------------------------------------------------------------ level phase --- */
@@ -467,12 +503,12 @@ __device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, in
     int64_t st = 0;
     uint32_t len = 0;
     if (i < cnt) {
-      const int32_t u = a.lv[base + i];
+      const int32_t u = cld(&a.lv[base + i]);
       st = (int64_t)a.off[u];
       const int64_t dg = (int64_t)a.off[u + 1] - st;
       if (dg <= kHub) {
         len = (uint32_t)dg;
-        fm = Mc[u];
+        fm = cld(&Mc[u]);
       }
     }
     const uint32_t incl = wave_inclusive_scan(len);
@@ -506,8 +542,8 @@ __device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC*
                                               int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t c = gwave; c < nc; c += nwaves) {
-    const HubC h = hc[c];
-    const uint64_t fm = Mc[h.v];
+    const HubC h = cld16(&hc[c]);
+    const uint64_t fm = cld(&Mc[h.v]);
     const int64_t s = (int64_t)a.off[h.v] + (int64_t)h.c * kHub;
     const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.off[h.v + 1]);
     for (int64_t j0 = s; j0 < e; j0 += 4 * 64) {
@@ -538,7 +574,7 @@ __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_
     uint64_t want = 0, acc = 0;
     int64_t b = 0, e = 0;
     if (v < a.n) {
-      want = a.full & ~(a.vis[v] | Mc[v]);
+      want = a.full & ~(cld(&a.vis[v]) | cld(&Mc[v]));
       if (want) {
         b = (int64_t)a.ioff[v];
         e = (int64_t)a.ioff[v + 1];
@@ -547,7 +583,7 @@ __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_
     const bool coop = e - b > 64;
     if (!coop)
       for (int64_t j = b; j < e; ++j) {
-        acc |= Mc[a.icols[j]];
+        acc |= cld(&Mc[a.icols[j]]);
         if ((acc & want) == want) break;
       }
     uint64_t heavy = ballot(coop);
@@ -558,7 +594,7 @@ __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_
       const uint64_t kwant = shfl_u64(want, k);
       uint64_t part = 0;
       for (int64_t j0 = kb; j0 < ke; j0 += 64) {
-        if (j0 + lane < ke) part |= Mc[a.icols[j0 + lane]];
+        if (j0 + lane < ke) part |= cld(&Mc[a.icols[j0 + lane]]);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) part |= shfl_u64(part, lane ^ o);
         if ((part & kwant) == kwant) break;
@@ -567,7 +603,7 @@ __device__ __forceinline__ void expand_pull(const MsArgs<OffT>& a, const uint64_
     }
     const uint64_t add = acc & want;
     const bool first = add != 0;
-    if (first) Mn[v] = add;
+    if (first) cst(&Mn[v], add);
     stage_add(a, nx, nb, st, first, (int32_t)v);
   }
 }
@@ -583,11 +619,11 @@ __device__ __forceinline__ void visit_level(const MsArgs<OffT>& a, int32_t d, in
     uint64_t m = 0;
     uint64_t fdeg = 0;
     if (i < cnt) {
-      u = a.lv[base + i];
-      m = Mc[u];
-      const uint64_t old = a.vis[u];
-      a.vis[u] = old | m;
-      if (a.bc) a.lm[base + i] = m;
+      u = cld(&a.lv[base + i]);
+      m = cld(&Mc[u]);
+      const uint64_t old = cld(&a.vis[u]);
+      cst(&a.vis[u], old | m);
+      if (a.bc) cst(&a.lm[base + i], m);
       if (a.dir_policy >= SG_POLICY_HYBRID && old != a.full && (old | m) == a.full)
         fdeg = (uint64_t)(a.off[u + 1] - a.off[u]);
     }
@@ -613,7 +649,38 @@ template<typename OffT>
 __device__ __forceinline__ void clear_level(const MsArgs<OffT>& a, int64_t base, int64_t cnt, uint64_t* Mold,
                                             int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
-  for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) Mold[a.lv[base + i]] = 0ull;
+  for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) cst(&Mold[cld(&a.lv[base + i])], 0ull);
+}
+
+/* One list [vb, ve) walked by the whole wave, lane = source l: the wave
+ * loads the ids and neighbour masks of 64 arcs at once (lane k: arc k), then
+ * takes the arcs in list order, eight at a time, so each lane issues the
+ * row loads of eight arcs before it adds any (the walk is latency-bound:
+ * four arcs in flight left it three times slower).  hit[u]: arc u's
+ * neighbour has bit l in its mask and this lane takes part (ok).  All lanes
+ * call; `term(x, hit)` loads and adds the eight arcs' terms in order. */
+template<typename Term>
+__device__ __forceinline__ void wave_list_walk(const int32_t* ids, int64_t vb, int64_t ve, const uint64_t* masks,
+                                               int l, bool ok, Term&& term) {
+  const int lane = lane_id();
+  for (int64_t j0 = vb; j0 < ve; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const int32_t xk = j < ve ? ids[j] : -1;
+    const uint64_t mk = xk >= 0 ? cld(&masks[xk]) : 0ull;
+    const int cnt = (int)(ve - j0 < 64 ? ve - j0 : 64);
+    for (int t0 = 0; t0 < cnt; t0 += 8) {
+      int32_t x[8];
+      bool hit[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int t = t0 + u;
+        x[u] = __shfl(xk, t, 64);
+        const uint64_t m = shfl_u64(mk, t);
+        hit[u] = ok && t < cnt && ((m >> l) & 1ull);
+      }
+      term(x, hit);
+    }
+  }
 }
 
 /* Path counts of the pairs of L_d whose in-lists are short: each pair sums
@@ -628,10 +695,10 @@ __device__ __forceinline__ void sigma_pairs(const MsArgs<OffT>& a, int64_t base,
     uint64_t own = 0;
     int64_t b = 0, e = 0;
     if (i < cnt) {
-      u = a.lv[base + i];
+      u = cld(&a.lv[base + i]);
       b = (int64_t)a.ioff[u];
       e = (int64_t)a.ioff[u + 1];
-      if (e - b <= kHub) own = Mc[u];
+      if (e - b <= kHub) own = cld(&Mc[u]);
     }
     PairWalk w = pair_walk(own, e - b);
     for (uint32_t step = 0; step < w.steps; ++step) {
@@ -639,22 +706,38 @@ __device__ __forceinline__ void sigma_pairs(const MsArgs<OffT>& a, int64_t base,
       const bool ok = pair_item(w, own, step, owner, l);
       const int32_t v = __shfl(u, owner, 64);
       const int64_t vb = shfl_i64(b, owner), ve = shfl_i64(e, owner);
-      if (!ok) continue;
       float S = 0.f;
       int E = kNoExp;
+      if (!w.flat) { /* one vertex, every lane a source */
+        wave_list_walk(a.icols, vb, ve, Mp, l, ok, [&](const int32_t (&x)[8], const bool (&hit)[8]) {
+          float sv[8];
+          int ev[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            sv[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + l]) : 0.f;
+            ev[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + l]) : 0;
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (hit[k]) xadd(S, E, sv[k], ev[k]);
+        });
+        if (ok) store_norm(a, (int64_t)v * 64 + l, S, E);
+        continue;
+      }
+      if (!ok) continue;
       for (int64_t j = vb; j < ve; j += 4) {
         int32_t x[4];
         bool hit[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = j + k < ve ? a.icols[j + k] : -1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((Mp[x[k]] >> l) & 1ull);
+        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((cld(&Mp[x[k]]) >> l) & 1ull);
         float sv[4];
         int ev[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          sv[k] = hit[k] ? a.sm[(int64_t)x[k] * 64 + l] : 0.f;
-          ev[k] = hit[k] ? (int)a.se[(int64_t)x[k] * 64 + l] : 0;
+          sv[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + l]) : 0.f;
+          ev[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + l]) : 0;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -672,27 +755,28 @@ __device__ __forceinline__ void sigma_chunks(const MsArgs<OffT>& a, const HubC* 
                                              const uint64_t* Mp, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t c = gwave; c < nc; c += nwaves) {
-    const HubC h = hc[c];
-    const uint64_t own = Mc[h.v];
+    const HubC h = cld16(&hc[c]);
+    const uint64_t own = cld(&Mc[h.v]);
     const bool mine = (own >> lane) & 1ull;
     const int64_t s = (int64_t)a.ioff[h.v] + (int64_t)h.c * kHub;
     const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.ioff[h.v + 1]);
     float S = 0.f;
     int E = kNoExp;
-    for (int64_t j = s; j < e; j += 4) {
-      int32_t x[4];
-      bool hit[4];
+    wave_list_walk(a.icols, s, e, Mp, lane, mine, [&](const int32_t (&x)[8], const bool (&hit)[8]) {
+      float sv[8];
+      int ev[8];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) x[k] = j + k < e ? a.icols[j + k] : -1;
+      for (int k = 0; k < 8; ++k) {
+        sv[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + lane]) : 0.f;
+        ev[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + lane]) : 0;
+      }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) hit[k] = mine && x[k] >= 0 && ((Mp[x[k]] >> lane) & 1ull);
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (hit[k]) xadd(S, E, a.sm[(int64_t)x[k] * 64 + lane], (int)a.se[(int64_t)x[k] * 64 + lane]);
-    }
+      for (int k = 0; k < 8; ++k)
+        if (hit[k]) xadd(S, E, sv[k], ev[k]);
+    });
     if (c < a.p_cap) {
-      a.pS[c * 64 + lane] = S;
-      a.pE[c * 64 + lane] = E;
+      cst(&a.pS[c * 64 + lane], S);
+      cst(&a.pE[c * 64 + lane], E);
     } else if (lane == 0) {
       raise_flag(a.ctl, kOvPart);
     }
@@ -705,14 +789,14 @@ __device__ __forceinline__ void sigma_combine(const MsArgs<OffT>& a, const HubV*
                                               int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t i = gwave; i < nv; i += nwaves) {
-    const HubV h = hv[i];
-    if (!((Mc[h.v] >> lane) & 1ull)) continue;
+    const HubV h = cld16(&hv[i]);
+    if (!((cld(&Mc[h.v]) >> lane) & 1ull)) continue;
     float S = 0.f;
     int E = kNoExp;
     for (int32_t c = 0; c < h.nch; ++c) {
       const int64_t slot = (int64_t)h.first + c;
-      const float s = a.pS[slot * 64 + lane];
-      if (s > 0.f) xadd(S, E, s, a.pE[slot * 64 + lane]);
+      const float s = cld(&a.pS[slot * 64 + lane]);
+      if (s > 0.f) xadd(S, E, s, cld(&a.pE[slot * 64 + lane]));
     }
     store_norm(a, (int64_t)h.v * 64 + lane, S, E);
   }
@@ -722,9 +806,9 @@ __device__ __forceinline__ void sigma_combine(const MsArgs<OffT>& a, const HubV*
 /* delta of a pair from its arc sum: R = (1 + delta) / sigma replaces sigma */
 template<typename OffT>
 __device__ __forceinline__ void finish_delta(const MsArgs<OffT>& a, int64_t at, float delta) {
-  const float m = a.sm[at];
-  const int e = (int)a.se[at];
-  a.dl[at] = delta;
+  const float m = cld(&a.sm[at]);
+  const int e = (int)cld(&a.se[at]);
+  cst(&a.dl[at], delta);
   const float r = (1.0f + delta) / m;
   store_norm(a, at, r, -e);
 }
@@ -745,10 +829,10 @@ __device__ __forceinline__ void delta_pairs(const MsArgs<OffT>& a, int64_t base,
     uint64_t own = 0;
     int64_t b = 0, e = 0;
     if (i < cnt) {
-      u = a.lv[base + i];
+      u = cld(&a.lv[base + i]);
       b = (int64_t)a.off[u];
       e = (int64_t)a.off[u + 1];
-      if (e - b <= kHub) own = a.lm[base + i];
+      if (e - b <= kHub) own = cld(&a.lm[base + i]);
     }
     PairWalk w = pair_walk(own, e - b);
     for (uint32_t step = 0; step < w.steps; ++step) {
@@ -756,10 +840,29 @@ __device__ __forceinline__ void delta_pairs(const MsArgs<OffT>& a, int64_t base,
       const bool ok = pair_item(w, own, step, owner, l);
       const int32_t v = __shfl(u, owner, 64);
       const int64_t vb = shfl_i64(b, owner), ve = shfl_i64(e, owner);
-      if (!ok) continue;
       const int64_t at = (int64_t)v * 64 + l;
-      const float mv = a.sm[at];
-      const int ev = (int)a.se[at];
+      if (!w.flat) { /* one vertex, every lane a source */
+        const float mv = ok ? cld(&a.sm[at]) : 0.f;
+        const int ev = ok ? (int)cld(&a.se[at]) : 0;
+        float acc = 0.f;
+        wave_list_walk(a.cols, vb, ve, Xa, l, ok, [&](const int32_t (&x)[8], const bool (&hit)[8]) {
+          float r[8];
+          int er[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            r[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + l]) : 0.f;
+            er[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + l]) : 0;
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (hit[k]) acc += delta_term(mv, ev, r[k], er[k]);
+        });
+        if (ok) finish_delta(a, at, acc);
+        continue;
+      }
+      if (!ok) continue;
+      const float mv = cld(&a.sm[at]);
+      const int ev = (int)cld(&a.se[at]);
       float acc = 0.f;
       for (int64_t j = vb; j < ve; j += 4) {
         int32_t x[4];
@@ -767,13 +870,13 @@ __device__ __forceinline__ void delta_pairs(const MsArgs<OffT>& a, int64_t base,
 #pragma unroll
         for (int k = 0; k < 4; ++k) x[k] = j + k < ve ? a.cols[j + k] : -1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((Xa[x[k]] >> l) & 1ull);
+        for (int k = 0; k < 4; ++k) hit[k] = x[k] >= 0 && ((cld(&Xa[x[k]]) >> l) & 1ull);
         float r[4];
         int er[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          r[k] = hit[k] ? a.sm[(int64_t)x[k] * 64 + l] : 0.f;
-          er[k] = hit[k] ? (int)a.se[(int64_t)x[k] * 64 + l] : 0;
+          r[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + l]) : 0.f;
+          er[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + l]) : 0;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -789,28 +892,28 @@ __device__ __forceinline__ void delta_chunks(const MsArgs<OffT>& a, const HubC* 
                                              const uint64_t* Xa, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t c = gwave; c < nc; c += nwaves) {
-    const HubC h = hc[c];
-    const uint64_t own = a.lm[lbase + h.pos];
+    const HubC h = cld16(&hc[c]);
+    const uint64_t own = cld(&a.lm[lbase + h.pos]);
     const bool mine = (own >> lane) & 1ull;
     const int64_t at = (int64_t)h.v * 64 + lane;
-    const float mv = mine ? a.sm[at] : 0.f;
-    const int ev = mine ? (int)a.se[at] : 0;
+    const float mv = mine ? cld(&a.sm[at]) : 0.f;
+    const int ev = mine ? (int)cld(&a.se[at]) : 0;
     const int64_t s = (int64_t)a.off[h.v] + (int64_t)h.c * kHub;
     const int64_t e = std::min<int64_t>(s + kHub, (int64_t)a.off[h.v + 1]);
     float acc = 0.f;
-    for (int64_t j = s; j < e; j += 4) {
-      int32_t x[4];
-      bool hit[4];
+    wave_list_walk(a.cols, s, e, Xa, lane, mine, [&](const int32_t (&x)[8], const bool (&hit)[8]) {
+      float r[8];
+      int er[8];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) x[k] = j + k < e ? a.cols[j + k] : -1;
+      for (int k = 0; k < 8; ++k) {
+        r[k] = hit[k] ? cld(&a.sm[(int64_t)x[k] * 64 + lane]) : 0.f;
+        er[k] = hit[k] ? (int)cld(&a.se[(int64_t)x[k] * 64 + lane]) : 0;
+      }
 #pragma unroll
-      for (int k = 0; k < 4; ++k) hit[k] = mine && x[k] >= 0 && ((Xa[x[k]] >> lane) & 1ull);
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (hit[k])
-          acc += delta_term(mv, ev, a.sm[(int64_t)x[k] * 64 + lane], (int)a.se[(int64_t)x[k] * 64 + lane]);
-    }
-    if (c < a.p_cap) a.pS[c * 64 + lane] = acc;
+      for (int k = 0; k < 8; ++k)
+        if (hit[k]) acc += delta_term(mv, ev, r[k], er[k]);
+    });
+    if (c < a.p_cap) cst(&a.pS[c * 64 + lane], acc);
     else if (lane == 0) raise_flag(a.ctl, kOvPart);
   }
 }
@@ -820,10 +923,10 @@ __device__ __forceinline__ void delta_combine(const MsArgs<OffT>& a, const HubV*
                                               int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t i = gwave; i < nv; i += nwaves) {
-    const HubV h = hv[i];
-    if (!((a.lm[lbase + h.pos] >> lane) & 1ull)) continue;
+    const HubV h = cld16(&hv[i]);
+    if (!((cld(&a.lm[lbase + h.pos]) >> lane) & 1ull)) continue;
     float acc = 0.f;
-    for (int32_t c = 0; c < h.nch; ++c) acc += a.pS[((int64_t)h.first + c) * 64 + lane];
+    for (int32_t c = 0; c < h.nch; ++c) acc += cld(&a.pS[((int64_t)h.first + c) * 64 + lane]);
     finish_delta(a, (int64_t)h.v * 64 + lane, acc);
   }
 }
@@ -835,8 +938,8 @@ __device__ __forceinline__ void scatter_masks(const MsArgs<OffT>& a, int64_t bas
                                               bool set, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) {
-    const int32_t v = a.lv[base + i];
-    const uint64_t m = a.lm[base + i];
+    const int32_t v = cld(&a.lv[base + i]);
+    const uint64_t m = cld(&a.lm[base + i]);
     if (set) atomic_or(&Xb[v], m);
     else atomic_and(&Xb[v], ~m);
   }
@@ -848,35 +951,38 @@ __device__ __forceinline__ void accumulate(const MsArgs<OffT>& a, int64_t base, 
                                            int64_t nwaves) {
   const int lane = lane_id();
   for (int64_t i = gwave * 64 + lane; i < cnt; i += nwaves * 64) {
-    const int32_t v = a.lv[base + i];
-    uint64_t m = a.lm[base + i];
+    const int32_t v = cld(&a.lv[base + i]);
+    uint64_t m = cld(&a.lm[base + i]);
     double s = 0.0;
     while (m) {
       const int l = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      s += (double)a.dl[(int64_t)v * 64 + l];
+      s += (double)cld(&a.dl[(int64_t)v * 64 + l]);
     }
-    a.bc64[v] += s;
+    cst(&a.bc64[v], cld(&a.bc64[v]) + s);
   }
 }
 
 /* ------------------------------------------------------------- the kernel --- */
 /* Direction of level d's expansion (the same in every workgroup: the inputs
  * are the totals every workgroup reads after the barrier).  dir: the
- * previous level's direction. */
+ * previous level's direction; cnt / fe / mx: the level's vertices, arcs and
+ * largest degree; full_deg: the arcs of the vertices every lane has reached. */
 template<typename OffT>
 __device__ __forceinline__ int32_t ms_direction(const MsArgs<OffT>& a, const sg::policy::Staged* pol, int32_t d,
-                                                int32_t dir, const LevelInfo* L, int64_t cnt) {
+                                                int32_t dir, int64_t cnt, unsigned long long fe, unsigned long long mx,
+                                                unsigned long long full_deg) {
   switch (a.dir_policy) {
   case SG_POLICY_PUSH: return 0;
   case SG_POLICY_PULL: return 1;
   case SG_POLICY_HYBRID:
   case SG_POLICY_LEARNED: {
     if (d == 0) return 0; /* the sources push, as a single-source traversal starts */
-    const unsigned long long fe = atomic_load(&L->arcs), mx = atomic_load(&L->mx);
-    const unsigned long long unexplored = a.total_degree - atomic_load(&a.ctl->full_deg);
-    if (a.dir_policy == SG_POLICY_LEARNED)
+    const unsigned long long unexplored = a.total_degree - full_deg;
+    if (a.dir_policy == SG_POLICY_LEARNED) {
+      if (fe * 1024ull < a.total_degree) return 0; /* the single-source BFS's default gate */
       return sg::policy::infer_staged(pol, (double)cnt, (double)fe, (double)unexplored, d, (double)mx, a.f6, a.f7) / 3;
+    }
     if (dir == 0) return ((float)(uint32_t)fe > (float)(uint32_t)unexplored / a.alpha) ? 1 : 0;
     return ((float)cnt < (float)a.n / a.beta) ? 0 : 1;
   }
@@ -884,36 +990,62 @@ __device__ __forceinline__ int32_t ms_direction(const MsArgs<OffT>& a, const sg:
   }
 }
 
+/* A level's header words (LevelInfo's first kLevelInfoZero), read once per
+ * workgroup by its first lanes and shared through LDS: every thread loading
+ * them from the point of coherence put hundreds of thousands of loads on a
+ * few lines per level. */
+enum : int { kHCount, kHArcs, kHBase, kHHov, kHHovBase, kHHoc, kHHocBase, kHHiv, kHHivBase, kHHic, kHHicBase, kHMx };
+__device__ __forceinline__ void load_head(unsigned long long* dst, const LevelInfo* L, int lane0) {
+  const int k = (int)threadIdx.x - lane0;
+  if (k >= 0 && k < kLevelInfoZero) dst[k] = atomic_load(reinterpret_cast<const unsigned long long*>(L) + k);
+}
+
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
   __shared__ sg::policy::Staged s_pol;
   __shared__ int32_t s_stage[kBlock / 64][kStage];
   __shared__ unsigned long long s_red[2 * (kBlock / 64)];
+  __shared__ unsigned long long s_h[3][kLevelInfoZero];
+  __shared__ unsigned long long s_full;
   Stage st{s_stage[threadIdx.x >> 6], 0u, 0ull, 0ull};
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
   const int64_t n = a.n;
   uint32_t t = 0;
   int32_t d = 0, dir = 0, n_pull = 0;
-  if (a.dir_policy == SG_POLICY_LEARNED) {
-    sg::policy::stage(&s_pol, a.mlp, (int)threadIdx.x, kBlock);
-    __syncthreads();
-  }
+  if (a.dir_policy == SG_POLICY_LEARNED) sg::policy::stage(&s_pol, a.mlp, (int)threadIdx.x, kBlock);
   int64_t max_chunks = 0; /* largest chunk list of a level (the backward pass's partial rows) */
   for (;; ++d) {
     LevelInfo* L = level(a, d);
-    const int64_t cnt = (int64_t)atomic_load(&L->count);
+    load_head(s_h[0], L, 0);
+    if (d >= 2) load_head(s_h[1], level(a, d - 2), 32);
+    if (threadIdx.x == 63) s_full = atomic_load(&a.ctl->full_deg);
+    __syncthreads();
+    const unsigned long long* h = s_h[0];
+    const int64_t cnt = (int64_t)h[kHCount];
     if (cnt == 0) break;
     if (!a.ring && d + 2 >= a.li_cap) { /* every workgroup takes this exit */
       if (blockIdx.x == 0 && threadIdx.x == 0) raise_flag(a.ctl, kOvLevels);
       return;
     }
-    const int64_t base = list_base(a, d);
+    const int64_t base = a.ring ? (int64_t)(d & 3) * a.lcap : (int64_t)h[kHBase];
     uint64_t* Mc = a.M + (int64_t)(d & 3) * n;
     uint64_t* Mn = a.M + (int64_t)((d + 1) & 3) * n;
     const uint64_t* Mp = a.M + (int64_t)((d + 3) & 3) * n;
     LevelInfo* nx = level(a, d + 1);
-    const Bases nb = next_bases(a, d);
+    Bases nb;
+    if (a.ring) {
+      const int64_t sl = (d + 1) & 3;
+      nb.l = sl * a.lcap;
+      nb.hov = nb.hiv = sl * a.hv_cap;
+      nb.hoc = nb.hic = sl * a.hc_cap;
+    } else {
+      nb.l = (int64_t)(h[kHBase] + h[kHCount]);
+      nb.hov = (int64_t)(h[kHHovBase] + h[kHHov]);
+      nb.hoc = (int64_t)(h[kHHocBase] + h[kHHoc]);
+      nb.hiv = (int64_t)(h[kHHivBase] + h[kHHiv]);
+      nb.hic = (int64_t)(h[kHHicBase] + h[kHHic]);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       LevelInfo* z = level(a, d + 2); /* the level after next starts from zero */
       unsigned long long* zp = reinterpret_cast<unsigned long long*>(z);
@@ -924,11 +1056,22 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
       atomic_store(&nx->hiv_base, (unsigned long long)nb.hiv);
       atomic_store(&nx->hic_base, (unsigned long long)nb.hic);
     }
-    const int64_t hoc = (int64_t)atomic_load(&L->hoc);
+    const int64_t hoc = (int64_t)h[kHHoc];
     max_chunks = hoc > max_chunks ? hoc : max_chunks;
-    const HubC* hocp = a.hoc + (a.ring ? (int64_t)(d & 3) * a.hc_cap : (int64_t)atomic_load(&L->hoc_base));
-    dir = ms_direction(a, &s_pol, d, dir, L, cnt);
+    const HubC* hocp = a.hoc + (a.ring ? (int64_t)(d & 3) * a.hc_cap : (int64_t)h[kHHocBase]);
+    dir = ms_direction(a, &s_pol, d, dir, cnt, h[kHArcs], h[kHMx], s_full);
     n_pull += dir;
+    const int64_t o_cnt = d >= 2 ? (int64_t)s_h[1][kHCount] : 0;
+    const int64_t o_base = d >= 2 ? (a.ring ? (int64_t)((d - 2) & 3) * a.lcap : (int64_t)s_h[1][kHBase]) : 0;
+    int64_t hiv = 0, hic = 0;
+    const HubV* hivp = nullptr;
+    const HubC* hicp = nullptr;
+    if (a.bc && d >= 1) {
+      hicp = a.directed ? a.hic + (int64_t)h[kHHicBase] : hocp;
+      hic = a.directed ? (int64_t)h[kHHic] : hoc;
+      hiv = a.directed ? (int64_t)h[kHHiv] : (int64_t)h[kHHov];
+      hivp = a.directed ? a.hiv + (int64_t)h[kHHivBase] : a.hov + (int64_t)h[kHHovBase];
+    }
     if (dir) {
       expand_pull(a, Mc, Mn, nx, nb, st, gwave, nwaves);
     } else {
@@ -937,20 +1080,10 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
     }
     stage_finish(a, nx, nb, st, s_red);
     visit_level(a, d, base, cnt, Mc, gwave, nwaves);
-    if (d >= 2) {
-      LevelInfo* O = level(a, d - 2);
-      clear_level(a, list_base(a, d - 2), (int64_t)atomic_load(&O->count), a.M + (int64_t)((d + 2) & 3) * n, gwave,
-                  nwaves);
-    }
-    int64_t hiv = 0;
-    const HubV* hivp = nullptr;
+    if (d >= 2) clear_level(a, o_base, o_cnt, a.M + (int64_t)((d + 2) & 3) * n, gwave, nwaves);
     if (a.bc && d >= 1) {
       sigma_pairs(a, base, cnt, Mc, Mp, gwave, nwaves);
-      const HubC* hicp = a.directed ? a.hic + (int64_t)atomic_load(&L->hic_base) : hocp;
-      const int64_t hic = a.directed ? (int64_t)atomic_load(&L->hic) : hoc;
       sigma_chunks(a, hicp, hic, Mc, Mp, gwave, nwaves);
-      hiv = a.directed ? (int64_t)atomic_load(&L->hiv) : (int64_t)atomic_load(&L->hov);
-      hivp = a.directed ? a.hiv + (int64_t)atomic_load(&L->hiv_base) : a.hov + (int64_t)atomic_load(&L->hov_base);
     }
     if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
     if (hiv > 0) {
@@ -973,29 +1106,29 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
   /* backward: levels d-1 .. 1; X[j] = M[j] hold the memberships */
   const int32_t deepest = d - 1;
   for (int64_t i = gwave * 64 + lane_id(); i < n; i += nwaves * 64) {
-    a.M[i] = 0ull;
-    a.M[n + i] = 0ull;
+    cst(&a.M[i], 0ull);
+    cst(&a.M[n + i], 0ull);
   }
   if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
   for (int32_t dd = deepest; dd >= 1; --dd) {
-    LevelInfo* L = level(a, dd);
-    const int64_t base = (int64_t)atomic_load(&L->base), cnt = (int64_t)atomic_load(&L->count);
+    load_head(s_h[0], level(a, dd), 0);
+    if (dd + 1 <= deepest) load_head(s_h[1], level(a, dd + 1), 32);
+    if (dd + 2 <= deepest) load_head(s_h[2], level(a, dd + 2), 64);
+    __syncthreads();
+    const unsigned long long* h = s_h[0];
+    const int64_t base = (int64_t)h[kHBase], cnt = (int64_t)h[kHCount];
     const uint64_t* Xa = a.M + (int64_t)((dd + 1) & 1) * n;
     uint64_t* Xb = a.M + (int64_t)(dd & 1) * n;
-    const int64_t hov = (int64_t)atomic_load(&L->hov), hoc = (int64_t)atomic_load(&L->hoc);
-    const HubV* hovp = a.hov + (int64_t)atomic_load(&L->hov_base);
-    const HubC* hocp = a.hoc + (int64_t)atomic_load(&L->hoc_base);
+    const int64_t hov = (int64_t)h[kHHov], hoc = (int64_t)h[kHHoc];
+    const HubV* hovp = a.hov + (int64_t)h[kHHovBase];
+    const HubC* hocp = a.hoc + (int64_t)h[kHHocBase];
+    const int64_t b1 = (int64_t)s_h[1][kHBase], c1 = (int64_t)s_h[1][kHCount];
+    const int64_t b2 = (int64_t)s_h[2][kHBase], c2 = (int64_t)s_h[2][kHCount];
     delta_pairs(a, base, cnt, Xa, gwave, nwaves);
     delta_chunks(a, hocp, hoc, base, Xa, gwave, nwaves);
     scatter_masks(a, base, cnt, Xb, true, gwave, nwaves);
-    if (dd + 2 <= deepest) {
-      LevelInfo* L2 = level(a, dd + 2);
-      scatter_masks(a, (int64_t)atomic_load(&L2->base), (int64_t)atomic_load(&L2->count), Xb, false, gwave, nwaves);
-    }
-    if (dd + 1 <= deepest) {
-      LevelInfo* L1 = level(a, dd + 1);
-      accumulate(a, (int64_t)atomic_load(&L1->base), (int64_t)atomic_load(&L1->count), gwave, nwaves);
-    }
+    if (dd + 2 <= deepest) scatter_masks(a, b2, c2, Xb, false, gwave, nwaves);
+    if (dd + 1 <= deepest) accumulate(a, b1, c1, gwave, nwaves);
     if (!grid_sync(a.ctl, t, a.wait_ticks)) return;
     if (hov > 0) {
       delta_combine(a, hovp, hov, base, gwave, nwaves);

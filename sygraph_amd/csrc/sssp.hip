@@ -665,6 +665,7 @@ struct SsspWorkspace {
   QEntry* burst_list[2] = {nullptr, nullptr}; /* kSsspBurstMax entries each */
   RoundLog* log = nullptr; /* kRing entries, pinned coherent host memory */
   hipEvent_t ev[kRing] = {};
+  hipEvent_t ev_pre[kRing] = {}; /* before each iteration's relax launch (record_relax) */
   QueueSet qs;
   int64_t w0 = 0;
 };
@@ -684,6 +685,8 @@ void sssp_ws_free(SsspWorkspace* w) {
   (void)hipFree(w->burst_list[1]);
   if (w->log) (void)hipHostFree(w->log);
   for (hipEvent_t e : w->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : w->ev_pre)
     if (e) (void)hipEventDestroy(e);
   w->qs.release();
   delete w;
@@ -716,7 +719,8 @@ int sssp_workspace_for(sg_graph_t g, SsspWorkspace** out) {
       return st;
     }
     bool ok = hipHostMalloc((void**)&ws->log, sizeof(RoundLog) * kRing, hipHostMallocCoherent) == hipSuccess;
-    for (int r = 0; r < kRing && ok; ++r) ok = hipEventCreate(&ws->ev[r]) == hipSuccess;
+    for (int r = 0; r < kRing && ok; ++r)
+      ok = hipEventCreate(&ws->ev[r]) == hipSuccess && hipEventCreate(&ws->ev_pre[r]) == hipSuccess;
     if (!ok) {
       sssp_ws_free(ws);
       sg::set_error("sssp workspace: event or pinned log allocation failed");
@@ -753,8 +757,8 @@ int ensure_split(sg_graph_t g, W dlt) {
 }
 
 template<typename OffT, typename W>
-int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_t burst_arcs, W* dist_out,
-             sg_sssp_stats* stats) {
+int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_t burst_arcs, bool record_relax,
+             W* dist_out, sg_sssp_stats* stats) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   const int64_t n = g->n;
@@ -858,6 +862,7 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
   auto enqueue = [&](int32_t it) -> int {
     sssp_select_kernel<OffT, W><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, sa, ws->ctl, it);
     near_write_kernel<OffT, W><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, sb, ws->ctl);
+    if (record_relax) SG_HIP(hipEventRecord(ws->ev_pre[it % kRing], s));
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
     SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
@@ -867,6 +872,8 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
   SG_TRY(enqueue(1));
   int32_t round = 0;
   int64_t relaxed = 0;
+  float relax_ms = 0.f;
+  int32_t relax_launches = 0;
   const int64_t max_iter = 4 * n + 256; /* every iteration relaxes or advances T past a distance */
   for (int32_t it = 0;; ++it) {
     SG_HIP(hipEventSynchronize(ws->ev[it % kRing]));
@@ -880,6 +887,12 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
     if (e.relax && stats && e.rounds - 1 < SG_MAX_LEVELS_RECORDED) {
       stats->round_vertices[e.rounds - 1] = e.near_v;
       stats->round_edges[e.rounds - 1] = e.near_e;
+    }
+    if (e.relax && record_relax) { /* the relax launch of a relaxing iteration, on its own */
+      float ms = 0.f;
+      SG_HIP(hipEventElapsedTime(&ms, ws->ev_pre[it % kRing], ws->ev[it % kRing]));
+      relax_ms += ms;
+      ++relax_launches;
     }
     round = e.rounds;
     relaxed = (int64_t)e.relaxed;
@@ -900,6 +913,8 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
     (void)hipEventDestroy(e1);
     stats->kernel_ms = ms;
     stats->rounds = round;
+    stats->relax_ms = relax_ms;
+    stats->relax_launches = relax_launches;
     {
       unsigned long long rs[16 * kSlots];
       SG_HIP(hipMemcpyAsync(rs, ws->relax_slots, sizeof(rs), hipMemcpyDeviceToHost, s));
@@ -962,8 +977,8 @@ int mean_weight(sg_graph_t g, double* out) {
 constexpr int64_t kAutoBucketArcs = int64_t(1) << 25;
 
 template<typename OffT, typename W>
-int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, int64_t burst_arcs, W* dist,
-                  sg_sssp_stats* stats) {
+int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, int64_t burst_arcs,
+                  bool record_relax, W* dist, sg_sssp_stats* stats) {
   if (delta == 0.0 && g->m < kAutoBucketArcs) {
     delta = -1.0;
   } else if (delta == 0.0) {
@@ -971,7 +986,7 @@ int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull
     SG_TRY(mean_weight<W>(g, &mw));
     delta = std::max(kAutoDeltaFactor * mw, DistOps<W>::is_float ? 1e-30 : 1.0);
   }
-  return run_sssp<OffT, W>(g, source, delta, heavy_pull, burst_arcs, dist, stats);
+  return run_sssp<OffT, W>(g, source, delta, heavy_pull, burst_arcs, record_relax, dist, stats);
 }
 
 extern "C" {
@@ -1001,15 +1016,19 @@ int sg_sssp_ex(sg_graph_t g, int32_t source, const sg_sssp_options* opt_in, void
     stats->rounds = 0;
     stats->reached_vertices = stats->reached_edges = stats->relaxations = 0;
     stats->kernel_ms = 0.f;
+    stats->relax_ms = 0.f;
+    stats->relax_launches = 0;
   }
   SG_HIP(hipSetDevice(g->dev->ordinal));
   const bool fw = (g->flags & SG_GRAPH_WEIGHT_FLOAT) != 0;
   const int64_t ba = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kSsspBurstDefault : opt.burst_arcs);
+  const bool rr = opt.record_relax != 0 && stats != nullptr;
   if (g->offset_bytes == 8)
-    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, opt.heavy_pull, ba, (float*)dist, stats)
-              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, (uint32_t*)dist, stats);
-  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, opt.heavy_pull, ba, (float*)dist, stats)
-            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, (uint32_t*)dist, stats);
+    return fw ? run_sssp_auto<int64_t, float>(g, source, opt.delta, opt.heavy_pull, ba, rr, (float*)dist, stats)
+              : run_sssp_auto<int64_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, rr, (uint32_t*)dist,
+                                                 stats);
+  return fw ? run_sssp_auto<uint32_t, float>(g, source, opt.delta, opt.heavy_pull, ba, rr, (float*)dist, stats)
+            : run_sssp_auto<uint32_t, uint32_t>(g, source, opt.delta, opt.heavy_pull, ba, rr, (uint32_t*)dist, stats);
 }
 
 int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_sssp_stats* stats) {

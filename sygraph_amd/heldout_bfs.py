@@ -7,13 +7,15 @@ train_policy.make_graph / sources_of: four sources per graph).  Per graph the
 device time (hipEvents on the library stream, best of three) of traversing
 from all four sources under
 
-  plain      push with the bucketing balancer, one source after another (the
-             reference's plain method on this engine);
-  heuristic  the reference's direction rule (alpha 15, beta 24), one source
-             after another;
-  learned    the shipped learned policy, one source after another;
-  batched    the four sources as bit-lanes of one multi-source BFS
-             (sg_bfs_multi).
+  plain            push with the bucketing balancer, one source after
+                   another (the reference's plain method on this engine);
+  heuristic        the reference's direction rule (alpha 15, beta 24), one
+                   source after another;
+  learned          the shipped learned policy, one source after another;
+  batched          the four sources as bit-lanes of one multi-source BFS
+                   (sg_bfs_multi; its default direction rule);
+  learned_batched  the same batch with the learned policy choosing each
+                   level's direction on the batch's totals (sg_bfs_multi_ex).
 
 The depths of every source are compared across the four (device digests) and
 must be equal.  Writes the per-graph times and the totals / geometric means.
@@ -35,7 +37,7 @@ if os.path.dirname(HERE) not in sys.path:
 import sygraph_amd as sg  # noqa: E402
 from sygraph_amd.train_policy import make_graph, sources_of  # noqa: E402
 
-KEYS = ["plain", "heuristic", "learned", "batched"]
+KEYS = ["plain", "heuristic", "learned", "batched", "learned_batched"]
 
 
 def log(*a):
@@ -79,14 +81,16 @@ def evaluate(dev, pol, seeds, quick):
                 digests.setdefault(s, set()).add(b.depth.digest(G.n, 4))
         src_arr = np.ascontiguousarray(np.asarray(srcs, dtype=np.int32))
         out = sg.DeviceBuffer(dev, 4 * len(srcs) * G.n, dtype=np.int32)
-
-        def run_batched():
-            sg.check(sg.lib().sg_bfs_multi(G.h, len(srcs), src_arr.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr)))
-        t["batched"] = _timed(dev, run_batched)
-        for i, s in enumerate(srcs):
-            h = C.c_uint64()
-            sg.check(sg.lib().sg_digest(dev.h, C.c_void_p(out.ptr + 4 * i * G.n), G.n, 4, C.byref(h)))
-            digests[s].add(h.value)
+        for key, mo in (("batched", sg.msbfs_options()),
+                        ("learned_batched", sg.msbfs_options(sg.POLICY_LEARNED, learned=pol))):
+            def run_batched():
+                sg.check(sg.lib().sg_bfs_multi_ex(G.h, len(srcs), src_arr.ctypes.data_as(C.c_void_p), C.byref(mo),
+                                                  C.c_void_p(out.ptr)))
+            t[key] = _timed(dev, run_batched)
+            for i, s in enumerate(srcs):
+                h = C.c_uint64()
+                sg.check(sg.lib().sg_digest(dev.h, C.c_void_p(out.ptr + 4 * i * G.n), G.n, 4, C.byref(h)))
+                digests[s].add(h.value)
         if any(len(d) != 1 for d in digests.values()):
             raise RuntimeError(f"seed {seed}: depths differ between the methods")
         out.free()
@@ -108,8 +112,9 @@ def main():
     rows = evaluate(dev, pol, seeds, a.quick)
     tot = {k: float(sum(r[k] for r in rows)) for k in KEYS}
     geo = lambda num, den: float(np.exp(np.mean([np.log(r[num] / r[den]) for r in rows])))  # noqa: E731
-    summary = {"total_ms": tot}
-    for k in ("heuristic", "learned", "batched"):
+    summary = {"total_ms": tot, "learned_speedup_vs_heuristic_total": tot["heuristic"] / tot["learned"],
+               "learned_speedup_vs_heuristic_geomean": geo("heuristic", "learned")}
+    for k in ("heuristic", "learned", "batched", "learned_batched"):
         summary[f"{k}_speedup_vs_plain_total"] = tot["plain"] / tot[k]
         summary[f"{k}_speedup_vs_plain_geomean"] = geo("plain", k)
     log(json.dumps(summary, indent=1))

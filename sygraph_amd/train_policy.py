@@ -3,10 +3,11 @@
     python -m sygraph_amd.train_policy [--out sygraph_amd/data] [--quick]
 
 1. Graph sets.  Training: seeds 100-299; held-out: seeds 300-349.  The seed
-   fixes the family and size: RMAT (scale 16-22, edge factor 8/16/32,
-   a in [0.45, 0.60], b = c = (1 - a) / 2.6), 4-neighbour grids (side
-   256-1024, 0-30 % of edges deleted) and Erdos-Renyi graphs (2^16-2^21
-   vertices, average degree 4-32).  All are generated on the device.
+   fixes the family and size: RMAT (scale 16-22, a in [0.45, 0.65],
+   b = c = (0.95 - a) / 2 so d = 0.05, edge factor 4-32), 4-neighbour grids
+   (side 512, 1024, 2048 or 4096, 0-20 % of edges deleted) and Erdos-Renyi
+   graphs (2^16-2^21 vertices, average degree 4-64).  All are generated on
+   the device.
 2. Per-level costs.  From four non-isolated sources per graph, the BFS runs
    once per action with that action forced at every level (push or pull x
    work mapping variant 0/1/2).  The frontier sequence of a BFS does not
@@ -52,23 +53,23 @@ def log(*a):
 
 def make_graph(dev, seed: int, quick: bool, weight_seed: int = 0):
     """The graph of a seed (family and size drawn from the seed); with
-    weight_seed != 0 it carries u32 weights U[1,255]."""
+    weight_seed != 0 it carries u32 weights U[1,255].  quick: smaller sizes
+    of the same families (smoke runs)."""
     rng = np.random.default_rng(seed)
     kind = rng.choice(["rmat", "rmat", "rmat", "grid", "er"])
-    top = 19 if quick else 22
     if kind == "rmat":
-        scale = int(rng.integers(16, top + 1))
-        ef = int(rng.choice([8, 16, 32]))
-        a = float(rng.uniform(0.45, 0.60))
-        bc = (1.0 - a) / 2.6
+        scale = int(rng.integers(16, (19 if quick else 22) + 1))
+        ef = int(rng.integers(4, 33))
+        a = float(rng.uniform(0.45, 0.65))
+        bc = (0.95 - a) / 2.0
         return f"rmat s{scale} ef{ef} a{a:.3f}", sg.Graph.rmat(dev, scale, ef, a, bc, bc, seed=seed,
                                                               weight_seed=weight_seed)
     if kind == "grid":
-        side = int(rng.choice([128, 256, 512] if quick else [256, 512, 1024]))
-        p = float(rng.uniform(0.0, 0.3))
+        side = int(rng.choice([128, 256, 512] if quick else [512, 1024, 2048, 4096]))
+        p = float(rng.uniform(0.0, 0.2))
         return f"grid {side}x{side} p{p:.2f}", sg.Graph.grid(dev, side, side, p, seed=seed, weight_seed=weight_seed)
     lg = int(rng.integers(16, (19 if quick else 21) + 1))
-    deg = int(rng.choice([4, 8, 16, 32]))
+    deg = int(rng.integers(4, 65))
     n = 1 << lg
     return f"er 2^{lg} d{deg}", sg.Graph.erdos_renyi(dev, n, n * deg // 2, seed=seed, weight_seed=weight_seed)
 
@@ -93,11 +94,12 @@ def level_costs(G, src: int):
     for pol, lb in ACTIONS:
         b.init(src)
         st = b.run(pol, 15.0, 24.0, load_balancer=lb, tuning=NO_BURSTS)
-        c = [st.level[i].kernel_ms + st.level[i].convert_ms for i in range(st.levels)]
+        nrec = min(st.levels, sg.MAX_LEVELS_RECORDED)  # levels past the record limit give no samples
+        c = [st.level[i].kernel_ms + st.level[i].convert_ms for i in range(nrec)]
         costs.append(c)
         if stats0 is None:
             stats0 = [(st.level[i].frontier_vertices, st.level[i].frontier_edges, st.level[i].unexplored_edges,
-                       st.level[i].max_degree) for i in range(st.levels)]
+                       st.level[i].max_degree) for i in range(nrec)]
     L = min(len(c) for c in costs)
     return np.array([[costs[a][i] for a in range(len(ACTIONS))] for i in range(L)]), stats0[:L]
 
@@ -116,18 +118,28 @@ def level_features(stats0, n: int, m: int):
     return np.array(out, dtype=np.float32).reshape(-1, 8)
 
 
+MAX_LEVELS_PER_SOURCE = 1024  # long traversals (grids) give at most this many samples, spread evenly
+
+
 def collect(dev, seeds, quick):
     X, Y, C = [], [], []
     for seed in seeds:
+        t0 = time.time()
         name, G = make_graph(dev, seed, quick)
         for src in sources_of(G, seed):
             cost, stats0 = level_costs(G, src)
             if len(stats0) < 2:
                 continue
-            X.append(level_features(stats0, G.n, G.m))
-            Y.append(np.argmin(cost[1:], axis=1).astype(np.int32))
-            C.append(cost[1:])
-        log(f"  seed {seed}: {name} n={G.n} m={G.m}")
+            x, c = level_features(stats0, G.n, G.m), cost[1:]
+            if len(c) > MAX_LEVELS_PER_SOURCE:
+                keep = np.unique(np.linspace(0, len(c) - 1, MAX_LEVELS_PER_SOURCE).astype(np.int64))
+                # each kept level stands for len(c) / len(keep) levels: its costs are scaled so
+                # that the stake-weighted sampling of the training keeps the source's weight
+                x, c = x[keep], c[keep] * (len(c) / len(keep))
+            X.append(x)
+            Y.append(np.argmin(c, axis=1).astype(np.int32))
+            C.append(c)
+        log(f"  seed {seed}: {name} n={G.n} m={G.m} ({time.time() - t0:.1f} s)")
         G.destroy()
     return np.concatenate(X), np.concatenate(Y), np.concatenate(C)
 
@@ -189,6 +201,11 @@ def main():
     ap.add_argument("--quick", action="store_true", help="smaller graphs and sets (smoke run)")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--candidates", type=int, default=4,
+                    help="networks trained from different initialisations; the one with the smallest time "
+                         "regret on the training levels is kept")
+    ap.add_argument("--no-eval", action="store_true", help="skip the held-out evaluation")
+    ap.add_argument("--samples", default=None, help="also save the training levels (npz) here")
     a = ap.parse_args()
     dev = sg.Device(0)
     train_seeds = list(range(100, 300)) if not a.quick else list(range(100, 116))
@@ -197,11 +214,31 @@ def main():
     log("collecting training levels ...")
     X, Y, C = collect(dev, train_seeds, a.quick)
     log(f"{len(Y)} samples, class counts {np.bincount(Y, minlength=6).tolist()} ({time.time() - t0:.1f}s)")
-    pol, losses = train(dev, X, Y, C, a.steps, a.lr)
-    pred = np.argmax(pol.forward(X), axis=1)
+    if a.samples:
+        np.savez(a.samples, X=X, Y=Y, C=C)
+    best = None
+    for cand in range(a.candidates):
+        p_, l_ = train(dev, X, Y, C, a.steps, a.lr, seed=cand)
+        pred = np.argmax(p_.forward(X), axis=1)
+        reg = float((C[np.arange(len(Y)), pred] - C.min(axis=1)).sum() / C.min(axis=1).sum())
+        log(f"candidate {cand}: loss {l_[0]:.3f} -> {l_[-1]:.3f}, time regret {reg:.3%}")
+        if best is None or reg < best[0]:
+            best = (reg, cand, p_, l_, pred)
+    regret, chosen, pol, losses, pred = best
     acc = float((pred == Y).mean())
-    regret = float((C[np.arange(len(Y)), pred] - C.min(axis=1)).sum() / C.min(axis=1).sum())
-    log(f"train: loss {losses[0]:.3f} -> {losses[-1]:.3f}, accuracy {acc:.3f}, time regret {regret:.3%}")
+    log(f"train: candidate {chosen}, loss {losses[0]:.3f} -> {losses[-1]:.3f}, accuracy {acc:.3f}, "
+        f"time regret {regret:.3%}")
+    os.makedirs(a.out, exist_ok=True)
+    pol.get_params().astype(np.float32).tofile(os.path.join(a.out, "bfs_policy.f32"))
+    meta = {"train_seeds": [train_seeds[0], train_seeds[-1]], "held_out_seeds": [test_seeds[0], test_seeds[-1]],
+            "quick": a.quick, "samples": int(len(Y)), "class_counts": np.bincount(Y, minlength=6).tolist(),
+            "steps": a.steps, "lr": a.lr, "candidates": a.candidates, "chosen": chosen,
+            "loss_first": losses[0], "loss_last": losses[-1], "train_accuracy": acc, "train_time_regret": regret}
+    if a.no_eval:
+        with open(os.path.join(a.out, "bfs_policy.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+        print(json.dumps({k: meta[k] for k in ("samples", "train_accuracy", "train_time_regret")}))
+        return
     rows = evaluate(dev, pol, test_seeds, a.quick)
     tot = {k: float(sum(r[k] for r in rows)) for k in ["plain", "heuristic", "learned", "per_level_best",
                                                         "plain_bursts", "heuristic_bursts", "learned_bursts"]}
@@ -216,12 +253,7 @@ def main():
                                "learned_speedup_vs_plain_geomean": geo("plain_bursts", "learned_bursts"),
                                "learned_speedup_vs_heuristic_geomean": geo("heuristic_bursts", "learned_bursts")}}
     log(json.dumps(summary, indent=1))
-    os.makedirs(a.out, exist_ok=True)
-    pol.get_params().astype(np.float32).tofile(os.path.join(a.out, "bfs_policy.f32"))
-    meta = {"train_seeds": [train_seeds[0], train_seeds[-1]], "held_out_seeds": [test_seeds[0], test_seeds[-1]],
-            "quick": a.quick, "samples": int(len(Y)), "class_counts": np.bincount(Y, minlength=6).tolist(),
-            "steps": a.steps, "lr": a.lr, "loss_first": losses[0], "loss_last": losses[-1],
-            "train_accuracy": acc, "train_time_regret": regret, "held_out": summary, "held_out_runs": rows}
+    meta.update({"held_out": summary, "held_out_runs": rows})
     with open(os.path.join(a.out, "bfs_policy.json"), "w") as f:
         json.dump(meta, f, indent=1)
     print(json.dumps(summary))

@@ -81,10 +81,19 @@ def test_learned_bfs_follows_the_policy_and_stays_exact(sg, device, oracle_mod, 
     pol = sg.Policy(device, _bias_only(action))
     b = sg.BFS(G)
     b.init(src)
-    st = b.run(sg.POLICY_LEARNED, learned=pol)
+    st = b.run(sg.POLICY_LEARNED, learned=pol, tuning={"learned_gate": -1})  # the network decides every level
     assert np.array_equal(b.get_distances(), ref) and st.levels == levels
     acts = [st.level[i].action for i in range(st.levels)]
     assert acts[0] == 0 and all(a == action for a in acts[1:]), acts
+    # with the default gate, the levels whose frontier has fewer arcs than 1/1024 of the
+    # graph's arcs push with the bucketing mapping and the others follow the network
+    b.init(src)
+    st = b.run(sg.POLICY_LEARNED, learned=pol)
+    assert np.array_equal(b.get_distances(), ref)
+    lv = [st.level[i] for i in range(st.levels)]
+    for L in range(1, st.levels):
+        gated = lv[L].frontier_edges * 1024 < g.m
+        assert lv[L].action == (0 if gated else action), (L, lv[L].action, gated)
 
 
 @pytest.mark.parametrize("lb", ["LB_WORKGROUP", "LB_SUBGROUP", "LB_WORKITEM"])
@@ -125,7 +134,7 @@ def test_in_situ_decisions_match_the_oracle_forward(sg, device, oracle_mod, kind
         src = int(rng.choice(np.nonzero(g.degrees() > 0)[0]))
         b = sg.BFS(G)
         b.init(src)
-        st = b.run(sg.POLICY_LEARNED, learned=pol)
+        st = b.run(sg.POLICY_LEARNED, learned=pol, tuning={"learned_gate": -1})
         assert np.array_equal(b.get_distances(), oracle_mod.bfs(g, src)[0])
         lv = [st.level[i] for i in range(st.levels)]
         for L in range(1, st.levels):

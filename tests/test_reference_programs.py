@@ -129,10 +129,84 @@ def test_example_tc(device, oracle_mod, rmat_file):
 
 @pytest.mark.parametrize("prog", ["cc", "bc"])
 def test_example_runs(device, rmat_file, prog):
+    """The cc and bc examples print no result (their print branch is a TODO
+    and their validate() returns false, examples/cc/cc.cpp and
+    examples/bc/bc.cpp), so what can be compared is their own output: the run
+    line and, with -c, the reference's validation stub reporting Failed."""
     _, plain, _ = rmat_file
-    r = _run([_exe("examples", prog), "-u", "-s", "3", plain])
+    r = _run([_exe("examples", prog), "-u", "-c", "-s", "3", plain])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "[!] Done" in r.stderr
+    assert f"[*] Running {prog.upper()} on source 3" in r.stdout
+    assert "Validation: [Failed] | Validation Time: " in r.stdout
+
+
+def _run_lengths(dirs):
+    """examples/bfs/bfs.cpp:74-97 printAdvanceDetails: the per-level directions
+    as runs, "(push x2, pull x3, push x1)"."""
+    runs = []
+    for d in dirs:
+        name = "push" if d == 0 else "pull"
+        if runs and runs[-1][0] == name:
+            runs[-1][1] += 1
+        else:
+            runs.append([name, 1])
+    return "(" + ", ".join(f"{k} x{c}" for k, c in runs) + ")"
+
+
+def test_example_bfs_hybrid_run_details(device, oracle_mod, rmat_file):
+    """The hybrid run details the bfs example prints (Iterations, Push steps,
+    Pull steps and the direction runs, examples/bfs/bfs.cpp:74-97 and
+    :145-151) against the reference rule (alpha 15, beta 24) restated in the
+    oracle's OpenMP port of the plain method."""
+    g, plain, _ = rmat_file
+    deg = np.diff(g.offsets)
+    for src in (int(np.argmax(deg)), int(np.nonzero(deg == 1)[0][0])):
+        r = _run([_exe("examples", "bfs"), "-u", "-s", str(src), "--advance", "hybrid", plain])
+        assert r.returncode == 0, r.stderr[-2000:]
+        _, levels, dirs = oracle_mod.bfs_plain_omp(g, src, oracle_mod.BFS_HYBRID, 15.0, 24.0, with_dirs=True)
+        assert f"Iterations: {levels}" in r.stderr
+        assert f"Push steps: {int((dirs == 0).sum())}" in r.stderr
+        assert f"Pull steps: {int((dirs == 1).sum())}" in r.stderr
+        assert _run_lengths(dirs) in r.stdout.splitlines()
+        assert f"[*] Running BFS (hybrid advance, alpha=15, beta=24) from source vertex {src}" in r.stdout
+
+
+def _profile_lines(stdout):
+    kernels = re.findall(r"^ Kernel \[(\S+) x (\d+)\] (.*)Time: ([0-9.eE+-]+) ms$", stdout, re.M)
+    total = re.search(r"^Total GPU Time: ([0-9.eE+-]+) ms$", stdout, re.M)
+    mteps = re.search(r"^Total Edge-Througput \(MTEPS\): ([0-9.eE+-]+) MTEPS$", stdout, re.M)
+    return kernels, total, mteps
+
+
+@pytest.mark.parametrize("detailed", [False, True])
+def test_example_bfs_profiler_layout(device, oracle_mod, rmat_file, detailed):
+    """Profiler::print (utils/profiler.hpp:47-68): one " Kernel [tag x n] "
+    line per tag (with -V each event's "(i) t ms " before "| "), then "Total
+    GPU Time: t ms" = the sum of the tags' times and "Total Edge-Througput
+    (MTEPS): x MTEPS" = visited edges / 1e6 / (t / 1e3), the visited edges
+    being the degrees of the reached vertices (algorithms/bfs.hpp:88-99),
+    here from the oracle's depths."""
+    g, plain, _ = rmat_file
+    src = int(np.argmax(np.diff(g.offsets)))
+    r = _run([_exe("examples", "bfs_profiling"), "-u", "-V" if detailed else "-v", "-s", str(src), plain])
+    assert r.returncode == 0, r.stderr[-2000:]
+    kernels, total, mteps = _profile_lines(r.stdout)
+    assert kernels and total and mteps, r.stdout[-2000:]
+    t_sum = sum(float(k[3]) for k in kernels)
+    t_tot = float(total.group(1))
+    assert abs(t_sum - t_tot) <= 1e-3 * max(t_tot, 1e-3) + 1e-4
+    for tag, count, detail, t in kernels:
+        if detailed:
+            per = [float(m[1]) for m in re.findall(r"\((\d+)\) ([0-9.eE+-]+) ms ", detail)]
+            assert len(per) == int(count) and detail.endswith("| ")
+            assert abs(sum(per) - float(t)) <= 1e-3 * max(float(t), 1e-3) + 1e-4 * int(count)
+        else:
+            assert detail == ""
+    depth, _ = oracle_mod.bfs(g, src)
+    visited = int(np.diff(g.offsets)[depth != g.n + 1].sum())
+    want = visited / 1e6 / (t_tot / 1e3)
+    assert abs(float(mteps.group(1)) - want) <= 2e-3 * want + 1e-3
 
 
 @pytest.mark.parametrize("prog", ["bfs", "sssp", "cc", "tc", "bc"])
