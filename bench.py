@@ -123,6 +123,28 @@ def copy_ceiling(sg, dev, mib: int = 1024, reps: int = 5):
     return 2.0 * nbytes / (best / 1e3) / 1e9
 
 
+def gather_calibration(sg, dev, src_gib: int = 8, count: int = 1 << 24):
+    """One random 4-byte gather (sg_gather_device) of `count` words from a
+    `src_gib` GiB source, far larger than the caches, so nearly every word
+    brings in a line of its own: the reference point of the HBM counters for
+    random probes (the streaming copy is the other).  Returns the gather's
+    rate in G words/s (hipEvents on the library stream)."""
+    import ctypes as C
+
+    words = (src_gib << 30) // 4
+    src, dst = dev.alloc(4 * words), dev.alloc(4 * count)
+    lib = sg.lib()
+    e0, e1 = dev.event(), dev.event()
+    e0.record()
+    sg.check(lib.sg_gather_device(dev.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), words, count, 7))
+    e1.record()
+    dev.synchronize()
+    ms = e0.elapsed_ms(e1)
+    src.free()
+    dst.free()
+    return count / (ms / 1e3) / 1e9
+
+
 def measured_traffic(kernel: str, summary: str = "bfs_hybrid_rmat24_traffic.json"):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the same counter read on
@@ -439,7 +461,8 @@ def main():
         sources = [int(x) for x in rng.choice(cand, 16, replace=False)]
     pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[a.policy]
     if a.algo == "sssp":
-        copy_ceiling(sg, dev, mib=256, reps=2)  # the counters' calibration kernel, in the same process
+        copy_ceiling(sg, dev, mib=256, reps=2)  # the counters' calibration kernels, in the same process
+        gather_calibration(sg, dev)
         for _ in range(a.warmup):
             run_sssp(sg, G, sources, a.delta)
         e, w = 0, 0.0
@@ -487,6 +510,9 @@ def main():
     cc = copy_ceiling(sg, dev)
     roof["copy_ceiling_gbs"] = round(cc, 1)
     log(f"copy kernel ceiling: {cc:.0f} GB/s (read + write)")
+    gw = gather_calibration(sg, dev)
+    roof["random_gather_gwords_s"] = round(gw, 2)
+    log(f"random 4-byte gather: {gw:.2f} G words/s")
     if not a.no_extra:
         bp = batched_vs_plain(sg, G, sources)
         extra["bfs_64_sources_gteps"] = bp

@@ -104,6 +104,12 @@ int sg_memcpy(sg_device_t dev, void* dst, const void* src, size_t bytes);
  * it is the copy kernel against which HBM bandwidth and the FETCH_SIZE /
  * WRITE_SIZE counters are calibrated. */
 int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes);
+/* Random gather of count 4-byte words, dst[i] = src[h(i, seed) mod src_words]
+ * (h: splitmix64).  No counterpart in the reference: the random-probe
+ * reference point for the same counters, where a 4-byte read brings in a
+ * whole line. */
+int sg_gather_device(sg_device_t dev, uint32_t* dst, const uint32_t* src, uint64_t src_words, size_t count,
+                     uint64_t seed);
 /* queue.prefetch of shared (managed) memory to the device, used by
  * memoryAlloc<T, space::shared> under ENABLE_PREFETCH (utils/memory.hpp:24-26).
  * A hint: memory the runtime cannot migrate is left where it is (SG_OK). */
@@ -264,7 +270,7 @@ typedef struct sg_bfs_options {
   int32_t t_medium;        /* degree from which a push vertex gets a wave (default 64) */
   int32_t t_large;         /* degree from which its arcs are split into chunks (default 1024) */
   int32_t chunk;           /* arcs per chunk of a large vertex (default 1024, at least 64) */
-  int32_t blocks_per_cu;   /* workgroups per compute unit of the traversal kernel (default 8) */
+  int32_t blocks_per_cu;   /* workgroups per compute unit of the traversal kernel (default 4) */
   int32_t burst_arcs;      /* push levels whose frontier has at most this many arcs run back to back
                               inside one single-workgroup launch (default 4096, at most 65536;
                               negative: never) */

@@ -122,6 +122,7 @@ _SIGS = {
     "sg_memcpy": (_I, [_P, _P, _P, C.c_size_t]),
     "sg_fill": (_I, [_P, _P, C.c_uint64, _I, C.c_size_t]),
     "sg_copy_device": (_I, [_P, _P, _P, C.c_size_t]),
+    "sg_gather_device": (_I, [_P, _P, _P, C.c_uint64, C.c_size_t, C.c_uint64]),
     "sg_prefetch": (_I, [_P, _P, C.c_size_t]),
     "sg_sssp_options_default": (None, [_P]),
     "sg_sssp_ex": (_I, [_P, C.c_int32, _P, _P, _P]),

@@ -87,133 +87,180 @@ __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, 
       variant);
 }
 
-/* Pull level.  A wave loads 64 visited words at once and compacts their
- * unvisited vertices in registers (popcount scan + bit select), so all 64 lanes
- * pull for real vertices; new frontier bits are gathered per word in LDS. */
+/* Pull level.  A wave takes a chunk of up to 64 visited words (fewer when
+ * the bitmap has fewer words than 64 per wave, so every wave has a chunk) and
+ * compacts the chunk's unvisited vertices in registers (popcount scan + bit
+ * select).  Each lane then probes the hints of kPullBatch of them at once --
+ * their hint entries loaded together, then their bitmap probes -- so a lane
+ * keeps kPullBatch dependent chains in flight instead of one.  A hint that
+ * hits settles its vertex; the misses go to a per-wave list in LDS and are
+ * pulled one per lane from their adjacency lists.  New frontier bits are
+ * gathered per word in LDS. */
+constexpr int kPullBatch = 4;
+
 template<typename OffT>
 __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
   zero_slots(a.stats_zero);
   /* arcs a lane scans alone before the wave helps: 16 / 4 / 64 */
   const int head_steps = variant == 1 ? 1 : (variant == 2 ? kPullHead : kPullHead / 4);
   __shared__ unsigned long long lds_new[kBlock];
+  /* a miss: its vertex and the hint's two words (list start / end, or out-degree) */
+  __shared__ uint32_t lds_miss[3][kBlock / 64][64 * kPullBatch];
   uint64_t st_disc = 0, st_deg = 0, st_edges = 0, st_scanned = 0, st_max = 0, st_miss = 0;
   const int lane = lane_id();
   unsigned long long* wnew = lds_new + (threadIdx.x & ~63);
+  uint32_t* wmiss = lds_miss[0][threadIdx.x >> 6];
+  uint32_t* wm_y = lds_miss[1][threadIdx.x >> 6];
+  uint32_t* wm_e = lds_miss[2][threadIdx.x >> 6];
   const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
   const int tail_bits = (int)(a.n & 63);
-  for (int64_t base = gwave * 64; base < a.w0; base += nwaves * 64) {
+  /* words per chunk: 64, halved while the chunks would leave waves idle (at least 8) */
+  int chunk = 64;
+  while (chunk > 8 && (a.w0 + chunk - 1) / chunk < nwaves) chunk >>= 1;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (int64_t base = gwave * chunk; base < a.w0; base += nwaves * chunk) {
     const int64_t myw = base + lane;
-    const uint64_t myvis = myw < a.w0 ? a.visited[myw] : ~0ull;
+    const bool mine = lane < chunk && myw < a.w0;
+    const uint64_t myvis = mine ? a.visited[myw] : ~0ull;
     uint64_t myun = ~myvis;
     if (myw == a.w0 - 1 && tail_bits) myun &= (1ull << tail_bits) - 1ull;
     const uint32_t cnt = (uint32_t)__popcll(myun);
     const uint32_t incl = wave_inclusive_scan(cnt);
     const uint32_t total = shfl(incl, 63);
     wnew[lane] = 0ull;
-    for (uint32_t t0 = 0; t0 < total; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      uint32_t lo = 0, hi = 63; /* owner word: first lane with incl > t */
+    for (uint32_t t0 = 0; t0 < total; t0 += 64 * kPullBatch) {
+      uint32_t vv[kPullBatch];
+      bool act[kPullBatch];
 #pragma unroll
-      for (int it = 0; it < 6; ++it) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (__shfl(incl, (int)mid, 64) > t) hi = mid;
-        else lo = mid + 1;
-      }
-      const int owner = (int)lo;
-      const uint32_t before = __shfl(incl - cnt, owner, 64);
-      const uint64_t un = shfl_u64(myun, owner);
-      const bool active = t < total;
-      const uint32_t bit = active ? select_bit(un, t - before) : 0u;
-      const int64_t v = (base + owner) * 64 + bit;
-      bool found = false, miss = false;
-      OffT b = 0, e = 0, j = 0;
-      int64_t kk = 0;
-      int32_t odeg = 0;
-      if (active) {
-        /* first probe: the in-neighbour of largest out-degree, read with the
-         * vertex's out-degree from one 8-byte entry; the offsets and the
-         * adjacency list are read only on a miss */
-        const int2 hd = a.hint[v];
-        uint32_t rb = 0, re = 0;
-        if (a.hint_offsets) { /* the list's bounds come with the hint, usually from the same line */
-          rb = (uint32_t)hd.y;
-          re = (uint32_t)a.hint[v + 1].y;
-          odeg = (int32_t)(re - rb);
-        } else {
-          odeg = hd.y;
+      for (int k = 0; k < kPullBatch; ++k) {
+        const uint32_t t = t0 + 64 * k + lane;
+        uint32_t lo = 0, hi = 63; /* owner word: first lane with incl > t */
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (__shfl(incl, (int)mid, 64) > t) hi = mid;
+          else lo = mid + 1;
         }
-        if (test_bit(a.cur_bm, (uint32_t)hd.x)) {
-          found = true;
-        } else {
-          miss = true;
-          if (a.hint_offsets) {
-            b = (OffT)rb;
-            e = (OffT)re;
+        const int owner = (int)lo;
+        const uint32_t before = __shfl(incl - cnt, owner, 64);
+        const uint64_t un = shfl_u64(myun, owner);
+        act[k] = t < total;
+        vv[k] = (uint32_t)((base + owner) * 64 + (act[k] ? select_bit(un, t - before) : 0u));
+      }
+      /* first probes: the in-neighbour of largest out-degree, read with the
+       * vertex's out-degree (or its list start) from one 8-byte entry */
+      int2 hd[kPullBatch];
+      uint32_t rend[kPullBatch];
+#pragma unroll
+      for (int k = 0; k < kPullBatch; ++k) {
+        hd[k] = act[k] ? a.hint[vv[k]] : int2{0, 0};
+        rend[k] = (act[k] && a.hint_offsets) ? (uint32_t)a.hint[vv[k] + 1].y : 0u;
+      }
+      uint32_t nmiss = 0;
+#pragma unroll
+      for (int k = 0; k < kPullBatch; ++k) {
+        const bool hit = act[k] && test_bit(a.cur_bm, (uint32_t)hd[k].x);
+        if (hit) {
+          const int32_t odeg = a.hint_offsets ? (int32_t)(rend[k] - (uint32_t)hd[k].y) : hd[k].y;
+          a.depth[vv[k]] = level + 1;
+          atomicOr(&wnew[(int64_t)(vv[k] >> 6) - base], 1ull << (vv[k] & 63));
+          ++st_scanned; /* active vertices have in-arcs: in-degree 0 is pre-marked visited */
+          ++st_edges;   /* the hint probe */
+          ++st_disc;
+          st_deg += odeg;
+          st_max = odeg > (int64_t)st_max ? odeg : st_max;
+        }
+        const uint64_t mm = ballot(act[k] && !hit);
+        if (act[k] && !hit) {
+          const uint32_t slot = nmiss + (uint32_t)__popcll(mm & below);
+          wmiss[slot] = vv[k];
+          wm_y[slot] = (uint32_t)hd[k].y;
+          wm_e[slot] = rend[k];
+        }
+        nmiss += (uint32_t)__popcll(mm);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+      /* the misses, one per lane: the adjacency list after the hint */
+      for (uint32_t m0 = 0; m0 < nmiss; m0 += 64) {
+        const bool active = m0 + lane < nmiss;
+        const uint32_t v = active ? wmiss[m0 + lane] : 0u;
+        const uint32_t hy = active ? wm_y[m0 + lane] : 0u, he = active ? wm_e[m0 + lane] : 0u;
+        bool found = false;
+        OffT b = 0, e = 0, j = 0;
+        int64_t kk = 0;
+        int32_t odeg = 0;
+        if (active) {
+          if (a.hint_offsets) { /* the list's bounds came with the hints */
+            b = (OffT)hy;
+            e = (OffT)he;
+            odeg = (int32_t)(he - hy);
           } else {
+            odeg = (int32_t)hy;
             b = a.in_off[v];
             e = a.in_off[v + 1];
           }
           j = b;
           kk = (int64_t)(e - b);
+          /* lane-serial head: four independent id loads then four
+           * independent bitmap probes per step */
+          for (int step = 0; step < head_steps && j < e; ++step, j += 4) {
+            const uint32_t u0 = (uint32_t)a.in_cols[j];
+            const uint32_t u1 = j + 1 < e ? (uint32_t)a.in_cols[j + 1] : u0;
+            const uint32_t u2 = j + 2 < e ? (uint32_t)a.in_cols[j + 2] : u0;
+            const uint32_t u3 = j + 3 < e ? (uint32_t)a.in_cols[j + 3] : u0;
+            const bool h0 = test_bit(a.cur_bm, u0), h1 = test_bit(a.cur_bm, u1);
+            const bool h2 = test_bit(a.cur_bm, u2), h3 = test_bit(a.cur_bm, u3);
+            if (h0 | h1 | h2 | h3) {
+              const int first = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
+              kk = (int64_t)(j - b) + first + 1;
+              found = true;
+              break;
+            }
+          }
         }
-        /* lane-serial head: four independent id loads then four independent
-         * bitmap probes per step */
-        for (int step = 0; step < head_steps && j < e; ++step, j += 4) {
-          const uint32_t u0 = (uint32_t)a.in_cols[j];
-          const uint32_t u1 = j + 1 < e ? (uint32_t)a.in_cols[j + 1] : u0;
-          const uint32_t u2 = j + 2 < e ? (uint32_t)a.in_cols[j + 2] : u0;
-          const uint32_t u3 = j + 3 < e ? (uint32_t)a.in_cols[j + 3] : u0;
-          const bool h0 = test_bit(a.cur_bm, u0), h1 = test_bit(a.cur_bm, u1);
-          const bool h2 = test_bit(a.cur_bm, u2), h3 = test_bit(a.cur_bm, u3);
-          if (h0 | h1 | h2 | h3) {
-            const int first = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
-            kk = (int64_t)(j - b) + first + 1;
+        /* cooperative tail: the whole wave scans the rest of each long list,
+         * 64 ids per step; the ballot's lowest set bit is the first hit */
+        uint64_t pending = ballot(active && !found && j < e);
+        while (pending) {
+          const int k2 = __ffsll((unsigned long long)pending) - 1;
+          pending &= pending - 1;
+          const int64_t jb = shfl_i64((int64_t)j, k2);
+          const int64_t je = shfl_i64((int64_t)e, k2);
+          const int64_t bb = shfl_i64((int64_t)b, k2);
+          int64_t hit_pos = -1;
+          for (int64_t jj = jb; jj < je; jj += 64) {
+            const int64_t my = jj + lane;
+            const bool h = my < je && test_bit(a.cur_bm, (uint32_t)a.in_cols[my]);
+            const uint64_t hm = ballot(h);
+            if (hm) {
+              hit_pos = jj + (__ffsll((unsigned long long)hm) - 1);
+              break;
+            }
+          }
+          if (lane == k2 && hit_pos >= 0) {
             found = true;
-            break;
+            kk = hit_pos - bb + 1;
+          }
+        }
+        if (active) {
+          ++st_scanned;
+          st_edges += 1 + kk; /* hint probe + list probes after the miss */
+          ++st_miss;
+          if (found) {
+            a.depth[v] = level + 1;
+            atomicOr(&wnew[(int64_t)(v >> 6) - base], 1ull << (v & 63));
+            ++st_disc;
+            st_deg += odeg;
+            st_max = odeg > (int64_t)st_max ? odeg : st_max;
           }
         }
       }
-      /* cooperative tail: the whole wave scans the rest of each long list,
-       * 64 ids per step; the ballot's lowest set bit is the first hit */
-      uint64_t pending = ballot(active && !found && j < e);
-      while (pending) {
-        const int k2 = __ffsll((unsigned long long)pending) - 1;
-        pending &= pending - 1;
-        const int64_t jb = shfl_i64((int64_t)j, k2);
-        const int64_t je = shfl_i64((int64_t)e, k2);
-        const int64_t bb = shfl_i64((int64_t)b, k2);
-        int64_t hit_pos = -1;
-        for (int64_t jj = jb; jj < je; jj += 64) {
-          const int64_t my = jj + lane;
-          const bool h = my < je && test_bit(a.cur_bm, (uint32_t)a.in_cols[my]);
-          const uint64_t hm = ballot(h);
-          if (hm) {
-            hit_pos = jj + (__ffsll((unsigned long long)hm) - 1);
-            break;
-          }
-        }
-        if (lane == k2 && hit_pos >= 0) {
-          found = true;
-          kk = hit_pos - bb + 1;
-        }
-      }
-      if (active) {
-        ++st_scanned; /* active vertices have in-arcs: in-degree 0 is pre-marked visited */
-        st_edges += 1 + kk; /* hint probe + list probes after a miss */
-        st_miss += miss ? 1 : 0;
-        if (found) {
-          a.depth[v] = level + 1;
-          atomicOr(&wnew[owner], 1ull << bit);
-          ++st_disc;
-          st_deg += odeg;
-          st_max = odeg > (int64_t)st_max ? odeg : st_max;
-        }
-      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     const uint64_t mynew = wnew[lane];
-    if (myw < a.w0) {
+    if (mine) {
       a.next_bm[myw] = mynew;
       if (mynew) a.visited[myw] = myvis | mynew;
     }
@@ -346,9 +393,11 @@ __global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, 
   *ctl = c;
 }
 
-/* The traversal of one level, push or pull as the control block says. */
+/* The traversal of one level, push or pull as the control block says.  Six
+ * waves per SIMD: the pull's batched probes need 80 registers. */
+constexpr int kLevelWavesPerSimd = 6;
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLevelWavesPerSimd))) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
   if (ctl->done) return;
   if (ctl->skip) { /* the level was traversed by the burst that ended the previous step */
     zero_slots(a.stats_zero);
@@ -1371,6 +1420,9 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   if (opt.policy != SG_POLICY_PUSH) SG_TRY(sg::graph_ensure_pull_hint(g));
   const int64_t n = g->n, w0 = ws->w0;
   const int grid_main = dev->cus * (opt.blocks_per_cu > 0 ? opt.blocks_per_cu : 8);
+  /* the level kernel's workgroups, all resident at once (a pull gives each wave a chunk of the bitmap);
+   * four per compute unit measured best on RMAT scale 24 (2 to 8 swept: 14 % less pull time than 8) */
+  const int grid_level = dev->cus * (opt.blocks_per_cu > 0 ? opt.blocks_per_cu : 4);
   const int grid_small = std::max(1, std::min(dev->cus * 4, (int)((w0 + 255) / 256)));
   Tuning tune = default_tuning();
   if (opt.t_medium > 0) tune.t_medium = opt.t_medium;
@@ -1496,7 +1548,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
       void* args[] = {&gba, &ws->ctl, &P, &stp, &ws->levlog, &ws->steplog};
       SG_HIP(launch_resident(reinterpret_cast<const void*>(gburst_kernel<OffT>), gb_grid, kGbThreads, args, s));
     }
-    level_kernel<OffT><<<grid_main, kBlock, 0, s>>>(la, ws->ctl);
+    level_kernel<OffT><<<grid_level, kBlock, 0, s>>>(la, ws->ctl);
     if (timed) SG_HIP(hipEventRecord(ws->ev_mid[r], s));
     FinishArgs fa{Q.blocks, Q.nblocks, Q.hdr + nxt, step, ws->levlog, ws->steplog, ws->arrive};
     count_dyn_kernel<OffT><<<Q.grid_tiles(), kBlock, 0, s>>>(ca, ws->ctl, la.stats, P, fa);

@@ -19,6 +19,21 @@ __global__ void __launch_bounds__(256) copy_kernel_u128(const uint4* __restrict_
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+/* Random 4-byte gather, dst[i] = src[h(i) mod src_words] with a splitmix64
+ * hash of (i, seed): the reference point for counters read on random probes,
+ * where each 4-byte read brings in a whole line.  Every index lies in
+ * [0, src_words). */
+__global__ void __launch_bounds__(256) gather_kernel_u32(const uint32_t* __restrict__ src, uint64_t src_words,
+                                                         uint32_t* __restrict__ dst, size_t n, uint64_t seed) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t z = (uint64_t)i + seed * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    dst[i] = src[z % src_words];
+  }
+}
+
 __global__ void fill_kernel_u8(uint8_t* p, uint8_t v, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -233,6 +248,18 @@ int sg_copy_device(sg_device_t dev, void* dst, const void* src, size_t bytes) {
            "sg_copy_device: pointers and size must be multiples of 16 bytes");
   const size_t n16 = bytes / 16;
   copy_kernel_u128<<<sg::grid_for(dev, (int64_t)n16, 256, 8), 256, 0, dev->stream>>>((const uint4*)src, (uint4*)dst, n16);
+  SG_HIP(hipGetLastError());
+  SG_HIP(hipStreamSynchronize(dev->stream));
+  return SG_OK;
+}
+
+int sg_gather_device(sg_device_t dev, uint32_t* dst, const uint32_t* src, uint64_t src_words, size_t count,
+                     uint64_t seed) {
+  SG_CHECK(dev, SG_ERR_INVALID, "sg_gather_device: null device");
+  if (count == 0) return SG_OK;
+  SG_CHECK(dst && src && src_words > 0, SG_ERR_INVALID, "sg_gather_device: null pointer or empty source");
+  gather_kernel_u32<<<sg::grid_for(dev, (int64_t)count, 256, 8), 256, 0, dev->stream>>>(src, src_words, dst, count,
+                                                                                      seed);
   SG_HIP(hipGetLastError());
   SG_HIP(hipStreamSynchronize(dev->stream));
   return SG_OK;

@@ -788,6 +788,8 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
   if (!directed) {
     SG_TRY(sg::graph_ensure_isolated(g));
     SG_HIP(hipMemcpyAsync(settled_all, g->isolated, 8 * ws->w0, hipMemcpyDeviceToDevice, s));
+  } else {
+    SG_TRY(sg::zero_async(dev, settled_all, 8 * ws->w0));
   }
   SG_TRY(sg::zero_async(dev, ws->relax_slots, 8 * 16 * kSlots));
   sssp_init_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(off, dist, unreached, n, src, Q.q[0][0],

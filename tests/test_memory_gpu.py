@@ -49,6 +49,27 @@ def test_copy_device_kernel(sg, device):
     assert sg.lib().sg_copy_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), 20) == sg.SG_ERR_INVALID
 
 
+def _splitmix(i, seed):
+    with np.errstate(over="ignore"):
+        z = i.astype(np.uint64) + np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def test_gather_device_kernel(sg, device):
+    """The random-gather calibration kernel reads src at splitmix64(i, seed)
+    mod src_words (the published splitmix64 finaliser, restated in numpy)."""
+    src = np.random.default_rng(4).integers(0, 2**32, size=1001, dtype=np.uint32)
+    a = device.to_device(src)
+    count = 5000
+    b = device.alloc(4 * count)
+    sg.check(sg.lib().sg_gather_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), src.size, count, 7))
+    idx = _splitmix(np.arange(count), 7) % np.uint64(src.size)
+    assert np.array_equal(b.download(np.uint32, count), src[idx.astype(np.int64)])
+    assert sg.lib().sg_gather_device(device.h, C.c_void_p(b.ptr), C.c_void_p(a.ptr), 0, count, 7) == sg.SG_ERR_INVALID
+
+
 def test_prefetch_shared_memory(sg, device):
     """sg_prefetch migrates managed memory to the device (a hint: memory it
     cannot migrate is accepted too) and leaves the contents intact."""
