@@ -278,8 +278,8 @@ typedef struct sg_bfs_options {
                               many run back to back inside one cooperative launch of the whole grid,
                               one grid barrier per level, once two steps in a row have left such a
                               level (default 262144, at most 2^24; negative: never) */
-  int32_t grid_burst_workgroups; /* 512-thread workgroups of that launch (default: one per compute unit;
-                                     at most 1024, and never more than fit on the device at once) */
+  int32_t grid_burst_workgroups; /* 512-thread workgroups of that launch (default 32; at most 1024, and
+                                     never more than fit on the device at once) */
   int32_t learned_gate;    /* SG_POLICY_LEARNED: a level whose frontier has fewer arcs than the graph's
                               arcs / learned_gate pushes with the bucketing mapping without asking the
                               network (default 1024; negative: the network decides every level) */

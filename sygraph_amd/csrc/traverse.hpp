@@ -891,6 +891,7 @@ inline hipError_t launch_resident(const void* fn, int grid, int block, void** ar
 constexpr int kBarGroups = 8;
 constexpr int kBarStride = 32; /* counters 128 bytes apart */
 constexpr uint32_t kBarTimeout = 1u << 31;
+constexpr int kBarFlat = 64; /* grids up to this size use one arrival counter */
 
 struct GridBar {
   uint32_t grp[kBarGroups * kBarStride];
@@ -922,10 +923,13 @@ __device__ __noinline__ bool grid_barrier(GridBar* c, uint32_t& t, uint64_t wait
     const uint32_t grid = gridDim.x, g = blockIdx.x % kBarGroups;
     const uint32_t ngroups = grid < (uint32_t)kBarGroups ? grid : (uint32_t)kBarGroups;
     const uint32_t in_group = (grid - g + kBarGroups - 1) / kBarGroups;
+    /* up to kBarFlat workgroups arrive on the top counter directly: one
+     * atomic round trip fewer, and few enough not to queue long on it */
+    const bool flat = grid <= (uint32_t)kBarFlat;
     bool released = false;
-    if (atomic_add(&c->grp[g * kBarStride], 1u) == t * in_group - 1) {
+    if (flat || atomic_add(&c->grp[g * kBarStride], 1u) == t * in_group - 1) {
       if (kFences) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      if (atomic_add(&c->top[0], 1u) == t * ngroups - 1) {
+      if (atomic_add(&c->top[0], 1u) == t * (flat ? grid : ngroups) - 1) {
         if (kFences) {
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
           __hip_atomic_store(&c->gen[0], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
