@@ -77,7 +77,6 @@ __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, 
         uint64_t w[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) w[k] = ok[k] ? a.visited[y[k] >> 6] : ~0ull;
-        uint64_t old[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint64_t bit = 1ull << (y[k] & 63);
@@ -365,18 +364,28 @@ __device__ __forceinline__ void stage_policy(const DecideParams& p, sg::policy::
   if (p.policy == SG_POLICY_LEARNED) sg::policy::stage(pol, p.mlp, (int)threadIdx.x, (int)blockDim.x);
 }
 
-/* Level-0 setup: depth[src] = 0, visited = prev = isolated | {src},
- * cur_bm = {src}, F0 queue = {src}, control block for level 0. */
+/* A traversal's start in one launch (each kernel boundary costs a few
+ * microseconds, a fifth of a small level): depth = n + 1 except depth[src] =
+ * 0, visited = prev = isolated | {src}, cur_bm = {src}, the stat slots
+ * cleared, and (thread 0) the F0 queue = {src} and the control block for
+ * level 0. */
 template<typename OffT>
-__global__ void init_source_kernel(BfsArgs<OffT> a, uint64_t* prev, QEntry* q0, QEntry* q1, QEntry* q2, QueueHdr* hdr,
-                                   Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir, int32_t first_lb,
-                                   int64_t burst_arcs) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void bfs_start_kernel(BfsArgs<OffT> a, uint64_t* prev, const uint64_t* isolated,
+                                 unsigned long long* stat_words, int64_t n_stat_words, QEntry* q0, QEntry* q1,
+                                 QEntry* q2, QueueHdr* hdr, Tuning t, int32_t src, BfsCtl* ctl, int32_t first_dir,
+                                 int32_t first_lb, int64_t burst_arcs) {
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t sw = src >> 6;
   const uint64_t bit = 1ull << (src & 63);
-  a.depth[src] = 0;
-  a.visited[src >> 6] |= bit;
-  prev[src >> 6] |= bit;
-  a.cur_bm[src >> 6] |= bit;
+  for (int64_t i = tid; i < a.n; i += stride) a.depth[i] = i == src ? 0 : (int32_t)(a.n + 1);
+  for (int64_t i = tid; i < a.w0; i += stride) {
+    const uint64_t x = isolated[i] | (i == sw ? bit : 0ull);
+    a.visited[i] = x;
+    prev[i] = x;
+    a.cur_bm[i] = i == sw ? bit : 0ull;
+  }
+  for (int64_t i = tid; i < n_stat_words; i += stride) stat_words[i] = 0ull;
+  if (tid != 0) return;
   QEntry* const q[3] = {q0, q1, q2};
   enqueue_single(a.off, t, q, hdr, src);
   const unsigned long long d = (unsigned long long)(a.off[src + 1] - a.off[src]);
@@ -1093,7 +1102,11 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
   if (ctl->done || ctl->skip || ctl->dir != 0 || ctl->f_edges > a.gb_arcs) return; /* the same for every workgroup */
   stage_policy(p, &s_pol);
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  const int64_t gwave = ((int64_t)blockIdx.x * kGbThreads + threadIdx.x) >> 6;
+  /* the waves' ranks run across the workgroups first (rank = wave * grid +
+   * workgroup): a level of a few thousand entries needs a few dozen waves,
+   * and ranked within a workgroup first they all sat on the first few
+   * compute units, queueing their memory requests there */
+  const int64_t gwave = (int64_t)(threadIdx.x >> 6) * gridDim.x + blockIdx.x;
   const int64_t nwaves = ((int64_t)gridDim.x * kGbThreads) >> 6;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   int32_t level = ctl->level, lb = ctl->lb;
@@ -1251,18 +1264,6 @@ __global__ void gb_reset_kernel(GridBar* bar, GbSums* sums) {
   for (int i = threadIdx.x; i < (int)(sizeof(GridBar) / 4); i += blockDim.x) b[i] = 0u;
   unsigned long long* z = reinterpret_cast<unsigned long long*>(sums);
   for (int i = threadIdx.x; i < (int)(3 * sizeof(GbSums) / 8); i += blockDim.x) z[i] = 0ull;
-}
-
-__global__ void copy_words_kernel(uint64_t* dst, uint64_t* dst2, const uint64_t* src, int64_t nw) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t x = src[i];
-    dst[i] = x;
-    dst2[i] = x;
-  }
-}
-
-__global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 /* Per-graph BFS workspace, allocated on first use and reused. */
@@ -1425,11 +1426,9 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   SG_TRY(sg::graph_ensure_isolated(g));
   if (opt.policy != SG_POLICY_PUSH) SG_TRY(sg::graph_ensure_pull_hint(g));
   const int64_t n = g->n, w0 = ws->w0;
-  const int grid_main = dev->cus * (opt.blocks_per_cu > 0 ? opt.blocks_per_cu : 8);
   /* the level kernel's workgroups, all resident at once (a pull gives each wave a chunk of the bitmap);
    * four per compute unit measured best on RMAT scale 24 (2 to 8 swept: 14 % less pull time than 8) */
   const int grid_level = dev->cus * (opt.blocks_per_cu > 0 ? opt.blocks_per_cu : 4);
-  const int grid_small = std::max(1, std::min(dev->cus * 4, (int)((w0 + 255) / 256)));
   Tuning tune = default_tuning();
   if (opt.t_medium > 0) tune.t_medium = opt.t_medium;
   if (opt.t_large > 0) tune.t_large = opt.t_large;
@@ -1456,10 +1455,6 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   a.depth = depth;
   a.cur_bm = ws->bm[0];
 
-  fill_i32_kernel<<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(depth, (int32_t)(n + 1), n);
-  copy_words_kernel<<<grid_small, 256, 0, s>>>(ws->visited, ws->prev, g->isolated, w0);
-  SG_TRY(sg::zero_async(dev, ws->bm[0], 8 * w0));
-  SG_TRY(sg::zero_async(dev, ws->stats, sizeof(StatSlot) * kSlots * 2));
   /* work-mapping variant under the fixed policies (see sg_bfs_options) */
   const int32_t lb_fixed = opt.load_balancer == SG_LB_WORKGROUP ? 1 : (opt.load_balancer == SG_LB_SUBGROUP ? 2 : 0);
   const int64_t burst_arcs = opt.burst_arcs < 0 ? 0 : (opt.burst_arcs == 0 ? kBurstDefaultArcs : opt.burst_arcs);
@@ -1477,9 +1472,10 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     gb_grid = std::min(opt.grid_burst_workgroups > 0 ? opt.grid_burst_workgroups : kGbDefaultWgs, dev->cus * occ);
     gb_grid = std::min(gb_grid, kGbMaxWgs);
   }
-  init_source_kernel<OffT><<<1, 64, 0, s>>>(a, ws->prev, Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
-                                            opt.policy == SG_POLICY_PULL ? 1 : 0,
-                                            opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed, burst_arcs);
+  bfs_start_kernel<OffT><<<sg::grid_for(dev, n, 256, 4), 256, 0, s>>>(
+      a, ws->prev, g->isolated, reinterpret_cast<unsigned long long*>(ws->stats),
+      (int64_t)(sizeof(StatSlot) * kSlots * 2 / 8), Q.q[0][0], Q.q[0][1], Q.q[0][2], Q.hdr, tune, src, ws->ctl,
+      opt.policy == SG_POLICY_PULL ? 1 : 0, opt.policy == SG_POLICY_LEARNED ? 0 : lb_fixed, burst_arcs);
   SG_HIP(hipGetLastError());
 
   const bool timed = opt.record_levels && stats;

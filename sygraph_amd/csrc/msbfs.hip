@@ -1008,7 +1008,9 @@ __global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
   __shared__ unsigned long long s_h[3][kLevelInfoZero];
   __shared__ unsigned long long s_full;
   Stage st{s_stage[threadIdx.x >> 6], 0u, 0ull, 0ull};
-  const int64_t gwave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  /* wave ranks run across the workgroups first, so a small level's few busy
+   * waves spread over as many compute units as there are of them */
+  const int64_t gwave = (int64_t)(threadIdx.x >> 6) * gridDim.x + blockIdx.x;
   const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
   const int64_t n = a.n;
   uint32_t t = 0;
