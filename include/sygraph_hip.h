@@ -272,7 +272,7 @@ typedef struct sg_bfs_options {
   int32_t chunk;           /* arcs per chunk of a large vertex (default 1024, at least 64) */
   int32_t blocks_per_cu;   /* workgroups per compute unit of the traversal kernel (default 4) */
   int32_t burst_arcs;      /* push levels whose frontier has at most this many arcs run back to back
-                              inside one single-workgroup launch (default 4096, at most 65536;
+                              inside one single-workgroup launch (default 2048, at most 65536;
                               negative: never) */
   int32_t grid_burst_arcs; /* push levels whose frontier has more arcs than burst_arcs but at most this
                               many run back to back inside one cooperative launch of the whole grid,
