@@ -44,6 +44,10 @@ def level_bytes(ls, w0_bytes: int, off_bytes: int = 8) -> float:
 
 
 def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, learned=None):
+    """The timed steps (one traversal per source each), then, with collect, one
+    more untimed pass over the sources with each step's kernels timed on their
+    own (an extra event per step, a stream barrier, kept out of the timed
+    steps) for the per-kernel figures."""
     b = sg.BFS(G)
     edges_total = 0
     # -> [bytes, ms, launches]; "step rest": the count / scan / write launches
@@ -55,30 +59,33 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
     for _ in range(warmup):
         for s in sources:
             b.init(s)
-            b.run(policy, alpha, beta, learned=learned)
+            b.run(policy, alpha, beta, learned=learned, record_levels=False)
     G.dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         for s in sources:
             b.init(s)
-            st = b.run(policy, alpha, beta, learned=learned)
+            st = b.run(policy, alpha, beta, learned=learned, record_levels=False)
             edges_total += st.reached_edges
             kernel_ms_total += st.kernel_ms
             if st.kernel_ms > 0:
                 per_trav.append(st.reached_edges / (st.kernel_ms / 1e3) / 1e9)
-            if collect:
-                for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
-                    ls = st.level[L]
-                    # levels run inside a burst are not level_kernel launches
-                    k = per_kernel["burst" if ls.burst else ls.direction]
-                    k[0] += level_bytes(ls, w0_bytes)
-                    k[1] += ls.kernel_ms
-                    k[2] += 1
-                    if not ls.burst:
-                        per_kernel["step rest"][1] += ls.convert_ms
-                        per_kernel["step rest"][2] += 1
     G.dev.synchronize()
     wall = time.perf_counter() - t0
+    if collect:
+        for s in sources:
+            b.init(s)
+            st = b.run(policy, alpha, beta, learned=learned, record_levels=True)
+            for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
+                ls = st.level[L]
+                # levels run inside a burst are not level_kernel launches
+                k = per_kernel["burst" if ls.burst else ls.direction]
+                k[0] += level_bytes(ls, w0_bytes)
+                k[1] += ls.kernel_ms
+                k[2] += 1
+                if not ls.burst:
+                    per_kernel["step rest"][1] += ls.convert_ms
+                    per_kernel["step rest"][2] += 1
     return edges_total, wall, kernel_ms_total, per_kernel, per_trav
 
 

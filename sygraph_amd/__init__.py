@@ -564,13 +564,15 @@ class BFS:
 
     def run(self, policy: int = POLICY_PUSH, alpha: float = 1.0, beta: float = 1.0,
             load_balancer: int = LB_BUCKETING, sync_per_level: bool = False,
-            learned: Optional["Policy"] = None, tuning: Optional[dict] = None) -> BfsStats:
+            learned: Optional["Policy"] = None, tuning: Optional[dict] = None,
+            record_levels: bool = True) -> BfsStats:
         """tuning: optional knobs t_medium, t_large, chunk, blocks_per_cu, burst_arcs, grid_burst_arcs,
-        grid_burst_workgroups, learned_gate."""
+        grid_burst_workgroups, learned_gate.  record_levels: time each step's traversal kernel and its
+        helpers on their own (stats.level[i].kernel_ms / convert_ms; one more event per step)."""
         if self.source is None:
             raise SygraphError(SG_ERR_STATE, "BFS instance not initialized")
-        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=True, sync_per_level=sync_per_level,
-                        learned=learned, tuning=tuning)
+        o = bfs_options(policy, load_balancer, alpha, beta, record_levels=record_levels,
+                        sync_per_level=sync_per_level, learned=learned, tuning=tuning)
         check(lib().sg_bfs(self.g.h, self.source, C.byref(o), C.c_void_p(self.depth.ptr), C.byref(self.stats)))
         return self.stats
 
