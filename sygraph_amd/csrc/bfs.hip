@@ -307,6 +307,18 @@ struct alignas(64) StepLog {
 
 constexpr int kRing = 8; /* steps in the log / event rings (> lookahead + 1) */
 
+/* A step's record, published once, by the step's last writer: every field,
+ * then (after a system-scope fence) the step number the host polls for, so
+ * a host that sees the number reads a whole record. */
+__device__ __forceinline__ void publish_step(StepLog* slot, StepLog sl) {
+  const int32_t stp = sl.step;
+  sl.step = -1;
+  *slot = sl;
+  __threadfence_system();
+  __hip_atomic_store(&slot->step, stp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __threadfence_system();
+}
+
 struct DecideParams {
   int32_t policy;
   float alpha, beta;
@@ -519,7 +531,7 @@ __device__ __forceinline__ void finish_level(const FinishArgs& f, BfsCtl* ctl, c
     sl.reached_v = (int64_t)reached;
     sl.reached_e = (int64_t)vdeg;
     sl.next_f_edges = (int64_t)disc_deg;
-    f.steplog[f.step % kRing] = sl;
+    publish_step(&f.steplog[f.step % kRing], sl);
   }
   __threadfence_system();
 }
@@ -751,7 +763,7 @@ __device__ __forceinline__ void burst_visit_big(const BurstArgs<OffT>& a, const 
  * decision itself. */
 template<typename OffT>
 __device__ __forceinline__ void burst_body(const BurstArgs<OffT>& a, BfsCtl* ctl, const DecideParams& p,
-                                           int32_t step, LevelLog* levlog, StepLog* steplog) {
+                                           int32_t step, LevelLog* levlog, StepLog* steplog, bool publish) {
   __shared__ sg::policy::Staged s_pol;
   __shared__ uint32_t s_count[2][2];
   __shared__ unsigned long long s_red[2][kBurstWaves][5];
@@ -871,7 +883,9 @@ __device__ __forceinline__ void burst_body(const BurstArgs<OffT>& a, BfsCtl* ctl
     sl.reached_v = (int64_t)reached;
     sl.reached_e = (int64_t)vdeg;
     sl.next_f_edges = f_edges; /* the level handed back, a push level */
-    steplog[step % kRing] = sl;
+    /* the step's record, unless the step's own count pass writes it later
+     * (a burst ahead of the step that hands a level back) */
+    if (publish || done) publish_step(&steplog[step % kRing], sl);
     __threadfence_system();
   }
 }
@@ -882,7 +896,7 @@ template<typename OffT>
 __global__ void __launch_bounds__(kBurstThreads) burst_kernel(BurstArgs<OffT> a, BfsCtl* ctl, DecideParams p,
                                                               int32_t step, LevelLog* levlog, StepLog* steplog) {
   if (ctl->done || !ctl->burst) return;
-  burst_body(a, ctl, p, step, levlog, steplog);
+  burst_body(a, ctl, p, step, levlog, steplog, false);
 }
 
 /* Queue write of a level that the next one pushes.  A workgroup of
@@ -901,7 +915,7 @@ __global__ void __launch_bounds__(kBurstThreads) write_dyn_kernel(CompactArgs<Of
       compact_write_body_at<OffT>(a, 0, (int64_t)blockIdx.x * 2 + (threadIdx.x >> 8), (int)(threadIdx.x & (kBlock - 1)));
   /* a burst level has at most burst_arcs arcs, so few waves write entries and publish them */
   if (!burst || !last_to_arrive<false>(arrive, wrote)) return;
-  burst_body(ba, ctl, p, step, levlog, steplog);
+  burst_body(ba, ctl, p, step, levlog, steplog, true);
 }
 
 /* ------------------------------------------------------- grid bursts --- */
@@ -1231,8 +1245,7 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
       sl.last_level = level;
       sl.done = 1;
       sl.fail = 1;
-      steplog[step % kRing] = sl;
-      __threadfence_system();
+      publish_step(&steplog[step % kRing], sl);
     }
     return;
   }
@@ -1258,7 +1271,9 @@ __global__ void __launch_bounds__(kGbThreads) gburst_kernel(GbArgs<OffT> a, BfsC
   sl.reached_v = (int64_t)reached;
   sl.reached_e = (int64_t)vdeg;
   sl.next_f_edges = f_edges;
-  steplog[step % kRing] = sl;
+  /* a level handed back is counted and decided by this step's count pass,
+   * which writes the step's record */
+  if (done) publish_step(&steplog[step % kRing], sl);
   __threadfence_system();
 }
 
@@ -1420,6 +1435,26 @@ void bfs_workspace_release(sg_graph_t g) {
  * level.  A step runs one level, plus the levels of a burst when the next
  * frontiers are small.  Steps enqueued beyond the last level find `done` set
  * and return at once. */
+/* Wait until the device has published step `step`'s record (publish_step),
+ * spinning on the pinned record.  Should the stream drain without it (a
+ * kernel that did not run), the record is checked once more and the wait
+ * fails instead of spinning on. */
+int wait_step_record(const StepLog* rec, int32_t step, hipStream_t s) {
+  for (uint64_t it = 1;; ++it) {
+    if (__atomic_load_n(&rec->step, __ATOMIC_ACQUIRE) == step) return SG_OK;
+    if ((it & 4095) == 0) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(&rec->step, __ATOMIC_ACQUIRE) == step) return SG_OK;
+        sg::set_error("sg_bfs: the stream finished without the record of step %d", step);
+        return SG_ERR_STATE;
+      }
+      if (q != hipErrorNotReady) SG_HIP(q);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 template<typename OffT>
 int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats) {
   sg_device_t dev = g->dev;
@@ -1560,10 +1595,15 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     write_dyn_kernel<OffT><<<(Q.grid_tiles() + 1) / 2, kBurstThreads, 0, s>>>(ca, ba, ws->ctl, P, step, ws->levlog,
                                                                              ws->steplog, ws->arrive + kArriveWords);
     SG_HIP(hipGetLastError());
-    SG_HIP(hipEventRecord(ws->ev_done[r], s));
+    /* the host waits for an untimed step by polling its record: an event per
+     * step is a stream barrier, a few microseconds of every level */
+    if (timed) SG_HIP(hipEventRecord(ws->ev_done[r], s));
     return SG_OK;
   };
 
+  /* no record of an earlier traversal may pass for one of this one (the
+   * stream is idle: the previous call ended with a synchronize) */
+  for (int k = 0; k < kRing; ++k) __atomic_store_n(&ws->steplog[k].step, -1, __ATOMIC_RELEASE);
   /* a start in a burst: the levels from the source while they stay small,
    * handed back to step 0 (which then only counts and decides) or ending the
    * traversal; the kernel returns at once unless init set ctl->burst */
@@ -1579,7 +1619,8 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   int32_t small_push = 0;
   for (int32_t step = 0;; ++step) {
     const int r = step % kRing;
-    SG_HIP(hipEventSynchronize(ws->ev_done[r]));
+    if (timed) SG_HIP(hipEventSynchronize(ws->ev_done[r]));
+    else SG_TRY(wait_step_record(&ws->steplog[r], step, s));
     std::memcpy(&last, (const void*)&ws->steplog[r], sizeof(last));
     if (last.fail) {
       SG_HIP(hipStreamSynchronize(s));
