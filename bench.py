@@ -183,7 +183,7 @@ def batched_vs_plain(sg, G, sources16):
         e = 0
         for s in srcs:
             b.init(s)
-            st = b.run(pol, 15.0, 24.0)
+            st = b.run(pol, 15.0, 24.0, record_levels=False)
             e += st.reached_edges
             if name == "plain_push" and s in (srcs[0], srcs[-1]):
                 rows[s] = b.get_distances()

@@ -1435,26 +1435,6 @@ void bfs_workspace_release(sg_graph_t g) {
  * level.  A step runs one level, plus the levels of a burst when the next
  * frontiers are small.  Steps enqueued beyond the last level find `done` set
  * and return at once. */
-/* Wait until the device has published step `step`'s record (publish_step),
- * spinning on the pinned record.  Should the stream drain without it (a
- * kernel that did not run), the record is checked once more and the wait
- * fails instead of spinning on. */
-int wait_step_record(const StepLog* rec, int32_t step, hipStream_t s) {
-  for (uint64_t it = 1;; ++it) {
-    if (__atomic_load_n(&rec->step, __ATOMIC_ACQUIRE) == step) return SG_OK;
-    if ((it & 4095) == 0) {
-      const hipError_t q = hipStreamQuery(s);
-      if (q == hipSuccess) {
-        if (__atomic_load_n(&rec->step, __ATOMIC_ACQUIRE) == step) return SG_OK;
-        sg::set_error("sg_bfs: the stream finished without the record of step %d", step);
-        return SG_ERR_STATE;
-      }
-      if (q != hipErrorNotReady) SG_HIP(q);
-    }
-    __builtin_ia32_pause();
-  }
-}
-
 template<typename OffT>
 int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats) {
   sg_device_t dev = g->dev;
@@ -1620,7 +1600,7 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
   for (int32_t step = 0;; ++step) {
     const int r = step % kRing;
     if (timed) SG_HIP(hipEventSynchronize(ws->ev_done[r]));
-    else SG_TRY(wait_step_record(&ws->steplog[r], step, s));
+    else SG_TRY(sg::wait_host_flag(&ws->steplog[r].step, step, s, "sg_bfs"));
     std::memcpy(&last, (const void*)&ws->steplog[r], sizeof(last));
     if (last.fail) {
       SG_HIP(hipStreamSynchronize(s));

@@ -158,6 +158,10 @@ inline int grid_for(sg_device_t dev, int64_t items, int block, int max_blocks_pe
 
 int alloc_device(sg_device_t dev, void** p, size_t bytes);
 int zero_async(sg_device_t dev, void* p, size_t bytes);
+/* Wait until the device has stored `want` into *flag (pinned host memory),
+ * spinning on it; should the stream drain without it, fail (SG_ERR_STATE,
+ * "who" in the message) instead of spinning on. */
+int wait_host_flag(const int32_t* flag, int32_t want, hipStream_t s, const char* who);
 
 /* graph helpers */
 int graph_ensure_isolated(sg_graph_t g);

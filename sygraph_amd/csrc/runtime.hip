@@ -82,6 +82,22 @@ int zero_async(sg_device_t dev, void* p, size_t bytes) {
   return SG_OK;
 }
 
+int wait_host_flag(const int32_t* flag, int32_t want, hipStream_t s, const char* who) {
+  for (uint64_t it = 1;; ++it) {
+    if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == want) return SG_OK;
+    if ((it & 4095) == 0) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == want) return SG_OK;
+        set_error("%s: the stream finished without the record %d", who, want);
+        return SG_ERR_STATE;
+      }
+      if (q != hipErrorNotReady) SG_HIP(q);
+    }
+    __builtin_ia32_pause();
+  }
+}
+
 } // namespace sg
 
 namespace sg {

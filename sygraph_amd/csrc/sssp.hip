@@ -597,7 +597,13 @@ __global__ void __launch_bounds__(kBlock) sssp_select_kernel(CompactArgs<OffT> c
   e.relax = ctl->relax;
   e.rounds = ctl->rounds;
   e.relaxed = ctl->relaxed;
-  sa.log[iter % kRing] = e;
+  /* every field, then (after a system-scope fence) the iteration number the
+   * host polls for */
+  RoundLog* slot = &sa.log[iter % kRing];
+  e.iter = -1;
+  *slot = e;
+  __threadfence_system();
+  __hip_atomic_store(&slot->iter, iter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   __threadfence_system();
 }
 
@@ -867,9 +873,12 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
     if (record_relax) SG_HIP(hipEventRecord(ws->ev_pre[it % kRing], s));
     relax_kernel<OffT, W><<<grid_main, kBlock, 0, s>>>(a, ws->ctl, ws->relax_slots);
     SG_HIP(hipGetLastError());
-    SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
+    /* untimed iterations record no event (a stream barrier per iteration):
+     * the host polls the iteration's record, written by its selection */
+    if (record_relax) SG_HIP(hipEventRecord(ws->ev[it % kRing], s));
     return SG_OK;
   };
+  for (int k = 0; k < kRing; ++k) __atomic_store_n(&ws->log[k].iter, -1, __ATOMIC_RELEASE);
   SG_TRY(enqueue(0));
   SG_TRY(enqueue(1));
   int32_t round = 0;
@@ -878,7 +887,8 @@ int run_sssp(sg_graph_t g, int32_t src, double delta, int32_t heavy_pull, int64_
   int32_t relax_launches = 0;
   const int64_t max_iter = 4 * n + 256; /* every iteration relaxes or advances T past a distance */
   for (int32_t it = 0;; ++it) {
-    SG_HIP(hipEventSynchronize(ws->ev[it % kRing]));
+    if (record_relax) SG_HIP(hipEventSynchronize(ws->ev[it % kRing]));
+    else SG_TRY(sg::wait_host_flag(&ws->log[it % kRing].iter, it, s, "sg_sssp"));
     RoundLog e;
     std::memcpy(&e, (const void*)&ws->log[it % kRing], sizeof(e));
     if (e.iter != it) {
