@@ -409,13 +409,34 @@ __device__ __forceinline__ void stage_finish(const MsArgs<OffT>& a, LevelInfo* n
   st.arcs = st.mx = 0;
 }
 
-/* New lanes for v from a frontier mask fm: OR them into M_{d+1}[v]; true when
- * this call was the first to give v any lane at depth d+1. */
-__device__ __forceinline__ bool relax(uint64_t fm, int32_t v, const uint64_t* vis, const uint64_t* Mc, uint64_t* Mn) {
-  const uint64_t add = fm & ~(cld(&vis[v]) | cld(&Mc[v]));
-  if (!add) return false;
-  if ((cld(&Mn[v]) & add) == add) return false;
-  return atomic_or(&Mn[v], add) == 0ull;
+/* New lanes for a lane's four heads v[k] (< 0: none) from frontier masks
+ * fm[k]: the lanes not in vis or M_d are ORed into M_{d+1}[v]; first[k]: this
+ * OR was the first to give v[k] any lane at depth d+1.  The loads of the four
+ * arcs are in flight together, one round trip per stage: vis and M_d of every
+ * head, then M_{d+1} of the heads that would gain lanes (most arcs of a
+ * power-law graph stop at the first stage), then the ORs that still add. */
+__device__ __forceinline__ void relax4(const uint64_t (&fm)[4], const int32_t (&v)[4], const uint64_t* vis,
+                                       const uint64_t* Mc, uint64_t* Mn, bool (&first)[4]) {
+  uint64_t x[4], y[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = v[k] >= 0 ? cld(&vis[v[k]]) : ~0ull;
+    y[k] = v[k] >= 0 ? cld(&Mc[v[k]]) : ~0ull;
+  }
+  uint64_t add[4], z[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    add[k] = fm[k] & ~(x[k] | y[k]);
+    z[k] = add[k] ? cld(&Mn[v[k]]) : ~0ull;
+  }
+  uint64_t old[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    old[k] = ~0ull;
+    if (add[k] && (z[k] & add[k]) != add[k]) old[k] = atomic_or(&Mn[v[k]], add[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) first[k] = old[k] == 0ull;
 }
 
 /* A wave's walk over the (entry, lane) pairs of 64 entry masks (lane i holds
@@ -527,8 +548,7 @@ __device__ __forceinline__ void expand_push(const MsArgs<OffT>& a, int32_t d, in
         v[k] = t < total ? a.cols[s0 + (t - before)] : -1;
       }
       bool first[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(f[k], v[k], a.vis, Mc, Mn);
+      relax4(f, v, a.vis, Mc, Mn, first);
 #pragma unroll
       for (int k = 0; k < 4; ++k) stage_add(a, nx, nb, stg, first[k], v[k]);
     }
@@ -554,8 +574,8 @@ __device__ __forceinline__ void expand_chunks(const MsArgs<OffT>& a, const HubC*
         v[k] = j < e ? a.cols[j] : -1;
       }
       bool first[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) first[k] = v[k] >= 0 && relax(fm, v[k], a.vis, Mc, Mn);
+      const uint64_t f4[4] = {fm, fm, fm, fm};
+      relax4(f4, v, a.vis, Mc, Mn, first);
 #pragma unroll
       for (int k = 0; k < 4; ++k) stage_add(a, nx, nb, st, first[k], v[k]);
     }
@@ -1000,8 +1020,11 @@ __device__ __forceinline__ void load_head(unsigned long long* dst, const LevelIn
   if (k >= 0 && k < kLevelInfoZero) dst[k] = atomic_load(reinterpret_cast<const unsigned long long*>(L) + k);
 }
 
+/* three waves per SIMD: the kernel holds a level loop's state, the pair
+ * walks and four arcs' loads in flight; left alone the compiler takes 176
+ * registers (two waves), which costs the power-law graphs' big levels 7 % */
 template<typename OffT>
-__global__ void __launch_bounds__(kBlock) ms_kernel(MsArgs<OffT> a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) ms_kernel(MsArgs<OffT> a) {
   __shared__ sg::policy::Staged s_pol;
   __shared__ int32_t s_stage[kBlock / 64][kStage];
   __shared__ unsigned long long s_red[2 * (kBlock / 64)];
