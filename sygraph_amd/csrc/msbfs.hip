@@ -132,6 +132,12 @@ struct Ctl {
   /* on a line of its own: atomics on the polled flags line queue behind the polls */
   unsigned long long full_deg;   /* out-degrees of the vertices every lane has reached */
   uint32_t pad1[kBarStride - 2];
+  /* a level's arcs and largest degree, added by the workgroups that discover
+   * it into eight lines (workgroup % 8) instead of the list counter's line,
+   * three rotating copies (level % 3) */
+  struct alignas(128) LevelAcc {
+    unsigned long long arcs, mx, pad[14];
+  } acc[3][kBarGroups];
 };
 static_assert(sizeof(Ctl) % 8 == 0, "Ctl is cleared in 4-byte words");
 
@@ -350,13 +356,11 @@ struct Stage {
   uint64_t arcs, mx; /* per lane */
 };
 
+/* Place the wave's staged ids at list positions base, base + 1, ... */
 template<typename OffT>
-__device__ __forceinline__ void stage_flush(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st) {
-  if (st.n == 0) return;
+__device__ __forceinline__ void stage_place(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st,
+                                            unsigned long long base) {
   const int lane = lane_id();
-  unsigned long long base = 0;
-  if (lane == 0) base = atomic_add(&nx->count, (unsigned long long)st.n);
-  base = (unsigned long long)shfl_i64((int64_t)base, 0);
   for (uint32_t i0 = 0; i0 < st.n; i0 += 64) {
     const uint32_t i = i0 + lane;
     const bool ok = i < st.n;
@@ -371,7 +375,17 @@ __device__ __forceinline__ void stage_flush(const MsArgs<OffT>& a, LevelInfo* nx
   st.n = 0;
 }
 
-/* stage the vertices this wave touched first; all lanes call */
+/* A full stage in the middle of a phase: one reservation for the wave. */
+template<typename OffT>
+__device__ __forceinline__ void stage_flush(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st) {
+  if (st.n == 0) return;
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomic_add(&nx->count, (unsigned long long)st.n);
+  base = (unsigned long long)shfl_i64((int64_t)base, 0);
+  stage_place(a, nx, b, st, base);
+}
+
+/* Stage the lanes' first-touched vertices (all lanes call). */
 template<typename OffT>
 __device__ __forceinline__ void stage_add(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st, bool first,
                                           int32_t v) {
@@ -383,14 +397,31 @@ __device__ __forceinline__ void stage_add(const MsArgs<OffT>& a, LevelInfo* nx, 
   st.n += c;
 }
 
-/* End of a phase: flush every wave, then one add (arcs) and one max (largest
+/* End of a phase: the workgroup's waves place what they still stage under
+ * one reservation (one atomic on the list counter per workgroup, not per
+ * wave: on a level of a few hundred thousand entries spread over every wave
+ * those atomics queue on one line), then one add (arcs) and one max (largest
  * degree) per workgroup.  Every thread of the workgroup calls. */
 template<typename OffT>
 __device__ __forceinline__ void stage_finish(const MsArgs<OffT>& a, LevelInfo* nx, const Bases& b, Stage& st,
-                                             unsigned long long* s_red) {
-  stage_flush(a, nx, b, st);
-  const uint64_t arcs = wave_sum_u64(st.arcs), mx = wave_max_u64(st.mx);
+                                             unsigned long long* s_red, Ctl::LevelAcc* acc) {
+  __shared__ unsigned long long s_at[kBlock / 64];
   const int wv = threadIdx.x >> 6;
+  if (lane_id() == 0) s_at[wv] = st.n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      const unsigned long long c = s_at[w];
+      s_at[w] = tot;
+      tot += c;
+    }
+    const unsigned long long base = tot ? atomic_add(&nx->count, tot) : 0ull;
+    for (int w = 0; w < kBlock / 64; ++w) s_at[w] += base;
+  }
+  __syncthreads();
+  if (st.n) stage_place(a, nx, b, st, s_at[wv]);
+  const uint64_t arcs = wave_sum_u64(st.arcs), mx = wave_max_u64(st.mx);
   if (lane_id() == 0) {
     s_red[2 * wv] = arcs;
     s_red[2 * wv + 1] = mx;
@@ -402,8 +433,9 @@ __device__ __forceinline__ void stage_finish(const MsArgs<OffT>& a, LevelInfo* n
       t += s_red[2 * w];
       x = s_red[2 * w + 1] > x ? s_red[2 * w + 1] : x;
     }
-    if (t) atomic_add(&nx->arcs, t);
-    if (x) atomic_max(&nx->mx, x);
+    Ctl::LevelAcc* my = acc + (blockIdx.x % kBarGroups);
+    if (t) atomic_add(&my->arcs, t);
+    if (x) atomic_max(&my->mx, x);
   }
   __syncthreads();
   st.arcs = st.mx = 0;
@@ -1030,6 +1062,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
   __shared__ unsigned long long s_red[2 * (kBlock / 64)];
   __shared__ unsigned long long s_h[3][kLevelInfoZero];
   __shared__ unsigned long long s_full;
+  __shared__ unsigned long long s_acc[2 * kBarGroups];
   Stage st{s_stage[threadIdx.x >> 6], 0u, 0ull, 0ull};
   /* wave ranks run across the workgroups first, so a small level's few busy
    * waves spread over as many compute units as there are of them */
@@ -1045,6 +1078,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     load_head(s_h[0], L, 0);
     if (d >= 2) load_head(s_h[1], level(a, d - 2), 32);
     if (threadIdx.x == 63) s_full = atomic_load(&a.ctl->full_deg);
+    if (threadIdx.x >= 64 && threadIdx.x < 64 + 2 * kBarGroups) { /* the level's arcs and largest degree */
+      const int k = threadIdx.x - 64;
+      const Ctl::LevelAcc& c = a.ctl->acc[d % 3][k >> 1];
+      s_acc[k] = atomic_load((k & 1) ? &c.mx : &c.arcs);
+    }
     __syncthreads();
     const unsigned long long* h = s_h[0];
     const int64_t cnt = (int64_t)h[kHCount];
@@ -1075,6 +1113,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
       LevelInfo* z = level(a, d + 2); /* the level after next starts from zero */
       unsigned long long* zp = reinterpret_cast<unsigned long long*>(z);
       for (int k = 0; k < kLevelInfoZero; ++k) atomic_store(zp + k, 0ull);
+      for (int k = 0; k < kBarGroups; ++k) { /* read by every workgroup at level d - 1, all past its barrier */
+        atomic_store(&a.ctl->acc[(d + 2) % 3][k].arcs, 0ull);
+        atomic_store(&a.ctl->acc[(d + 2) % 3][k].mx, 0ull);
+      }
       atomic_store(&nx->base, (unsigned long long)nb.l);
       atomic_store(&nx->hov_base, (unsigned long long)nb.hov);
       atomic_store(&nx->hoc_base, (unsigned long long)nb.hoc);
@@ -1084,7 +1126,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
     const int64_t hoc = (int64_t)h[kHHoc];
     max_chunks = hoc > max_chunks ? hoc : max_chunks;
     const HubC* hocp = a.hoc + (a.ring ? (int64_t)(d & 3) * a.hc_cap : (int64_t)h[kHHocBase]);
-    dir = ms_direction(a, &s_pol, d, dir, cnt, h[kHArcs], h[kHMx], s_full);
+    unsigned long long lv_arcs = 0, lv_mx = 0;
+    for (int k = 0; k < kBarGroups; ++k) {
+      lv_arcs += s_acc[2 * k];
+      lv_mx = s_acc[2 * k + 1] > lv_mx ? s_acc[2 * k + 1] : lv_mx;
+    }
+    if (d == 0) { /* the sources' arcs, counted by the seeding */
+      lv_arcs = h[kHArcs];
+      lv_mx = h[kHMx];
+    }
+    dir = ms_direction(a, &s_pol, d, dir, cnt, lv_arcs, lv_mx, s_full);
     n_pull += dir;
     const int64_t o_cnt = d >= 2 ? (int64_t)s_h[1][kHCount] : 0;
     const int64_t o_base = d >= 2 ? (a.ring ? (int64_t)((d - 2) & 3) * a.lcap : (int64_t)s_h[1][kHBase]) : 0;
@@ -1103,7 +1154,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))
       expand_push(a, d, base, cnt, Mc, Mn, nx, nb, st, gwave, nwaves);
       expand_chunks(a, hocp, hoc, Mc, Mn, nx, nb, st, gwave, nwaves);
     }
-    stage_finish(a, nx, nb, st, s_red);
+    stage_finish(a, nx, nb, st, s_red, a.ctl->acc[(d + 1) % 3]);
     visit_level(a, d, base, cnt, Mc, gwave, nwaves);
     if (d >= 2) clear_level(a, o_base, o_cnt, a.M + (int64_t)((d + 2) & 3) * n, gwave, nwaves);
     if (a.bc && d >= 1) {
