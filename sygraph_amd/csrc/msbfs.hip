@@ -665,33 +665,33 @@ template<typename OffT>
 __device__ __forceinline__ void visit_level(const MsArgs<OffT>& a, int32_t d, int64_t base, int64_t cnt,
                                             const uint64_t* Mc, int64_t gwave, int64_t nwaves) {
   const int lane = lane_id();
+  uint64_t fdeg = 0; /* arcs of the vertices this lane saw every lane reach */
   for (int64_t g = gwave; g * 64 < cnt; g += nwaves) {
     const int64_t i = g * 64 + lane;
-    int32_t u = 0;
-    uint64_t m = 0;
-    uint64_t fdeg = 0;
-    if (i < cnt) {
-      u = cld(&a.lv[base + i]);
-      m = cld(&Mc[u]);
-      const uint64_t old = cld(&a.vis[u]);
-      cst(&a.vis[u], old | m);
-      if (a.bc) cst(&a.lm[base + i], m);
-      if (a.dir_policy >= SG_POLICY_HYBRID && old != a.full && (old | m) == a.full)
-        fdeg = (uint64_t)(a.off[u + 1] - a.off[u]);
+    if (i >= cnt) continue;
+    const int32_t u = cld(&a.lv[base + i]);
+    const uint64_t m = cld(&Mc[u]);
+    const uint64_t old = cld(&a.vis[u]);
+    cst(&a.vis[u], old | m);
+    if (a.bc) {
+      cst(&a.lm[base + i], m);
+    } else {
+      /* one store per source that reached u at this depth (a sparse mask
+       * costs its own bits, not the wave's union) */
+      for (uint64_t r = m; r; r &= r - 1) a.depth[(int64_t)(__ffsll((unsigned long long)r) - 1) * a.n + u] = d;
     }
-    if (a.dir_policy >= SG_POLICY_HYBRID) {
-      fdeg = wave_sum_u64(fdeg);
-      if (lane == 0 && fdeg) atomic_add(&a.ctl->full_deg, (unsigned long long)fdeg);
-    }
-    if (!a.bc) {
-      uint64_t w = m;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) w |= shfl_u64(w, lane ^ o);
-      while (w) { /* one store per source row for the wave's entries */
-        const int s = __ffsll((unsigned long long)w) - 1;
-        w &= w - 1;
-        if ((m >> s) & 1ull) a.depth[(int64_t)s * a.n + u] = d;
-      }
+    if (a.dir_policy >= SG_POLICY_HYBRID && old != a.full && (old | m) == a.full)
+      fdeg += (uint64_t)(a.off[u + 1] - a.off[u]);
+  }
+  if (a.dir_policy >= SG_POLICY_HYBRID) { /* one add per workgroup and level */
+    __shared__ unsigned long long s_fd[kBlock / 64];
+    fdeg = wave_sum_u64(fdeg);
+    if (lane == 0) s_fd[threadIdx.x >> 6] = fdeg;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int w = 0; w < kBlock / 64; ++w) t += s_fd[w];
+      if (t) atomic_add(&a.ctl->full_deg, t);
     }
   }
 }
