@@ -239,7 +239,8 @@ def _directed_rmat(oracle_mod, scale, ef, seed):
 
 @pytest.mark.parametrize("graph", ["grid", "path", "rmat", "rmat_directed"])
 @pytest.mark.parametrize("policy", ["push", "hybrid"])
-def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy):
+@pytest.mark.parametrize("timed", [True, False])
+def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy, timed):
     """Small push levels run back to back in one single-workgroup launch; every
     threshold (off, tiny, default, maximal) gives the oracle's depths and the
     same per-level records as running every level on its own."""
@@ -257,7 +258,7 @@ def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph,
         for burst in (-1, 64, 0, 65536):
             b = sg.BFS(G)
             b.init(src)
-            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": -1})
+            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": -1}, record_levels=timed)
             assert np.array_equal(b.get_distances(), ref), (src, burst)
             assert st.levels == levels
             lv = [(st.level[i].direction, st.level[i].frontier_vertices, st.level[i].frontier_edges,
@@ -276,14 +277,17 @@ def test_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph,
 
 @pytest.mark.parametrize("graph", ["grid", "path", "rmat", "rmat_directed", "star_chain"])
 @pytest.mark.parametrize("policy", ["push", "hybrid"])
-def test_grid_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy):
+@pytest.mark.parametrize("timed", [True, False])
+def test_grid_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, graph, policy, timed):
     """Push levels between the single-workgroup burst's limit and
     grid_burst_arcs run back to back in one cooperative launch with a grid
     barrier per level.  Every combination of thresholds gives the oracle's
     depths and the per-level records of running each level on its own; on the
     high-diameter graphs most levels run inside grid bursts when the
     single-workgroup bursts are off.  star_chain: hubs of 3,000 arcs on a long
-    path, so grid bursts cut long lists into pieces."""
+    path, so grid bursts cut long lists into pieces.  timed: each step timed
+    by events (the host waits on them) or not (the host polls each step's
+    record, published once by the step's last writer)."""
     def star_chain():
         hubs, leaves = 40, 3000
         u, v = [], []
@@ -311,7 +315,7 @@ def test_grid_bursts_leave_depths_and_levels_unchanged(sg, device, oracle_mod, g
         for burst, grid in configs:
             b = sg.BFS(G)
             b.init(src)
-            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": grid})
+            st = b.run(pol, 15.0, 24.0, tuning={"burst_arcs": burst, "grid_burst_arcs": grid}, record_levels=timed)
             assert np.array_equal(b.get_distances(), ref), (src, burst, grid)
             assert st.levels == levels
             lv = [(st.level[i].direction, st.level[i].frontier_vertices, st.level[i].frontier_edges,

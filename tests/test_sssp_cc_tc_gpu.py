@@ -40,14 +40,18 @@ def test_sssp_weighted_directed_known_answer(sg, device, oracle_mod, weight_floa
     assert s.get_distances().tolist() == [6, 6, 6, 6, 0]
 
 
-def test_sssp_rmat14_golden(sg, device, oracle_mod):
+@pytest.mark.parametrize("record_relax", [False, True])
+def test_sssp_rmat14_golden(sg, device, oracle_mod, record_relax):
+    """record_relax: each relaxing launch timed by events, the host waiting on
+    them; otherwise the host polls each iteration's record."""
     z = np.load(os.path.join(GOLDEN, "rmat14_seed1.npz"))
     G = sg.Graph.rmat(device, 14, 16, seed=1, weight_seed=3)
     s = sg.SSSP(G)
     s.init(int(z["source"]))
-    st = s.run()
+    st = s.run(record_relax=record_relax)
     assert np.array_equal(s.get_distances(), z["sssp"])
     assert st.reached_vertices == int((z["sssp"] != G.n + 1).sum())
+    assert (st.relax_launches > 0 and st.relax_ms > 0) == record_relax
 
 
 @pytest.mark.parametrize("lb", [0, 2, 3])
