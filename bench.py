@@ -60,22 +60,25 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
         for s in sources:
             b.init(s)
             b.run(policy, alpha, beta, learned=learned, record_levels=False)
+    # a step: one traversal per source, back to back on the device (sg_bfs_many);
+    # the rows stay on the device
+    rows = G.dev.alloc(4 * len(sources) * G.n)
+    sg.bfs_many(G, sources, policy, alpha, beta, out=rows, learned=learned)
     G.dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        for s in sources:
-            b.init(s)
-            st = b.run(policy, alpha, beta, learned=learned, record_levels=False)
-            edges_total += st.reached_edges
-            kernel_ms_total += st.kernel_ms
-            if st.kernel_ms > 0:
-                per_trav.append(st.reached_edges / (st.kernel_ms / 1e3) / 1e9)
+        st = sg.bfs_many(G, sources, policy, alpha, beta, out=rows, learned=learned)
+        edges_total += st.reached_edges
+        kernel_ms_total += st.kernel_ms
     G.dev.synchronize()
     wall = time.perf_counter() - t0
+    rows.free()
     if collect:
         for s in sources:
             b.init(s)
             st = b.run(policy, alpha, beta, learned=learned, record_levels=True)
+            if st.kernel_ms > 0:
+                per_trav.append(st.reached_edges / (st.kernel_ms / 1e3) / 1e9)
             for L in range(min(st.levels, sg.MAX_LEVELS_RECORDED)):
                 ls = st.level[L]
                 # levels run inside a burst are not level_kernel launches

@@ -322,6 +322,20 @@ void sg_bfs_options_default(sg_bfs_options* opt);
  * hop counts, unreached = n+1 (bfs.hpp:62-67).  stats may be NULL. */
 int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt, int32_t* depth, sg_bfs_stats* stats);
 
+typedef struct sg_bfs_many_stats {
+  int64_t reached_vertices; /* summed over the sources */
+  int64_t reached_edges;    /* TEPS numerator, summed over the sources */
+  float kernel_ms;          /* device time of all the traversals (hipEvents) */
+} sg_bfs_many_stats;
+/* sg_bfs from each of nsrc sources (host int32[nsrc]) in turn: row i of
+ * depth (device int32[nsrc][n]) receives the hop counts from sources[i].  No
+ * counterpart in the reference, whose callers run BFS::run once per source
+ * (examples/bfs/bfs.cpp:102-124, scripts/benchmark.sh); the traversals are
+ * sg_bfs's, enqueued back to back without a host synchronisation between
+ * them.  options.record_levels is ignored; stats may be NULL. */
+int sg_bfs_many(sg_graph_t g, int32_t nsrc, const int32_t* sources, const sg_bfs_options* opt, int32_t* depth,
+                sg_bfs_many_stats* stats);
+
 /* ---- learned per-level policy (no counterpart in the reference, whose only
  * policy is the alpha/beta rule of algorithms/bfs.hpp:325-350) -------------
  * An 8 -> 32 -> 32 -> 6 ReLU MLP in FP32 (1,542 parameters, layout W1[32][8]

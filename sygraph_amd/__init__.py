@@ -102,6 +102,10 @@ class BfsStats(C.Structure):
                 ("level", LevelStat * MAX_LEVELS_RECORDED)]
 
 
+class BfsManyStats(C.Structure):
+    _fields_ = [("reached_vertices", C.c_int64), ("reached_edges", C.c_int64), ("kernel_ms", C.c_float)]
+
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -169,6 +173,7 @@ _SIGS = {
     "sg_frontier_swap": (_I, [_P, _P]),
     "sg_bfs_options_default": (None, [C.POINTER(BfsOptions)]),
     "sg_bfs": (_I, [_P, C.c_int32, C.POINTER(BfsOptions), _P, C.POINTER(BfsStats)]),
+    "sg_bfs_many": (_I, [_P, C.c_int32, _P, C.POINTER(BfsOptions), _P, C.POINTER(BfsManyStats)]),
     "sg_sssp": (_I, [_P, C.c_int32, C.c_int32, _P, C.POINTER(SsspStats)]),
     "sg_graph_prepare": (_I, [_P, C.POINTER(C.c_float)]),
     "sg_cc": (_I, [_P, _P, C.POINTER(C.c_int32)]),
@@ -729,6 +734,22 @@ def msbfs_options(policy: int = MSBFS_AUTO, alpha: float = 15.0, beta: float = 2
     o.beta = beta
     o.learned = learned.h if learned is not None else None
     return o
+
+
+def bfs_many(g: Graph, sources, policy: int = POLICY_PUSH, alpha: float = 15.0, beta: float = 24.0,
+             out: Optional["DeviceBuffer"] = None, learned: Optional["Policy"] = None,
+             tuning: Optional[dict] = None) -> BfsManyStats:
+    """Single-source BFS from each source in turn, back to back on the device
+    (sg_bfs_many): row i of out (int32[len(sources)][n], allocated when None
+    and kept in the returned stats' .depth) holds the depths from sources[i]."""
+    s = np.ascontiguousarray(np.asarray(sources, dtype=np.int32))
+    buf = out if out is not None else DeviceBuffer(g.dev, 4 * max(len(s) * g.n, 1), dtype=np.int32)
+    o = bfs_options(policy, LB_BUCKETING, alpha, beta, learned=learned, tuning=tuning)
+    st = BfsManyStats()
+    check(lib().sg_bfs_many(g.h, len(s), s.ctypes.data_as(C.c_void_p), C.byref(o), C.c_void_p(buf.ptr),
+                            C.byref(st)))
+    st.depth = buf
+    return st
 
 
 def bfs_multi(g: Graph, sources, on_device: bool = False, policy: int = MSBFS_AUTO,

@@ -1436,7 +1436,8 @@ void bfs_workspace_release(sg_graph_t g) {
  * frontiers are small.  Steps enqueued beyond the last level find `done` set
  * and return at once. */
 template<typename OffT>
-int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats) {
+int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth, sg_bfs_stats* stats,
+            bool sync_end = true, int64_t* reached = nullptr) {
   sg_device_t dev = g->dev;
   hipStream_t s = dev->stream;
   BfsWorkspace* ws = nullptr;
@@ -1655,6 +1656,13 @@ int run_bfs(sg_graph_t g, int32_t src, const sg_bfs_options& opt, int32_t* depth
     small_push = qualifies ? small_push + 1 : 0;
     SG_TRY(enqueue(step + 1 + lookahead, small_push >= 2));
   }
+  if (reached) { /* reached vertices and their arcs */
+    reached[0] = last.reached_v;
+    reached[1] = last.reached_e;
+  }
+  /* a traversal of a batch leaves the stream running: the next one's start
+   * is ordered after this one's trailing steps (sg_bfs_many) */
+  if (!sync_end) return SG_OK;
   if (want_total) SG_HIP(hipEventRecord(ws->ev_end, s));
   SG_HIP(hipStreamSynchronize(s));
   if (stats) {
@@ -1681,11 +1689,11 @@ void sg_bfs_options_default(sg_bfs_options* opt) {
   opt->beta = 24.0f;
 }
 
-int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* depth, sg_bfs_stats* stats) {
-  SG_CHECK(g && depth, SG_ERR_INVALID, "sg_bfs: null argument");
-  SG_CHECK(source >= 0 && source < g->n, SG_ERR_INVALID, "sg_bfs: source %d out of range [0,%lld)", source,
-           (long long)g->n);
-  sg_bfs_options opt;
+} // extern "C"
+
+namespace {
+/* The options sg_bfs and sg_bfs_many accept (opt_in null: the defaults). */
+int bfs_options_checked(const sg_bfs_options* opt_in, sg_bfs_options& opt) {
   if (opt_in)
     opt = *opt_in;
   else
@@ -1710,6 +1718,18 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
            opt.grid_burst_arcs, (long long)kGbMaxArcs, opt.grid_burst_workgroups, kGbMaxWgs);
   SG_CHECK(opt.policy != SG_POLICY_HYBRID || (opt.alpha > 0.f && opt.beta > 0.f), SG_ERR_INVALID,
            "sg_bfs: alpha and beta must be positive");
+  return SG_OK;
+}
+} // namespace
+
+extern "C" {
+
+int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* depth, sg_bfs_stats* stats) {
+  SG_CHECK(g && depth, SG_ERR_INVALID, "sg_bfs: null argument");
+  SG_CHECK(source >= 0 && source < g->n, SG_ERR_INVALID, "sg_bfs: source %d out of range [0,%lld)", source,
+           (long long)g->n);
+  sg_bfs_options opt;
+  SG_TRY(bfs_options_checked(opt_in, opt));
   if (stats) {
     stats->levels = stats->n_push = stats->n_pull = 0;
     stats->reached_vertices = stats->reached_edges = 0;
@@ -1718,6 +1738,52 @@ int sg_bfs(sg_graph_t g, int32_t source, const sg_bfs_options* opt_in, int32_t* 
   SG_HIP(hipSetDevice(g->dev->ordinal));
   if (g->offset_bytes == 8) return run_bfs<int64_t>(g, source, opt, depth, stats);
   return run_bfs<uint32_t>(g, source, opt, depth, stats);
+}
+
+/* Single-source BFS from each of nsrc sources, one after another on the
+ * device: the next source's start is enqueued as soon as the host sees the
+ * previous traversal's last level, with no synchronisation in between. */
+int sg_bfs_many(sg_graph_t g, int32_t nsrc, const int32_t* sources, const sg_bfs_options* opt_in, int32_t* depth,
+                sg_bfs_many_stats* stats) {
+  SG_CHECK(g && depth && (nsrc == 0 || sources), SG_ERR_INVALID, "sg_bfs_many: null argument");
+  SG_CHECK(nsrc >= 0, SG_ERR_INVALID, "sg_bfs_many: negative source count");
+  for (int32_t i = 0; i < nsrc; ++i)
+    SG_CHECK(sources[i] >= 0 && sources[i] < g->n, SG_ERR_INVALID, "sg_bfs_many: source %d out of range [0,%lld)",
+             sources[i], (long long)g->n);
+  sg_bfs_options opt;
+  SG_TRY(bfs_options_checked(opt_in, opt));
+  opt.record_levels = 0;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  SG_HIP(hipSetDevice(g->dev->ordinal));
+  hipStream_t s = g->dev->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (stats) {
+    SG_HIP(hipEventCreate(&e0));
+    SG_HIP(hipEventCreate(&e1));
+    SG_HIP(hipEventRecord(e0, s));
+  }
+  int rc = SG_OK;
+  for (int32_t i = 0; i < nsrc && rc == SG_OK; ++i) {
+    int64_t reached[2] = {0, 0};
+    int32_t* row = depth + (int64_t)i * g->n;
+    rc = g->offset_bytes == 8 ? run_bfs<int64_t>(g, sources[i], opt, row, nullptr, false, reached)
+                              : run_bfs<uint32_t>(g, sources[i], opt, row, nullptr, false, reached);
+    if (stats) {
+      stats->reached_vertices += reached[0];
+      stats->reached_edges += reached[1];
+    }
+  }
+  if (stats) (void)hipEventRecord(e1, s);
+  const hipError_t se = hipStreamSynchronize(s);
+  if (stats) {
+    float ms = 0.f;
+    if (rc == SG_OK && se == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) stats->kernel_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  if (rc != SG_OK) return rc;
+  SG_HIP(se);
+  return SG_OK;
 }
 
 } // extern "C"

@@ -348,3 +348,32 @@ def test_bursts_with_fine_degree_classes(sg, device, oracle_mod):
             st = b.run(sg.POLICY_HYBRID, 15.0, 24.0, tuning=tuning)
             assert np.array_equal(b.get_distances(), ref), (src, tuning)
             assert st.levels == levels
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_bfs_many_rows_match_oracle(sg, device, oracle_mod, policy):
+    """sg_bfs_many: the sources' traversals back to back without a host
+    synchronisation between them; row i equals the oracle's depths from
+    sources[i], the totals are the sum of the rows' reached vertices and
+    their arcs, and a bad source is refused before anything runs."""
+    for g in (oracle_mod.rmat(12, 16, seed=41), oracle_mod.grid(60, 200, 0.1, seed=42)):
+        G = sg.Graph.from_csr(device, g.offsets, g.cols)
+        deg = g.degrees()
+        srcs = [int(x) for x in np.nonzero(deg > 0)[0][[0, 5, -1]]] + [int(np.argmax(deg))]
+        pol = {"push": sg.POLICY_PUSH, "pull": sg.POLICY_PULL, "hybrid": sg.POLICY_HYBRID}[policy]
+        st = sg.bfs_many(G, srcs, policy=pol)
+        rows = st.depth.download(np.int32, len(srcs) * G.n).reshape(len(srcs), G.n)
+        rv = re = 0
+        for i, s in enumerate(srcs):
+            ref, _ = oracle_mod.bfs(g, s)
+            assert np.array_equal(rows[i], ref), (i, s)
+            hit = ref != G.n + 1
+            rv += int(hit.sum())
+            re += int(deg[hit].sum())
+        assert (st.reached_vertices, st.reached_edges) == (rv, re)
+        assert st.kernel_ms > 0
+        empty = sg.bfs_many(G, [], policy=pol)
+        assert empty.reached_vertices == 0
+        with pytest.raises(sg.SygraphError):
+            sg.bfs_many(G, [0, G.n], policy=pol)
+        G.destroy()
