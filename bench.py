@@ -56,14 +56,11 @@ def run_policy(sg, G, sources, policy, alpha, beta, steps, warmup, collect, lear
     w0_bytes = ((G.n + 63) // 64) * 8
     kernel_ms_total = 0.0
     per_trav = []  # device-time GTEPS of each timed traversal
-    for _ in range(warmup):
-        for s in sources:
-            b.init(s)
-            b.run(policy, alpha, beta, learned=learned, record_levels=False)
     # a step: one traversal per source, back to back on the device (sg_bfs_many);
     # the rows stay on the device
     rows = G.dev.alloc(4 * len(sources) * G.n)
-    sg.bfs_many(G, sources, policy, alpha, beta, out=rows, learned=learned)
+    for _ in range(warmup):
+        sg.bfs_many(G, sources, policy, alpha, beta, out=rows, learned=learned)
     G.dev.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
