@@ -524,7 +524,8 @@ void or_bc(int64_t n, const int64_t* offsets, const int32_t* cols, const int64_t
 /* Bitmap probes of one pull level (the byte model of the pull kernel).  Each
  * unvisited vertex u with in-arcs first probes hint[u] (when hint != NULL);
  * on a miss it scans its in-list in CSR order up to the first in-neighbour in
- * the frontier (depth == level), k_u probes, or the whole list. */
+ * the frontier (depth == level), k_u probes, or the whole list -- unless the
+ * list is the hint alone (one in-arc), which the miss already settled. */
 int64_t or_pull_scan_count(int64_t n, const int64_t* offsets, const int32_t* cols, const int32_t* depth, int32_t level,
                            const int32_t* hint, int64_t* n_scanned, int64_t* n_missed) {
   int64_t total = 0, cnt = 0, missed = 0;
@@ -538,6 +539,7 @@ int64_t or_pull_scan_count(int64_t n, const int64_t* offsets, const int32_t* col
       total += 1;
       if (depth[hint[u]] == level) continue;
       missed += 1;
+      if (e - b == 1) continue;
     }
     int64_t k = e - b;
     for (int64_t j = b; j < e; ++j)

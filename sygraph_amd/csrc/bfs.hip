@@ -201,6 +201,12 @@ __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, 
           }
           j = b;
           kk = (int64_t)(e - b);
+          /* a single in-arc is the hint itself, which missed: no parent and
+           * no list to read */
+          if (e - b == 1) {
+            j = e;
+            kk = 0;
+          }
           /* lane-serial head: four independent id loads then four
            * independent bitmap probes per step */
           for (int step = 0; step < head_steps && j < e; ++step, j += 4) {

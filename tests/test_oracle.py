@@ -305,9 +305,12 @@ def test_pull_hint_brute_force(oracle_mod):
 
 
 def test_pull_scan_count_brute_force(oracle_mod):
+    """With hints, a vertex whose hint misses scans its in-list, except a
+    single in-arc: the list is the hint, already probed."""
     g = oracle_mod.rmat(8, 8, seed=3)
     d, lv = oracle_mod.bfs(g, int(np.argmax(g.degrees())))
     hint = oracle_mod.pull_hint(g)
+    single_misses = 0
     for level in range(lv):
         for use_hint in (False, True):
             tot, cnt, miss = oracle_mod.pull_scan_count(g, d, level, hint if use_hint else None)
@@ -324,9 +327,13 @@ def test_pull_scan_count_brute_force(oracle_mod):
                     if d[hint[u]] == level:
                         continue
                     m2 += 1
+                    if len(nb) == 1:
+                        single_misses += 1
+                        continue
                 hit = np.nonzero(d[nb] == level)[0]
                 t2 += (hit[0] + 1) if len(hit) else len(nb)
             assert (tot, cnt, miss) == (t2, c2, m2)
+    assert single_misses > 0
 
 
 # ----------------------------------------------------- committed golden vectors
