@@ -380,9 +380,11 @@ int sg_sssp(sg_graph_t g, int32_t source, int32_t load_balancer, void* dist, sg_
 
 typedef struct sg_sssp_options {
   int32_t load_balancer; /* sg_load_balancer (the bucketing bins in every case) */
-  double delta;          /* bucket width of the near/far split: 0 = automatic (none below 2^25 arcs,
-                            where an iteration's launches cost more than the relaxations buckets
-                            save; half the mean arc weight from there on), negative or infinite =
+  double delta;          /* bucket width of the near/far split: 0 = automatic (half the mean arc
+                            weight on a graph of at least 2^25 arcs and 8 arcs per vertex; none
+                            otherwise: on smaller graphs an iteration's launches cost more than the
+                            relaxations buckets save, on sparse long-diameter ones such as grids
+                            buckets multiply the iterations), negative or infinite =
                             none, i.e. every dirty vertex is relaxed each round (the reference's
                             frontier Bellman-Ford, sssp.hpp:152-210) */
   int32_t heavy_pull;    /* direction of a bucket's heavy round on an undirected graph: 0 = automatic

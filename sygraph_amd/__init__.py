@@ -614,8 +614,9 @@ class SSSP:
 
     def run(self, load_balancer: int = LB_BUCKETING, delta: float = 0.0, heavy_pull: int = 0,
             burst_arcs: int = 0, record_relax: bool = False) -> SsspStats:
-        """delta: bucket width of the near/far split (0 automatic: none below
-        2^25 arcs, half the mean arc weight above; negative or inf for the
+        """delta: bucket width of the near/far split (0 automatic: half the
+        mean arc weight on a graph of at least 2^25 arcs and 8 arcs per
+        vertex, none otherwise; negative or inf for the
         reference's frontier Bellman-Ford).  heavy_pull: direction of a
         bucket's heavy round (0 automatic, 1 pull, -1 push).  burst_arcs:
         small rounds of the reference's schedule run inside one workgroup (0

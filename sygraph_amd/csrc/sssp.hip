@@ -980,18 +980,24 @@ int mean_weight(sg_graph_t g, double* out) {
   return SG_OK;
 }
 
-/* The automatic schedule: below kAutoBucketArcs arcs an iteration's fixed
- * cost (its launches) outweighs the relaxations that buckets save, so the
- * reference's frontier Bellman-Ford runs; from there on, buckets of half the
- * mean arc weight.  The threshold is the best single arc count for the total
- * time of 100 generated graphs (RMAT scale 16-22, grids 256-1024, ER,
- * weights U[1,255]; within 0.2 % of choosing per graph). */
+/* The automatic schedule: buckets of half the mean arc weight on a graph of
+ * at least kAutoBucketArcs arcs and kAutoBucketDegree arcs per vertex, the
+ * reference's frontier Bellman-Ford otherwise.  Below the arc count an
+ * iteration's fixed cost (its launches) outweighs the relaxations that buckets
+ * save; on sparse, long-diameter graphs (grids: 3-4 arcs per vertex) buckets
+ * multiply the iterations, each near that fixed cost (3-4x slower there).  The
+ * pair has the least geometric mean of time / the faster schedule's time over
+ * 100 generated graphs (RMAT scale 16-22, grids 512^2-4096^2, ER; weights
+ * U[1,255]; four sources each) among arc counts 2^18..2^28 and degrees 0..32:
+ * 3.4 % over choosing per run (18 % for the reference's schedule everywhere),
+ * total time 1.8 % under the reference's schedule everywhere. */
 constexpr int64_t kAutoBucketArcs = int64_t(1) << 25;
+constexpr int64_t kAutoBucketDegree = 8;
 
 template<typename OffT, typename W>
 int run_sssp_auto(sg_graph_t g, int32_t source, double delta, int32_t heavy_pull, int64_t burst_arcs,
                   bool record_relax, W* dist, sg_sssp_stats* stats) {
-  if (delta == 0.0 && g->m < kAutoBucketArcs) {
+  if (delta == 0.0 && (g->m < kAutoBucketArcs || g->m < kAutoBucketDegree * (int64_t)g->n)) {
     delta = -1.0;
   } else if (delta == 0.0) {
     double mw = 1.0;

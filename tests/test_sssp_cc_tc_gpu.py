@@ -117,6 +117,36 @@ def test_sssp_buckets_relax_fewer_arcs(sg, device, oracle_mod):
     assert abs(auto - bf) < 0.05 * bf and abs(auto - bf) < abs(auto - buckets), (auto, bf, buckets)
 
 
+@pytest.mark.parametrize("kind", ["grid", "rmat"])
+def test_sssp_automatic_schedule_rule(sg, device, kind):
+    """The automatic schedule takes buckets only on a graph of at least 2^25
+    arcs and 8 arcs per vertex: RMAT scale 20 edge factor 32 (about 60 M
+    arcs, ~57 per vertex) relaxes fewer arcs than the reference's schedule;
+    a 3072 x 3072 grid with 5 % of its edges deleted (about 36 M arcs, 3.8 per
+    vertex) relaxes as many.  Distances equal the reference schedule's."""
+    if kind == "grid":
+        G = sg.Graph.grid(device, 3072, 3072, 0.05, seed=61, weight_seed=62)
+    else:
+        G = sg.Graph.rmat(device, 20, 32, seed=63, weight_seed=64)
+    assert G.m >= 1 << 25
+    buckets_expected = G.m >= 8 * G.n
+    assert buckets_expected == (kind == "rmat")
+    off, _, _ = G.download()
+    src = int(np.argmax(np.diff(off)))
+    s = sg.SSSP(G)
+    s.init(src)
+    bf = s.run(delta=-1.0).relaxations
+    ref = s.dist.digest(G.n, 4)
+    s.init(src)
+    auto = s.run().relaxations
+    assert s.dist.digest(G.n, 4) == ref
+    if buckets_expected:
+        assert auto < 0.9 * bf, (auto, bf)
+    else:  # a round's atomics race with each other: counts vary a little
+        assert abs(auto - bf) < 0.05 * bf, (auto, bf)
+    G.destroy()
+
+
 @pytest.mark.parametrize("delta", [0.0, 0.3, -1.0])
 def test_sssp_float_weights_buckets(sg, device, oracle_mod, delta):
     g = oracle_mod.rmat(11, 8, seed=46)
