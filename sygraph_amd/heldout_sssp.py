@@ -28,6 +28,14 @@ the graph has at least M arcs and at least D arcs per vertex" with the least
 geometric mean over the runs of the rule's time / the faster schedule's time
 (every graph weighs the same; the total time is dominated by the largest grids
 and barely tells arc thresholds apart; ties: the larger M, then the larger D).
+
+    python -m sygraph_amd.heldout_sssp --fit-delta [--out sygraph_amd/data/sssp_delta_fit.json]
+
+fits the bucket width the automatic schedule uses, on the training graphs that
+the rule sends to buckets: per run it times widths of f x 128 (128 = the mean
+of weights U[1,255]) for each f in FIT_FACTORS, heavy rounds automatic, and
+`fit_factor` picks the f with the least geometric mean of time / the run's
+best time (ties: the larger f).
 """
 from __future__ import annotations
 
@@ -50,16 +58,23 @@ MODES = [("plain", dict(delta=-1.0, heavy_pull=-1)), ("buckets", dict(delta=64.0
 FIT_MODES = [("plain", dict(delta=-1.0, heavy_pull=-1)), ("bucket_pull", dict(delta=64.0, heavy_pull=0))]
 FIT_ARCS = [0] + [1 << k for k in range(18, 29)]
 FIT_DEGREE = [0, 2, 4, 6, 8, 10, 12, 16, 24, 32]
+FIT_FACTORS = [0.125, 0.25, 0.375, 0.5, 0.75, 1.0]
+MEAN_WEIGHT = 128.0  # of weights U[1,255]
+RULE_ARCS, RULE_DEGREE = 1 << 25, 8  # the automatic schedule's rule (sssp.hip)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def evaluate(dev, seeds, quick, nsrc=4, reps=3, modes=MODES):
+def evaluate(dev, seeds, quick, nsrc=4, reps=3, modes=MODES, keep=None):
+    """keep(n, m): whether to time a graph (default: all)."""
     rows = []
     for seed in seeds:
         name, G = make_graph(dev, seed, quick, weight_seed=seed)
+        if keep is not None and not keep(G.n, G.m):
+            G.destroy()
+            continue
         s = sg.SSSP(G)
         for src in sources_of(G, seed, nsrc):
             t, digests = {}, set()
@@ -98,6 +113,22 @@ def fit(rows, arcs=FIT_ARCS, degree=FIT_DEGREE):
             "per_run_best_total_ms": sum(min(r["plain"], r["bucket_pull"]) for r in rows), "grid": grid}
 
 
+def factor_key(f: float) -> str:
+    return f"f{f:g}"
+
+
+def fit_factor(rows, factors=FIT_FACTORS):
+    """The width factor with the least geometric mean of time / the run's best
+    time over `rows` (one time per factor per run, keys factor_key(f)), and
+    every factor's mean and total."""
+    keys = [factor_key(f) for f in factors]
+    best_of = [min(r[k] for k in keys) for r in rows]
+    cand = [{"factor": f, "geomean_vs_best": float(np.exp(np.mean([np.log(r[k] / b) for r, b in zip(rows, best_of)]))),
+             "total_ms": float(sum(r[k] for r in rows))} for f, k in zip(factors, keys)]
+    best = min(cand, key=lambda c: (round(c["geomean_vs_best"], 6), -c["factor"]))
+    return {"best": best, "candidates": cand}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "data", "sssp_heldout.json"))
@@ -106,14 +137,28 @@ def main():
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--fit", action="store_true", help="fit the automatic schedule's rule (training graphs)")
+    ap.add_argument("--fit-delta", action="store_true", help="fit the automatic bucket width (training graphs)")
     a = ap.parse_args()
-    if a.fit and a.seeds == "300-349":
+    if (a.fit or a.fit_delta) and a.seeds == "300-349":
         a.seeds = "100-199"
     if a.fit and a.out == ap.get_default("out"):
         a.out = os.path.join(HERE, "data", "sssp_rule_fit.json")
+    if a.fit_delta and a.out == ap.get_default("out"):
+        a.out = os.path.join(HERE, "data", "sssp_delta_fit.json")
     dev = sg.Device(0)
     lo, hi = (int(x) for x in a.seeds.split("-"))
     seeds = list(range(lo, hi + 1)) if not a.quick else list(range(lo, min(hi, lo + 5) + 1))
+    if a.fit_delta:
+        modes = [(factor_key(f), dict(delta=f * MEAN_WEIGHT, heavy_pull=0)) for f in FIT_FACTORS]
+        rows = evaluate(dev, seeds, a.quick, a.sources, a.reps, modes=modes,
+                        keep=lambda n, m: m >= RULE_ARCS and m >= RULE_DEGREE * n)
+        res = fit_factor(rows)
+        log(json.dumps(res, indent=1))
+        with open(a.out, "w") as f:
+            json.dump({"seeds": [seeds[0], seeds[-1]], "quick": a.quick,
+                       "weights": "u32 U[1,255], weight seed = graph seed", **res, "runs": rows}, f, indent=1)
+        print(json.dumps(res["best"]))
+        return
     if a.fit:
         rows = evaluate(dev, seeds, a.quick, a.sources, a.reps, modes=FIT_MODES)
         res = fit(rows)

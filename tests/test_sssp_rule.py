@@ -1,7 +1,8 @@
-"""CPU pins of the SSSP automatic schedule's rule: the fit of
-sygraph_amd/heldout_sssp.py on hand-worked cases, the committed fit record
-(sygraph_amd/data/sssp_rule_fit.json) against the constants sssp.hip uses,
-and the held-out comparison record (sygraph_amd/data/sssp_heldout.json)."""
+"""CPU pins of the SSSP automatic schedule: the fits of
+sygraph_amd/heldout_sssp.py (the rule, the bucket width) on hand-worked cases,
+the committed fit records (sygraph_amd/data/sssp_rule_fit.json,
+sssp_delta_fit.json) against the constants sssp.hip uses, and the held-out
+comparison record (sygraph_amd/data/sssp_heldout.json)."""
 import json
 import math
 import os
@@ -82,3 +83,34 @@ def test_shipped_held_out_record():
     assert rec["summary"]["adaptive_speedup_vs_plain_geomean"] >= 1.0
     geo = math.exp(sum(math.log(r["plain"] / r["adaptive"]) for r in rec["runs"]) / len(rec["runs"]))
     assert geo == pytest.approx(rec["summary"]["adaptive_speedup_vs_plain_geomean"])
+
+
+def test_fit_factor_hand_worked_case():
+    rows = [{"f0.25": 2.0, "f0.5": 3.0}, {"f0.25": 5.0, "f0.5": 4.0}]
+    r = heldout_sssp.fit_factor(rows, [0.25, 0.5])
+    assert r["best"]["factor"] == 0.25
+    by = {c["factor"]: c for c in r["candidates"]}
+    assert by[0.25]["geomean_vs_best"] == pytest.approx(math.sqrt(1.25))
+    assert by[0.5]["geomean_vs_best"] == pytest.approx(math.sqrt(1.5))
+    assert by[0.5]["total_ms"] == pytest.approx(7.0)
+
+
+def test_shipped_bucket_width_is_the_recorded_fit():
+    """The committed record of `python -m sygraph_amd.heldout_sssp
+    --fit-delta` (the training graphs the rule sends to buckets, four sources
+    each, widths f x the mean weight) refits to the factor it records, and
+    sssp.hip's automatic width uses that factor."""
+    with open(os.path.join(DATA, "sssp_delta_fit.json")) as f:
+        rec = json.load(f)
+    assert tuple(rec["seeds"]) == (100, 199) and not rec["quick"]
+    assert len(rec["runs"]) >= 40
+    assert all(r["m"] >= heldout_sssp.RULE_ARCS and r["m"] >= heldout_sssp.RULE_DEGREE * r["n"] for r in rec["runs"])
+    best = heldout_sssp.fit_factor(rec["runs"])["best"]
+    assert best["factor"] == rec["best"]["factor"]
+    with open(os.path.join(ROOT, "sygraph_amd", "csrc", "sssp.hip")) as f:
+        src = f.read()
+    factor = float(re.search(r"kAutoDeltaFactor = ([0-9.]+);", src).group(1))
+    assert factor == best["factor"]
+    arcs = int(re.search(r"kAutoBucketArcs = int64_t\(1\) << (\d+);", src).group(1))
+    degree = int(re.search(r"kAutoBucketDegree = (\d+);", src).group(1))
+    assert (1 << arcs, degree) == (heldout_sssp.RULE_ARCS, heldout_sssp.RULE_DEGREE)
