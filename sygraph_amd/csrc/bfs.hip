@@ -95,7 +95,7 @@ __device__ __forceinline__ void push_level(const BfsArgs<OffT>& a, int variant, 
  * hits settles its vertex; the misses go to a per-wave list in LDS and are
  * pulled one per lane from their adjacency lists.  New frontier bits are
  * gathered per word in LDS. */
-constexpr int kPullBatch = 4;
+constexpr int kPullBatch = 8;
 
 template<typename OffT>
 __device__ __forceinline__ void pull_level(const BfsArgs<OffT>& a, int variant, int32_t level) {
@@ -420,9 +420,12 @@ __global__ void bfs_start_kernel(BfsArgs<OffT> a, uint64_t* prev, const uint64_t
   *ctl = c;
 }
 
-/* The traversal of one level, push or pull as the control block says.  Six
- * waves per SIMD: the pull's batched probes need 80 registers. */
-constexpr int kLevelWavesPerSimd = 6;
+/* The traversal of one level, push or pull as the control block says.  Four
+ * waves per SIMD, what four workgroups of four waves per CU place: the pull's
+ * eight hint chains per lane take 106 registers (four chains: 80 registers,
+ * 3 % slower; sixteen: 162 registers and three workgroups per CU, 11 %
+ * slower). */
+constexpr int kLevelWavesPerSimd = 4;
 template<typename OffT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kLevelWavesPerSimd))) level_kernel(BfsArgs<OffT> a, const BfsCtl* ctl) {
   if (ctl->done) return;
